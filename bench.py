@@ -1,0 +1,153 @@
+"""Benchmark of the CDKLM16 step on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
+
+Headline workload (printed as ONE JSON line): plain CDKLM16, 2048 x 2048 cells,
+dx=dy=500 m, random depth, periodic E/W + walls N/S, 20 balanced eddies,
+uniform wind, fp32, fixed dt from CFL 0.7; a "step" is one full SSP-RK2 step
+(two substep launches) over the whole domain with the state resident in HBM.
+value = Mcell-steps/s = nx*ny*K / wall_s / 1e6 (the reference's Mcells/s,
+gpu_ocean/papers/testCasesDemos/run_benchmark.py:278-288).
+
+roofline: HBM-bound.  Algorithmic bytes are 68 B per cell-step (rk0 reads
+Q^n 12 B + H 4 B and writes Q* 12 B; rk1 reads Q* 12 + Q^n 12 + H 4 B and
+writes 12 B), i.e. 34 B per cell per substep launch; achieved = 34 B x cells
+/ average substep-launch duration (HIP events on the launch stream).
+cpu_baseline: the NumPy oracle (single core) on a bounded sample of the same
+workload (one RK2 step of the 2048^2 domain).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "gpu-ocean_amd"))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_STEP = 68.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_sim(inp, ctx, tile_config=None, dt=None):
+    from SWESimulators import CDKLM16, Common, WindStress
+    bc = Common.BoundaryConditions(*inp.bc)
+    wind = WindStress.WindStress(t=[0.0], X=[np.array([[inp.wind_x]], np.float32)],
+                                 Y=[np.array([[inp.wind_y]], np.float32)])
+    sim = CDKLM16.CDKLM16(ctx, inp.eta, inp.hu, inp.hv, inp.Hi, inp.nx, inp.ny, inp.dx, inp.dy,
+                          1.0 if dt is None else dt, inp.g, inp.f, inp.r,
+                          boundary_conditions=bc, wind_stress=wind, tile_config=tile_config)
+    if dt is None:
+        # dt = courant * CFL minimum, computed once on the device and held fixed
+        s = sim.max_dt_device(courant_number=inp.courant)
+        sim.gpu_stream.synchronize()
+        sim.dt = float(s[1].item())
+    return sim
+
+
+def time_plain(sim, steps, warmup):
+    """Time ``steps`` fixed-dt RK2 steps; returns (seconds, launches)."""
+    import torch
+    sim.step_n(warmup)
+    st = sim.gpu_stream
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    sim.step_n(steps)
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev = e0.elapsed_time(e1) / 1000.0
+    return wall, ev, 2 * steps
+
+
+def cpu_baseline(inp, n_steps=1):
+    """NumPy oracle, one core, on ``n_steps`` RK2 steps of the same input."""
+    from oracle import cdklm16_np as O
+    p = O.Params(nx=inp.nx, ny=inp.ny, dx=inp.dx, dy=inp.dy, g=inp.g, r=inp.r, bc=inp.bc,
+                 f=inp.f, wind_x=(inp.wind_x, inp.wind_x), wind_y=(inp.wind_y, inp.wind_y))
+    sim = O.OracleSim(inp.eta, inp.hu, inp.hv, inp.Hi, p, dt=-1)
+    dt = np.float32(inp.courant * sim.dt / 0.8)
+    t0 = time.perf_counter()
+    sim.step_n(n_steps, dt)
+    s = time.perf_counter() - t0
+    return inp.nx * inp.ny * n_steps / s / 1e6, s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile", type=int, default=None)
+    ap.add_argument("--n", type=int, default=2048, help="domain side (headline: 2048)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the CDKLM16 path "
+              f"is built and measured on a single MI355X.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from SWESimulators import BathymetryAndICs as B, Common
+
+    ctx = Common.HIPContext(0)
+    log(f"[bench] device {torch.cuda.get_device_name(0)}")
+    inp = B.config2(nx=a.n, ny=a.n)
+    log(f"[bench] workload {inp.name}: {inp.nx}x{inp.ny}, steps {a.steps}, warmup {a.warmup}")
+    sim = make_sim(inp, ctx, tile_config=a.tile)
+    wall, ev, launches = time_plain(sim, a.steps, a.warmup)
+    cells = inp.nx * inp.ny
+    value = cells * a.steps / wall / 1e6
+    avg_launch = ev / launches
+    achieved = 0.5 * BYTES_PER_CELL_STEP * cells / avg_launch / 1e9
+    log(f"[bench] plain: {value:.1f} Mcell-steps/s, event time/launch {avg_launch*1e6:.1f} us")
+
+    cpu = None
+    if not a.no_cpu:
+        log("[bench] cpu baseline (oracle, 1 core, 1 RK2 step)")
+        v, s = cpu_baseline(inp, 1)
+        cpu = {"value": round(v, 3), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+               "sample": f"NumPy fp32 oracle, 1 RK2 step of the {inp.nx}x{inp.ny} workload "
+                         f"({s:.1f} s), {platform.processor() or platform.machine()}, "
+                         f"numpy {np.__version__}"}
+
+    out = {
+        "metric": "Mcell-steps/s CDKLM16 fwd & fwd+adjoint, batched vs plain; % of HBM peak",
+        "value": round(value, 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded random depth, balanced eddies; no datasets)",
+        "config": {"workload": f"CDKLM16 plain {inp.nx}x{inp.ny} periodic-EW walls-NS, "
+                               f"dx=dy=500 m, fixed dt (CFL 0.7), SSP-RK2",
+                   "nx": inp.nx, "ny": inp.ny, "cells": cells, "rk_order": 2,
+                   "tile_config": sim.tile_config},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+"""ctypes binding of ``libgocean_hip.so`` (declared in ``include/gocean.h``).
+
+The HIP library is the only compute path: if it is missing or does not load,
+every simulator call raises ``RuntimeError`` -- there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.abspath(os.path.join(_HERE, "..", "lib"))
+LIB_PATH = os.path.join(LIB_DIR, "libgocean_hip.so")
+
+c_float_p = ctypes.POINTER(ctypes.c_float)
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+
+
+class Domain(ctypes.Structure):
+    """gocean_domain_t"""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("pitch", ctypes.c_int32), ("hi_pitch", ctypes.c_int32),
+        ("dx", c_float), ("dy", c_float), ("g", c_float), ("r", c_float),
+        ("theta", c_float), ("desing_eps", c_float), ("flux_slope_eps", c_float),
+        ("depth_cutoff", c_float), ("land_value", c_float),
+        ("f", c_float), ("north_x", c_float), ("north_y", c_float),
+        ("bc_north", ctypes.c_int32), ("bc_east", ctypes.c_int32),
+        ("bc_south", ctypes.c_int32), ("bc_west", ctypes.c_int32),
+        ("rk_order", ctypes.c_int32),
+    ]
+
+
+class Wind(ctypes.Structure):
+    """gocean_wind_t"""
+    _fields_ = [("t", c_float), ("x0", c_float), ("x1", c_float),
+                ("y0", c_float), ("y1", c_float)]
+
+
+class Fields(ctypes.Structure):
+    """gocean_fields_t"""
+    _fields_ = [("f", c_void_p), ("north_x", c_void_p), ("north_y", c_void_p)]
+
+
+# (name, restype, argtypes) of every exported symbol declared in gocean.h
+PROTOTYPES = [
+    ("gocean_version", ctypes.c_char_p, []),
+    ("gocean_last_error", ctypes.c_char_p, []),
+    ("gocean_device_count", c_int, []),
+    ("gocean_tile_configs", c_int, [ctypes.POINTER(ctypes.c_char_p), c_int]),
+    ("gocean_bathymetry_init", c_int, [ctypes.POINTER(Domain), c_void_p, c_void_p, c_void_p]),
+    ("gocean_cdklm_substep", c_int,
+     [ctypes.POINTER(Domain), c_float, c_void_p, c_int,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_void_p,
+      ctypes.POINTER(Fields), ctypes.POINTER(Wind), c_int, c_void_p]),
+    ("gocean_rk_steps", c_int,
+     [ctypes.POINTER(Domain), c_int, c_float, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_void_p,
+      ctypes.POINTER(Fields), ctypes.POINTER(Wind), c_int, c_void_p]),
+    ("gocean_periodic_bc", c_int, [ctypes.POINTER(Domain), c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gocean_max_dt", c_int,
+     [ctypes.POINTER(Domain), c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
+]
+
+_lib = None
+_lock = threading.Lock()
+
+
+def library_exists() -> bool:
+    return os.path.isfile(LIB_PATH)
+
+
+def load(path: str = None):
+    """Load (once) and return the ctypes handle of the HIP library."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.isfile(p):
+            raise RuntimeError(
+                f"HIP library {p} is missing: build it with "
+                f"`python gpu-ocean_amd/build.py` (or __graft_entry__.build()). "
+                f"There is no CPU fallback.")
+        lib = ctypes.CDLL(p)
+        for name, res, args in PROTOTYPES:
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return _lib
+
+
+def check(rc: int, what: str = "gocean call"):
+    """Turn a non-zero return code into RuntimeError with the library's
+    message (the reference raises Python exceptions on launch failure)."""
+    if rc != 0:
+        msg = load().gocean_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (code {rc}): {msg}")
+
+
+def ptr(t) -> int:
+    """Device address of a torch tensor (or an int)."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+def stream_handle(stream) -> int:
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream

@@ -1,0 +1,594 @@
+// cdklm16_tile.hpp -- one CDKLM16 Runge-Kutta substep on one output tile.
+//
+// Device code for gfx950.  The arithmetic restates
+// gpu_ocean/SWESimulators/gpu_kernels/CDKLM16_kernel.cu:80-1006 and
+// common.cu:532-551,822-824 with the reference's operand order and rounding
+// points (the reference builds with --fmad=false; this file must be built with
+// -ffp-contract=off and IEEE division / square root), so that a float32 result
+// equals the reference's on the same inputs.
+//
+// The structure is this library's own:
+//   * one workgroup owns a BW x BH tile of output cells; the state tile with a
+//     2-cell halo is loaded once, with wall (mirror) and periodic (wrap)
+//     boundaries applied by index remapping at load time, so no ghost-cell
+//     kernels run between substeps;
+//   * (eta, u, v) conversion, x/y slopes (with the positivity adjustment) and
+//     every face flux are computed exactly once per tile and kept in LDS; the
+//     reference evaluates each face from both sides (4 flux evaluations per
+//     cell) and keeps only one slope set live at a time;
+//   * the cell depth Hm is rebuilt from the corner depths in LDS, so the
+//     substep reads 3 state fields + Hi and writes 3 fields;
+//   * work items of every phase are spread over all NT threads of the group.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gocean {
+
+constexpr float kDryFlag = 1.0e-30f;   // CDKLM16_kernel.cu:37
+constexpr float kDryEps = 1.0e-3f;     // CDKLM16_kernel.cu:38
+constexpr int kWall = 1;
+constexpr int kPeriodic = 2;
+
+// Everything one substep of one domain needs.  Plain launches pass it as a
+// kernel argument; the ensemble kernel builds it per workgroup from a table.
+struct SubstepArgs {
+    const float* eta_in; const float* hu_in; const float* hv_in;
+    const float* eta_qn; const float* hu_qn; const float* hv_qn;
+    float* eta_out; float* hu_out; float* hv_out;
+    const float* Hi;
+    const float* f_field; const float* nx_field; const float* ny_field;
+    const float* dt_dev;
+    int nx, ny, pitch, hi_pitch;
+    float dx, dy, g, r, theta, desing_eps, flux_eps, cutoff, land;
+    float f, north_x, north_y;
+    float wind_t, wx0, wx1, wy0, wy1;
+    float dt;
+    int bc_n, bc_e, bc_s, bc_w;
+    int rk_order, rk_step;
+    int tiles_x, tiles_y;
+};
+
+// ---------------------------------------------------------------------------
+// scalar helpers
+// ---------------------------------------------------------------------------
+
+// minmod of (b, c, f): 0.25*s(b)*(s(b)+s(c))*(s(c)+s(f))*min(|b|,|c|,|f|)
+// (common.cu:544-551).  The sign product is exactly +1, -1 or 0, so the value
+// is min(|b|,|c|,|f|) with b's sign when all three sign bits agree, else zero.
+__device__ __forceinline__ float minmod_raw(float b, float c, float f) {
+    const float m = fminf(fminf(fabsf(b), fabsf(c)), fabsf(f));
+    const uint32_t ub = __float_as_uint(b), uc = __float_as_uint(c), uf = __float_as_uint(f);
+    const bool agree = (((ub ^ uc) | (uc ^ uf)) & 0x80000000u) == 0u;
+    return agree ? copysignf(m, b) : 0.0f;
+}
+
+// common.cu:532-542
+__device__ __forceinline__ float minmod_slope(float l, float c, float r, float theta) {
+    return minmod_raw((c - l) * theta, (r - l) * 0.5f, (r - c) * theta);
+}
+
+// common.cu:822-824.  The reference's "0.5*eps" is a double literal, so that
+// one addition is evaluated in double there; this does the same.
+__device__ __forceinline__ float desingularize(float h, float hu, float eps) {
+    const float a = (h * h) / (2.0f * eps);
+    const float b = (float)((double)a + 0.5 * (double)eps);
+    return hu / fmaxf(fminf(b, eps), fabsf(h));
+}
+
+struct Consts {
+    float g, half_g, two_g, theta, theta_g;
+    float dx, dy, vdx, udy;        // vdx = DX/(2g), udy = DY/(2g)
+    float desing, flux_eps, cutoff, land;
+};
+
+// Central-upwind flux of primitive face states Qm=(hm,um,vm), Qp=(hp,up,vp)
+// (CDKLM16_F_func + CDKLM16_flux, CDKLM16_kernel.cu:80-140).
+__device__ __forceinline__ void cu_flux(float hm, float um, float vm,
+                                        float hp, float up, float vp,
+                                        const Consts& K,
+                                        float& Fx, float& Fy, float& Fz) {
+    float Fpx = 0.0f, Fpy = 0.0f, Fpz = 0.0f, u_p = 0.0f, c_p = 0.0f;
+    if (hp > K.cutoff) {
+        const float q = hp * up;
+        Fpx = q;
+        Fpy = q * up + (K.half_g * hp) * hp;
+        Fpz = q * vp;
+        u_p = up;
+        c_p = sqrtf(K.g * hp);
+    }
+    float Fmx = 0.0f, Fmy = 0.0f, Fmz = 0.0f, u_m = 0.0f, c_m = 0.0f;
+    if (hm > K.cutoff) {
+        const float q = hm * um;
+        Fmx = q;
+        Fmy = q * um + (K.half_g * hm) * hm;
+        Fmz = q * vm;
+        u_m = um;
+        c_m = sqrtf(K.g * hm);
+    }
+    const float am = fminf(fminf(u_m - c_m, u_p - c_p), 0.0f);
+    const float ap = fmaxf(fmaxf(u_m + c_m, u_p + c_p), 0.0f);
+    const float den = ap - am;
+    if (fabsf(den) < K.flux_eps) {
+        Fx = 0.0f; Fy = 0.0f; Fz = 0.0f;
+        return;
+    }
+    const float apam = ap * am;
+    Fx = ((ap * Fmx - am * Fpx) + apam * (hp - hm)) / den;
+    Fy = ((ap * Fmy - am * Fpy) + apam * (Fpx - Fmx)) / den;
+    Fz = (um + up > 0.0f) ? Fmz : Fpz;
+}
+
+// ---------------------------------------------------------------------------
+// LDS layout of one tile
+// ---------------------------------------------------------------------------
+template <int BW, int BH, bool FIELD>
+struct TileSmem {
+    float Hi[BH + 5][BW + 5];          // corner depths, land flagged
+    float E[BH + 4][BW + 4];           // eta (dry / land adjusted)
+    float U[BH + 4][BW + 4];           // u
+    float V[BH + 4][BW + 4];           // v
+    float Ux[BH][BW + 2], Vx[BH][BW + 2], Kx[BH][BW + 2];
+    float Uy[BH + 2][BW], Vy[BH + 2][BW], Ly[BH + 2][BW];
+    float F[3][BH][BW + 1];            // x faces (evaluated with the west cell's north)
+    float G[3][BH + 1][BW];            // y faces (south cell's east vector)
+    // per-cell Coriolis / north fields and the second face evaluation they need
+    float Fc[FIELD ? BH + 4 : 1][FIELD ? BW + 4 : 1];
+    float Nx[FIELD ? BH + 4 : 1][FIELD ? BW + 4 : 1];
+    float Ny[FIELD ? BH + 4 : 1][FIELD ? BW + 4 : 1];
+    float F2[FIELD ? 3 : 1][FIELD ? BH : 1][FIELD ? BW + 1 : 1];
+    float G2[FIELD ? 3 : 1][FIELD ? BH + 1 : 1][FIELD ? BW : 1];
+};
+
+// Map a workgroup index to a tile index so that each of the 8 XCDs (which
+// receive workgroups round-robin) works on one contiguous band of tiles and
+// shares halo rows through its own L2.
+__device__ __forceinline__ int xcd_band(int b, int nb) {
+    const int q = nb >> 3, rem = nb & 7;
+    const int xcd = b & 7, idx = b >> 3;
+    return (xcd < rem) ? xcd * (q + 1) + idx : rem * (q + 1) + (xcd - rem) * q + idx;
+}
+
+// ---------------------------------------------------------------------------
+// the tile
+// ---------------------------------------------------------------------------
+template <int BW, int BH, int NT, bool FIELD>
+__device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int tile_y,
+                                           TileSmem<BW, BH, FIELD>& S) {
+    const int tid = threadIdx.x;
+    const int nx = A.nx, ny = A.ny, pitch = A.pitch;
+    const int bx = tile_x * BW;     // padded column of the tile's first halo column
+    const int by = tile_y * BH;     // padded row of the tile's first halo row
+
+    Consts K;
+    K.g = A.g;
+    K.half_g = 0.5f * A.g;
+    K.two_g = 2.0f * A.g;
+    K.theta = A.theta;
+    K.theta_g = A.theta * A.g;
+    K.dx = A.dx;
+    K.dy = A.dy;
+    K.vdx = A.dx / K.two_g;
+    K.udy = A.dy / K.two_g;
+    K.desing = A.desing_eps;
+    K.flux_eps = A.flux_eps;
+    K.cutoff = A.cutoff;
+    K.land = A.land;
+
+    const bool wN = A.bc_n == kWall, wS = A.bc_s == kWall;
+    const bool wE = A.bc_e == kWall, wW = A.bc_w == kWall;
+    const bool pNS = A.bc_n == kPeriodic, pEW = A.bc_e == kPeriodic;
+
+    // ---- phase A: corner depths (CDKLM16_kernel.cu:509-525) ----
+    for (int it = tid; it < (BH + 5) * (BW + 5); it += NT) {
+        const int r = it / (BW + 5), c = it - r * (BW + 5);
+        const int gr = min(by + r, ny + 4), gc = min(bx + c, nx + 4);
+        float h = A.Hi[(size_t)gr * A.hi_pitch + gc];
+        if (fabsf(h - K.land) < kDryEps) h = kDryFlag;
+        S.Hi[r][c] = h;
+    }
+    __syncthreads();
+
+    // ---- phase B: load state with boundary remap, convert to (eta,u,v)
+    //      (:490-505, handleWallBC :373-424, :551-592) ----
+    // Only the cross-shaped part of the halo is ever used; the four 2x2
+    // corner blocks are skipped.
+    constexpr int kMid = BH * (BW + 4);
+    constexpr int kConv = kMid + 4 * BW;
+    for (int it = tid; it < kConv; it += NT) {
+        int rr, cc;
+        if (it < kMid) {
+            rr = 2 + it / (BW + 4);
+            cc = it - (rr - 2) * (BW + 4);
+        } else {
+            const int k = it - kMid;
+            const int band = k / BW;
+            rr = band < 2 ? band : BH + band;     // 0,1,BH+2,BH+3
+            cc = 2 + (k - band * BW);
+        }
+        const int R = min(by + rr, ny + 3), C = min(bx + cc, nx + 3);
+        int sr = R, sc = C;
+        bool neg_u = false, neg_v = false;
+        if (R < 2) {
+            if (wS) { sr = 3 - R; neg_v = true; } else if (pNS) { sr = R + ny; }
+        } else if (R > ny + 1) {
+            if (wN) { sr = 2 * ny + 3 - R; neg_v = true; } else if (pNS) { sr = R - ny; }
+        }
+        if (C < 2) {
+            if (wW) { sc = 3 - C; neg_u = true; } else if (pEW) { sc = C + nx; }
+        } else if (C > nx + 1) {
+            if (wE) { sc = 2 * nx + 3 - C; neg_u = true; } else if (pEW) { sc = C - nx; }
+        }
+        const size_t src = (size_t)sr * pitch + sc;
+        const float eta = A.eta_in[src];
+        float hu = A.hu_in[src];
+        float hv = A.hv_in[src];
+        if (neg_u) hu = -hu;
+        if (neg_v) hv = -hv;
+        // cell depth as initBm computes it (initBm_kernel.cu:84-105)
+        const float a = S.Hi[rr][cc], b = S.Hi[rr + 1][cc];
+        const float c = S.Hi[rr][cc + 1], d = S.Hi[rr + 1][cc + 1];
+        const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag)
+                        - (float)(c == kDryFlag) - (float)(d == kDryFlag);
+        const float sum = ((a + b) + c) + d;
+        const float Hm = (wet == 0.0f) ? K.land : (wet == 4.0f ? sum * 0.25f : sum / wet);
+        const float h = eta + Hm;
+        float E, U, V;
+        if (fabsf(Hm - K.land) <= kDryEps) {
+            E = kDryFlag; U = 0.0f; V = 0.0f;
+        } else if (h < K.desing) {
+            if (h <= K.cutoff) {
+                E = -Hm + K.cutoff; U = 0.0f; V = 0.0f;
+            } else {
+                E = eta;
+                U = desingularize(h, hu, K.desing);
+                V = desingularize(h, hv, K.desing);
+            }
+        } else {
+            E = eta; U = hu / h; V = hv / h;
+        }
+        S.E[rr][cc] = E;
+        S.U[rr][cc] = U;
+        S.V[rr][cc] = V;
+        if (FIELD) {
+            const size_t o = (size_t)R * pitch + C;   // textures are read unremapped
+            S.Fc[rr][cc] = A.f_field[o];
+            S.Nx[rr][cc] = A.nx_field[o];
+            S.Ny[rr][cc] = A.ny_field[o];
+        }
+    }
+    __syncthreads();
+
+    auto fcell = [&](int rr, int cc) -> float {
+        if (FIELD) return S.Fc[rr][cc];
+        return A.f;
+    };
+    auto nxcell = [&](int rr, int cc) -> float {
+        if (FIELD) return S.Nx[rr][cc];
+        return A.north_x;
+    };
+    auto nycell = [&](int rr, int cc) -> float {
+        if (FIELD) return S.Ny[rr][cc];
+        return A.north_y;
+    };
+    // geostrophic sign flip of the tangential velocity in wall ghost cells
+    auto ghost_col_flip = [&](int C) -> bool {
+        return (wW && C < 2) || (wE && C > nx + 1);
+    };
+    auto ghost_row_flip = [&](int R) -> bool {
+        return (wS && R < 2) || (wN && R > ny + 1);
+    };
+
+    // ---- phase C: slopes and K/L reconstruction (:607-666, :148-191,
+    //      :711-770, :198-241) ----
+    constexpr int kSx = BH * (BW + 2);
+    constexpr int kSy = (BH + 2) * BW;
+    for (int it = tid; it < kSx + kSy; it += NT) {
+        if (it < kSx) {
+            const int j = it / (BW + 2), i = it - j * (BW + 2);
+            const int rr = j + 2, cc = i + 1;
+            const int C = bx + cc;
+            const float el = S.E[rr][cc - 1], ec = S.E[rr][cc], er = S.E[rr][cc + 1];
+            const float ul = S.U[rr][cc - 1], uc = S.U[rr][cc], ur = S.U[rr][cc + 1];
+            float vl = S.V[rr][cc - 1], vc = S.V[rr][cc], vr = S.V[rr][cc + 1];
+            S.Ux[j][i] = minmod_slope(ul, uc, ur, K.theta);
+            S.Vx[j][i] = minmod_slope(vl, vc, vr, K.theta);
+            const float vc_raw = vc;
+            if (ghost_col_flip(C - 1)) vl = -vl;
+            if (ghost_col_flip(C)) vc = -vc;
+            if (ghost_col_flip(C + 1)) vr = -vr;
+            const float n_x = nxcell(rr, cc), n_y = nycell(rr, cc);
+            const float fl = fcell(rr, cc - 1), fc = fcell(rr, cc), fr = fcell(rr, cc + 1);
+            const float fv_l = (n_x * ul + n_y * vl) * fl;
+            const float fv_c = (n_x * uc + n_y * vc) * fc;
+            const float fv_r = (n_x * ur + n_y * vr) * fr;
+            const float bw_ = K.theta_g * ((ec - el) - K.vdx * (fv_c + fv_l));
+            const float ce_ = K.half_g * ((er - el) - K.vdx * ((fv_r + 2.0f * fv_c) + fv_l));
+            const float fw_ = K.theta_g * ((er - ec) - K.vdx * (fv_c + fv_r));
+            float Kx = minmod_raw(bw_, ce_, fw_);
+            // positivity adjustment; the reference flips v only in west ghost
+            // column 1 here (its east test, bx+k > NX+2, never hits a column
+            // an interior cell reads)
+            const float v_adj = (wW && C < 2) ? -vc_raw : vc_raw;
+            const float dxfv = (K.dx * fc) * v_adj;
+            const float H_west = 0.5f * (S.Hi[rr][cc] + S.Hi[rr + 1][cc]);
+            const float H_east = 0.5f * (S.Hi[rr][cc + 1] + S.Hi[rr + 1][cc + 1]);
+            const float t = (Kx + dxfv) / K.two_g;
+            const float h_west = (ec + H_west) - t;
+            const float h_east = (ec + H_east) + t;
+            Kx = (h_west > 0.0f) ? Kx : (-dxfv) + K.two_g * (ec + H_west);
+            Kx = (h_east > 0.0f) ? Kx : (-dxfv) - K.two_g * (ec + H_east);
+            S.Kx[j][i] = Kx;
+        } else {
+            const int k = it - kSx;
+            const int j = k / BW, i = k - j * BW;
+            const int rr = j + 1, cc = i + 2;
+            const int R = by + rr;
+            const float el = S.E[rr - 1][cc], ec = S.E[rr][cc], eu = S.E[rr + 1][cc];
+            float ul = S.U[rr - 1][cc], uc = S.U[rr][cc], uu = S.U[rr + 1][cc];
+            const float vl = S.V[rr - 1][cc], vc = S.V[rr][cc], vu = S.V[rr + 1][cc];
+            S.Uy[j][i] = minmod_slope(ul, uc, uu, K.theta);
+            S.Vy[j][i] = minmod_slope(vl, vc, vu, K.theta);
+            const float uc_raw = uc;
+            if (ghost_row_flip(R - 1)) ul = -ul;
+            if (ghost_row_flip(R)) uc = -uc;
+            if (ghost_row_flip(R + 1)) uu = -uu;
+            const float n_x = nxcell(rr, cc), n_y = nycell(rr, cc);
+            const float e_x = n_y, e_y = -n_x;
+            const float fl = fcell(rr - 1, cc), fc = fcell(rr, cc), fu = fcell(rr + 1, cc);
+            const float fu_l = (e_x * ul + e_y * vl) * fl;
+            const float fu_c = (e_x * uc + e_y * vc) * fc;
+            const float fu_u = (e_x * uu + e_y * vu) * fu;
+            const float bw_ = K.theta_g * ((ec - el) + K.udy * (fu_c + fu_l));
+            const float ce_ = K.half_g * ((eu - el) + K.udy * ((fu_u + 2.0f * fu_c) + fu_l));
+            const float fw_ = K.theta_g * ((eu - ec) + K.udy * (fu_c + fu_u));
+            float Ly = minmod_raw(bw_, ce_, fw_);
+            const float u_adj = (wS && R < 2) ? -uc_raw : uc_raw;
+            const float dyfu = (K.dy * fc) * u_adj;
+            const float H_south = 0.5f * (S.Hi[rr][cc] + S.Hi[rr][cc + 1]);
+            const float H_north = 0.5f * (S.Hi[rr + 1][cc] + S.Hi[rr + 1][cc + 1]);
+            const float t = (Ly - dyfu) / K.two_g;
+            const float h_south = (ec + H_south) - t;
+            const float h_north = (ec + H_north) + t;
+            Ly = (h_south > 0.0f) ? Ly : dyfu + K.two_g * (ec + H_south);
+            Ly = (h_north > 0.0f) ? Ly : dyfu - K.two_g * (ec + H_north);
+            S.Ly[j][i] = Ly;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase D: face fluxes, each face once (:246-370) ----
+    constexpr int kFx = BH * (BW + 1);
+    constexpr int kFy = (BH + 1) * BW;
+    for (int it = tid; it < kFx + kFy; it += NT) {
+        if (it < kFx) {
+            const int j = it / (BW + 1), i = it - j * (BW + 1);
+            const int rr = j + 2, cm = i + 1, cp = i + 2;   // tile columns of m and p
+            const int Cp = bx + cp;
+            const float eta_p = S.E[rr][cp], eta_m = S.E[rr][cm];
+            float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Fx2 = 0.0f, Fy2 = 0.0f, Fz2 = 0.0f;
+            if (!(eta_p == kDryFlag || eta_m == kDryFlag)) {
+                const float up = S.U[rr][cp], um = S.U[rr][cm];
+                float vp = S.V[rr][cp], vm = S.V[rr][cm];
+                const float Rpx = up - 0.5f * S.Ux[j][i + 1];
+                const float Rpy = vp - 0.5f * S.Vx[j][i + 1];
+                const float Rmx = um + 0.5f * S.Ux[j][i];
+                const float Rmy = vm + 0.5f * S.Vx[j][i];
+                const float H_face = 0.5f * (S.Hi[rr][cp] + S.Hi[rr + 1][cp]);
+                const float Kp = S.Kx[j][i + 1], Km = S.Kx[j][i];
+                if (wW && Cp == 2) vm = -vm;
+                if (wE && Cp == nx + 2) vp = -vp;
+                const float dfp = K.dx * fcell(rr, cp), dfm = K.dx * fcell(rr, cm);
+                {
+                    const float n_x = nxcell(rr, cm), n_y = nycell(rr, cm);
+                    const float vpn = up * n_x + vp * n_y;
+                    const float vmn = um * n_x + vm * n_y;
+                    const float hp = fmaxf(0.0f, (eta_p + H_face) - ((Kp + dfp * vpn) / K.two_g));
+                    const float hm = fmaxf(0.0f, (eta_m + H_face) + ((Km + dfm * vmn) / K.two_g));
+                    cu_flux(hm, Rmx, Rmy, hp, Rpx, Rpy, K, Fx, Fy, Fz);
+                }
+                if (FIELD) {
+                    const float n_x = nxcell(rr, cp), n_y = nycell(rr, cp);
+                    const float vpn = up * n_x + vp * n_y;
+                    const float vmn = um * n_x + vm * n_y;
+                    const float hp = fmaxf(0.0f, (eta_p + H_face) - ((Kp + dfp * vpn) / K.two_g));
+                    const float hm = fmaxf(0.0f, (eta_m + H_face) + ((Km + dfm * vmn) / K.two_g));
+                    cu_flux(hm, Rmx, Rmy, hp, Rpx, Rpy, K, Fx2, Fy2, Fz2);
+                }
+            }
+            S.F[0][j][i] = Fx; S.F[1][j][i] = Fy; S.F[2][j][i] = Fz;
+            if (FIELD) { S.F2[0][j][i] = Fx2; S.F2[1][j][i] = Fy2; S.F2[2][j][i] = Fz2; }
+        } else {
+            const int k = it - kFx;
+            const int j = k / BW, i = k - j * BW;
+            const int rm = j + 1, rp = j + 2, cc = i + 2;
+            const int Rp = by + rp;
+            const float eta_p = S.E[rp][cc], eta_m = S.E[rm][cc];
+            float Gx = 0.0f, Gy = 0.0f, Gz = 0.0f, Gx2 = 0.0f, Gy2 = 0.0f, Gz2 = 0.0f;
+            if (!(eta_p == kDryFlag || eta_m == kDryFlag)) {
+                float up = S.U[rp][cc], um = S.U[rm][cc];
+                const float vp = S.V[rp][cc], vm = S.V[rm][cc];
+                const float Rpx = up - 0.5f * S.Uy[j + 1][i];
+                const float Rpy = vp - 0.5f * S.Vy[j + 1][i];
+                const float Rmx = um + 0.5f * S.Uy[j][i];
+                const float Rmy = vm + 0.5f * S.Vy[j][i];
+                const float H_face = 0.5f * (S.Hi[rp][cc] + S.Hi[rp][cc + 1]);
+                const float Lp = S.Ly[j + 1][i], Lm = S.Ly[j][i];
+                if (wS && Rp == 2) um = -um;
+                if (wN && Rp == ny + 2) up = -up;
+                const float dfp = K.dy * fcell(rp, cc), dfm = K.dy * fcell(rm, cc);
+                {
+                    const float e_x = nycell(rm, cc), e_y = -nxcell(rm, cc);
+                    const float upe = up * e_x + vp * e_y;
+                    const float ume = um * e_x + vm * e_y;
+                    const float hp = fmaxf(0.0f, (eta_p + H_face) - ((Lp - dfp * upe) / K.two_g));
+                    const float hm = fmaxf(0.0f, (eta_m + H_face) + ((Lm - dfm * ume) / K.two_g));
+                    float a, b, c;
+                    cu_flux(hm, Rmy, Rmx, hp, Rpy, Rpx, K, a, b, c);
+                    Gx = a; Gy = c; Gz = b;
+                }
+                if (FIELD) {
+                    const float e_x = nycell(rp, cc), e_y = -nxcell(rp, cc);
+                    const float upe = up * e_x + vp * e_y;
+                    const float ume = um * e_x + vm * e_y;
+                    const float hp = fmaxf(0.0f, (eta_p + H_face) - ((Lp - dfp * upe) / K.two_g));
+                    const float hm = fmaxf(0.0f, (eta_m + H_face) + ((Lm - dfm * ume) / K.two_g));
+                    float a, b, c;
+                    cu_flux(hm, Rmy, Rmx, hp, Rpy, Rpx, K, a, b, c);
+                    Gx2 = a; Gy2 = c; Gz2 = b;
+                }
+            }
+            S.G[0][j][i] = Gx; S.G[1][j][i] = Gy; S.G[2][j][i] = Gz;
+            if (FIELD) { S.G2[0][j][i] = Gx2; S.G2[1][j][i] = Gy2; S.G2[2][j][i] = Gz2; }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase E: sources, RK update, store (:802-1004) ----
+    const float dt = A.dt_dev ? *A.dt_dev : A.dt;
+    const float wt = A.wind_t;
+    const float X = wt * A.wx1 + (1.0f - wt) * A.wx0;
+    const float Y = wt * A.wy1 + (1.0f - wt) * A.wy0;
+    for (int it = tid; it < BW * BH; it += NT) {
+        const int j = it / BW, i = it - j * BW;
+        const int rr = j + 2, cc = i + 2;
+        const int R = by + rr, C = bx + cc;
+        if (R > ny + 1 || C > nx + 1) continue;
+        const float Hm = 0.25f * (((S.Hi[rr][cc] + S.Hi[rr + 1][cc]) + S.Hi[rr][cc + 1]) + S.Hi[rr + 1][cc + 1]);
+        const float ec = S.E[rr][cc], uc = S.U[rr][cc], vc = S.V[rr][cc];
+        const float hreg = ec + Hm;
+        float hu = 0.0f, hv = 0.0f;
+        if (hreg > K.cutoff) {
+            hu = uc * hreg;
+            hv = vc * hreg;
+        }
+        // west face is F[i] (evaluated by this cell as "p"), east face F[i+1]
+        float fex, fey, fez, fwx, fwy, fwz, gnx, gny, gnz, gsx, gsy, gsz;
+        fex = S.F[0][j][i + 1]; fey = S.F[1][j][i + 1]; fez = S.F[2][j][i + 1];
+        gnx = S.G[0][j + 1][i]; gny = S.G[1][j + 1][i]; gnz = S.G[2][j + 1][i];
+        if (FIELD) {
+            fwx = S.F2[0][j][i]; fwy = S.F2[1][j][i]; fwz = S.F2[2][j][i];
+            gsx = S.G2[0][j][i]; gsy = S.G2[1][j][i]; gsz = S.G2[2][j][i];
+        } else {
+            fwx = S.F[0][j][i]; fwy = S.F[1][j][i]; fwz = S.F[2][j][i];
+            gsx = S.G[0][j][i]; gsy = S.G[1][j][i]; gsz = S.G[2][j][i];
+        }
+        const float fd0 = (fex - fwx) / K.dx + (gnx - gsx) / K.dy;
+        const float fd1 = (fey - fwy) / K.dx + (gny - gsy) / K.dy;
+        const float fd2 = (fez - fwz) / K.dx + (gnz - gsz) / K.dy;
+
+        const float fc = fcell(rr, cc);
+        const float n_x = nxcell(rr, cc), n_y = nycell(rr, cc);
+        const float e_x = n_y, e_y = -n_x;
+        const float tx = (S.Kx[j][i + 1] + (K.dx * fc) * vc) / K.two_g;
+        const float eta_west = ec - tx, eta_east = ec + tx;
+        const float ty = (S.Ly[j + 1][i] - (K.dy * fc) * uc) / K.two_g;
+        const float eta_south = ec - ty, eta_north = ec + ty;
+
+        float st1 = 0.0f, st2 = 0.0f;
+        if (hreg >= K.cutoff && ec != kDryFlag) {
+            const float RHxp = 0.5f * (S.Hi[rr][cc + 1] + S.Hi[rr + 1][cc + 1]);
+            const float RHxm = 0.5f * (S.Hi[rr][cc] + S.Hi[rr + 1][cc]);
+            const float RHyp = 0.5f * (S.Hi[rr + 1][cc] + S.Hi[rr + 1][cc + 1]);
+            const float RHym = 0.5f * (S.Hi[rr][cc] + S.Hi[rr][cc + 1]);
+            const float H_x = RHxp - RHxm;
+            const float H_y = RHyp - RHym;
+            const float eta_sn = 0.5f * (eta_north + eta_south);
+            const float eta_we = 0.5f * (eta_west + eta_east);
+            const float b1 = (K.g * (eta_we + Hm)) * H_x;
+            const float b2 = (K.g * (eta_sn + Hm)) * H_y;
+            const float hu_east = hu * e_x + hv * e_y;
+            const float hv_north = hu * n_x + hv * n_y;
+            const float hu_east_cor = fc * hv_north;
+            const float hv_north_cor = (-fc) * hu_east;
+            const float up_x = -n_x, up_y = n_y;
+            const float right_x = up_y, right_y = -up_x;
+            const float hu_cor = right_x * hu_east_cor + right_y * hv_north_cor;
+            const float hv_cor = up_x * hu_east_cor + up_y * hv_north_cor;
+            st1 = (X + hu_cor) + b1 / K.dx;
+            st2 = (Y + hv_cor) + b2 / K.dy;
+        }
+        const float L1 = -fd0;
+        const float L2 = -fd1 + st1;
+        const float L3 = -fd2 + st2;
+
+        float Cf = 0.0f;
+        if (A.r > 0.0f) {   // quadratic bottom friction (:884-900)
+            const float h = hreg;
+            if (h < K.desing) {
+                const float u = desingularize(h, hu, K.desing);
+                const float v = desingularize(h, hv, K.desing);
+                Cf = (dt * A.r) * desingularize(h, sqrtf(u * u + v * v), K.desing);
+            } else {
+                const float u = hu / h, v = hv / h;
+                Cf = ((dt * A.r) * sqrtf(u * u + v * v)) / h;
+            }
+        }
+
+        const size_t o = (size_t)R * pitch + C;
+        float ue, uu, uv;
+        if (A.rk_order < 3) {
+            if (A.rk_step == 0) {
+                ue = ec + dt * L1;
+                uu = (hu + dt * L2) / (1.0f + Cf);
+                uv = (hv + dt * L3) / (1.0f + Cf);
+            } else {
+                const float ea = A.eta_qn[o], ha = A.hu_qn[o], va = A.hv_qn[o];
+                ue = 0.5f * (ea + (ec + dt * L1));
+                uu = (0.5f * (ha + (hu + dt * L2))) / (1.0f + 0.5f * Cf);
+                uv = (0.5f * (va + (hv + dt * L3))) / (1.0f + 0.5f * Cf);
+            }
+        } else {
+            if (A.rk_step == 0) {
+                ue = ec + dt * L1;
+                uu = hu + dt * L2;
+                uv = hv + dt * L3;
+            } else if (A.rk_step == 1) {
+                const float ea = A.eta_qn[o], ha = A.hu_qn[o], va = A.hv_qn[o];
+                ue = 0.75f * ea + 0.25f * (ec + dt * L1);
+                uu = 0.75f * ha + 0.25f * (hu + dt * L2);
+                uv = 0.75f * va + 0.25f * (hv + dt * L3);
+            } else {
+                const float ea = A.eta_qn[o], ha = A.hu_qn[o], va = A.hv_qn[o];
+                ue = (ea + 2.0f * (ec + dt * L1)) / 3.0f;
+                uu = (ha + 2.0f * (hu + dt * L2)) / 3.0f;
+                uv = (va + 2.0f * (hv + dt * L3)) / 3.0f;
+            }
+        }
+        if (ue + Hm <= K.cutoff) {
+            ue = -Hm + K.cutoff;
+            uu = 0.0f;
+            uv = 0.0f;
+        }
+        ue = fmaxf(-Hm + K.cutoff, ue);
+        A.eta_out[o] = ue;
+        A.hu_out[o] = uu;
+        A.hv_out[o] = uv;
+        // keep the periodic ghost cells of the output consistent, as the
+        // reference's boundary kernels would after this substep
+        if (pNS || pEW) {
+            // ghost rows / columns this cell is the periodic image of:
+            // ghost index g in {0,1} copies g+n, g in {n+2,n+3} copies g-n
+            int rows[3] = {R, -1, -1}, cols[3] = {C, -1, -1};
+            int nr = 1, nc = 1;
+            if (pNS) {
+                if (R + ny <= ny + 3) rows[nr++] = R + ny;
+                if (R - ny >= 0) rows[nr++] = R - ny;
+            }
+            if (pEW) {
+                if (C + nx <= nx + 3) cols[nc++] = C + nx;
+                if (C - nx >= 0) cols[nc++] = C - nx;
+            }
+            for (int a = 0; a < nr; ++a) {
+                for (int b = 0; b < nc; ++b) {
+                    if (a == 0 && b == 0) continue;
+                    const size_t q = (size_t)rows[a] * pitch + cols[b];
+                    A.eta_out[q] = ue; A.hu_out[q] = uu; A.hv_out[q] = uv;
+                }
+            }
+        }
+    }
+}
+
+}  // namespace gocean

@@ -1,0 +1,427 @@
+// gocean_fwd.hip -- forward CDKLM16 kernels and their C-ABI launchers.
+//
+// Kernels: bathymetry halo + cell depth, periodic ghost exchange, the CFL
+// time-step reduction and the tiled RK substep (cdklm16_tile.hpp).  See
+// include/gocean.h for the contract of each entry point.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/gocean.h"
+#include "cdklm16_tile.hpp"
+#include "gocean_internal.hpp"
+
+namespace gocean {
+
+// ---------------------------------------------------------------------------
+// error reporting shared by every translation unit
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code == 0 ? -1 : code;
+}
+
+void clear_error() { g_last_error.clear(); }
+
+const char* last_error() { return g_last_error.c_str(); }
+
+int check_launch(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error((int)e, "%s: %s", what, hipGetErrorString(e));
+    return 0;
+}
+
+int validate_domain(const gocean_domain_t* d) {
+    if (!d) return set_error(GOCEAN_EINVAL, "domain descriptor is NULL");
+    if (d->nx < 2 || d->ny < 2)
+        return set_error(GOCEAN_EINVAL, "domain too small: nx=%d ny=%d (need >= 2)", d->nx, d->ny);
+    if (d->pitch < d->nx + 4)
+        return set_error(GOCEAN_EINVAL, "pitch %d < nx+4 = %d", d->pitch, d->nx + 4);
+    if (d->hi_pitch < d->nx + 5)
+        return set_error(GOCEAN_EINVAL, "hi_pitch %d < nx+5 = %d", d->hi_pitch, d->nx + 5);
+    const int bcs[4] = {d->bc_north, d->bc_east, d->bc_south, d->bc_west};
+    for (int k = 0; k < 4; ++k)
+        if (bcs[k] != kWall && bcs[k] != kPeriodic)
+            return set_error(GOCEAN_EINVAL,
+                             "boundary condition %d not supported (1 wall, 2 periodic)", bcs[k]);
+    if ((d->bc_north == kPeriodic) != (d->bc_south == kPeriodic))
+        return set_error(GOCEAN_EINVAL, "periodic north/south must be paired");
+    if ((d->bc_east == kPeriodic) != (d->bc_west == kPeriodic))
+        return set_error(GOCEAN_EINVAL, "periodic east/west must be paired");
+    if (d->rk_order < 1 || d->rk_order > 3)
+        return set_error(GOCEAN_EINVAL, "rk_order %d not in {1,2,3}", d->rk_order);
+    if (d->rk_order == 3 && d->r != 0.0f)
+        return set_error(GOCEAN_EINVAL, "rk_order 3 requires r == 0");
+    if (!(d->dx > 0.0f) || !(d->dy > 0.0f) || !(d->g > 0.0f))
+        return set_error(GOCEAN_EINVAL, "dx, dy and g must be positive");
+    return 0;
+}
+
+void fill_args(const gocean_domain_t* d, SubstepArgs& A) {
+    std::memset(&A, 0, sizeof(A));
+    A.nx = d->nx; A.ny = d->ny; A.pitch = d->pitch; A.hi_pitch = d->hi_pitch;
+    A.dx = d->dx; A.dy = d->dy; A.g = d->g; A.r = d->r; A.theta = d->theta;
+    A.desing_eps = d->desing_eps; A.flux_eps = d->flux_slope_eps;
+    A.cutoff = d->depth_cutoff; A.land = d->land_value;
+    A.f = d->f; A.north_x = d->north_x; A.north_y = d->north_y;
+    A.bc_n = d->bc_north; A.bc_e = d->bc_east; A.bc_s = d->bc_south; A.bc_w = d->bc_west;
+    A.rk_order = d->rk_order;
+}
+
+// ---------------------------------------------------------------------------
+// tile configurations
+// ---------------------------------------------------------------------------
+const TileConfig kTileConfigs[] = {
+    {32, 8, 256, "32x8/256"},
+    {32, 16, 256, "32x16/256"},
+    {64, 8, 256, "64x8/256"},
+    {16, 16, 256, "16x16/256"},
+    {64, 4, 256, "64x4/256"},
+    {32, 16, 512, "32x16/512"},
+};
+const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
+const int kDefaultTileConfig = 1;
+
+template <int BW, int BH, int NT, bool FIELD>
+__global__ __launch_bounds__(NT) void k_substep(SubstepArgs A) {
+    __shared__ TileSmem<BW, BH, FIELD> S;
+    const int t = xcd_band(blockIdx.x, gridDim.x);
+    const int ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
+    cdklm_tile<BW, BH, NT, FIELD>(A, tx, ty, S);
+}
+
+template <int BW, int BH, int NT, bool FIELD>
+static void launch_substep_t(SubstepArgs A, hipStream_t s) {
+    A.tiles_x = (A.nx + BW - 1) / BW;
+    A.tiles_y = (A.ny + BH - 1) / BH;
+    const int nb = A.tiles_x * A.tiles_y;
+    hipLaunchKernelGGL((k_substep<BW, BH, NT, FIELD>), dim3(nb), dim3(NT), 0, s, A);
+}
+
+int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s) {
+    if (cfg < 0) cfg = kDefaultTileConfig;
+    if (cfg >= kNumTileConfigs)
+        return set_error(GOCEAN_EINVAL, "tile configuration %d out of range [0,%d)", cfg, kNumTileConfigs);
+    if (field) {
+        // per-cell Coriolis / north fields: one general configuration
+        launch_substep_t<32, 8, 256, true>(A, s);
+    } else {
+        switch (cfg) {
+            case 0: launch_substep_t<32, 8, 256, false>(A, s); break;
+            case 1: launch_substep_t<32, 16, 256, false>(A, s); break;
+            case 2: launch_substep_t<64, 8, 256, false>(A, s); break;
+            case 3: launch_substep_t<16, 16, 256, false>(A, s); break;
+            case 4: launch_substep_t<64, 4, 256, false>(A, s); break;
+            case 5: launch_substep_t<32, 16, 512, false>(A, s); break;
+        }
+    }
+    return check_launch("cdklm substep launch");
+}
+
+// ---------------------------------------------------------------------------
+// bathymetry: Hi halo (boundary_kernels.cu:121-246) and Hm (initBm :38-107)
+// ---------------------------------------------------------------------------
+__global__ void k_hi_halo_ns(float* Hi, int nx, int ny, int hp, int periodic) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;   // column
+    if (i > nx + 4) return;
+    if (periodic) {
+        Hi[0 * hp + i] = Hi[(size_t)ny * hp + i];
+        Hi[1 * hp + i] = Hi[(size_t)(ny + 1) * hp + i];
+        for (int j = ny + 2; j <= ny + 4; ++j) Hi[(size_t)j * hp + i] = Hi[(size_t)(j - ny) * hp + i];
+    } else {
+        for (int j = 0; j < 2; ++j) Hi[(size_t)j * hp + i] = Hi[(size_t)(4 - j) * hp + i];
+        for (int j = 0; j < 2; ++j) Hi[(size_t)(ny + 4 - j) * hp + i] = Hi[(size_t)(ny + j) * hp + i];
+    }
+}
+
+__global__ void k_hi_halo_ew(float* Hi, int nx, int ny, int hp, int periodic) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;   // row
+    if (j > ny + 4) return;
+    float* row = Hi + (size_t)j * hp;
+    if (periodic) {
+        row[0] = row[nx];
+        row[1] = row[nx + 1];
+        for (int i = nx + 2; i <= nx + 4; ++i) row[i] = row[i - nx];
+    } else {
+        for (int i = 0; i < 2; ++i) row[i] = row[4 - i];
+        for (int i = 0; i < 2; ++i) row[nx + 4 - i] = row[nx + i];
+    }
+}
+
+__global__ void k_init_bm(const float* Hi, float* Hm, int nx, int ny, int hp, int pitch, float land) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= nx + 4 || j >= ny + 4) return;
+    auto flag = [&](float v) { return (fabsf(v - land) <= kDryEps) ? kDryFlag : v; };
+    const float a = flag(Hi[(size_t)j * hp + i]);
+    const float b = flag(Hi[(size_t)(j + 1) * hp + i]);
+    const float c = flag(Hi[(size_t)j * hp + i + 1]);
+    const float d = flag(Hi[(size_t)(j + 1) * hp + i + 1]);
+    const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag)
+                    - (float)(c == kDryFlag) - (float)(d == kDryFlag);
+    Hm[(size_t)j * pitch + i] = (wet == 0.0f) ? land : (((a + b) + c) + d) / wet;
+}
+
+// ---------------------------------------------------------------------------
+// periodic ghost exchange (boundary_kernels.cu:35-110), NS then EW
+// ---------------------------------------------------------------------------
+__global__ void k_periodic_ns(float* e, float* u, float* v, int nx, int ny, int pitch) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = blockIdx.y;        // 0..3 -> ghost rows 0,1,ny+2,ny+3
+    if (i > nx + 3) return;
+    const int j = g < 2 ? g : ny + g;
+    const int src = g < 2 ? j + ny : j - ny;
+    const size_t d = (size_t)j * pitch + i, s = (size_t)src * pitch + i;
+    e[d] = e[s]; u[d] = u[s]; v[d] = v[s];
+}
+
+__global__ void k_periodic_ew(float* e, float* u, float* v, int nx, int ny, int pitch) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int g = blockIdx.y;
+    if (j > ny + 3) return;
+    const int i = g < 2 ? g : nx + g;
+    const int src = g < 2 ? i + nx : i - nx;
+    const size_t d = (size_t)j * pitch + i, s = (size_t)j * pitch + src;
+    e[d] = e[s]; u[d] = u[s]; v[d] = v[s];
+}
+
+int periodic_bc(const gocean_domain_t* d, float* e, float* u, float* v, hipStream_t s) {
+    if (d->bc_north == kPeriodic) {
+        hipLaunchKernelGGL(k_periodic_ns, dim3((d->nx + 4 + 255) / 256, 4), dim3(256), 0, s,
+                           e, u, v, d->nx, d->ny, d->pitch);
+    }
+    if (d->bc_east == kPeriodic) {
+        hipLaunchKernelGGL(k_periodic_ew, dim3((d->ny + 4 + 255) / 256, 4), dim3(256), 0, s,
+                           e, u, v, d->nx, d->ny, d->pitch);
+    }
+    return check_launch("periodic boundary launch");
+}
+
+// ---------------------------------------------------------------------------
+// CFL time step (max_dt.cu:42-194).  The minimum is exact, so any reduction
+// order gives the reference's value; partial minima go through the caller's
+// buffer, so the two launches are graph-capturable and need no memset.
+// ---------------------------------------------------------------------------
+constexpr int kDtBlocks = 1024;
+constexpr int kDtThreads = 256;
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float block_min(float v, float* red) {
+    v = wave_min(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        v = (lane < (int)(blockDim.x >> 6)) ? red[lane] : 1.0e5f;
+        v = wave_min(v);
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(kDtThreads) void k_dt_partial(const float* eta, const float* hu,
+                                                           const float* hv, const float* Hm,
+                                                           int nx, int ny, int pitch, float dx, float dy,
+                                                           float g, float land, float* partial) {
+    __shared__ float red[kDtThreads / 64];
+    float m = 1.0e5f;   // FLT_MAX of max_dt.cu:29
+    const long long n = (long long)nx * ny;
+    const float qdx = 0.25f * dx, qdy = 0.25f * dy;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x) {
+        const int j = (int)(k / nx) + 2, i = (int)(k % nx) + 2;
+        const size_t o = (size_t)j * pitch + i;
+        const float H = Hm[o];
+        const float h = eta[o] + H;
+        if (h <= 0.0f || fabsf(H - land) < kDryEps) continue;
+        const float u = hu[o] / h, v = hv[o] / h;
+        const float c = sqrtf(g * h);
+        float d = qdx / fabsf(u + c);
+        d = fminf(d, qdx / fabsf(u - c));
+        d = fminf(d, qdy / fabsf(v + c));
+        d = fminf(d, qdy / fabsf(v - c));
+        m = fminf(m, d);
+    }
+    m = block_min(m, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+__global__ __launch_bounds__(kDtThreads) void k_dt_final(const float* partial, int n, float courant,
+                                                         float* out) {
+    __shared__ float red[kDtThreads / 64];
+    float m = 1.0e5f;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) m = fminf(m, partial[k]);
+    m = block_min(m, red);
+    if (threadIdx.x == 0) {
+        out[0] = m;
+        out[1] = courant > 0.0f ? courant * m : m;
+    }
+}
+
+int max_dt(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
+           const float* Hm, float courant, float* out, hipStream_t s) {
+    const long long n = (long long)d->nx * d->ny;
+    int nb = (int)((n + kDtThreads * 8 - 1) / (kDtThreads * 8));
+    nb = nb < 1 ? 1 : (nb > kDtBlocks ? kDtBlocks : nb);
+    hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kDtThreads), 0, s, eta, hu, hv, Hm, d->nx,
+                       d->ny, d->pitch, d->dx, d->dy, d->g, d->land_value, out + 2);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kDtThreads), 0, s, out + 2, nb, courant, out);
+    return check_launch("max dt launch");
+}
+
+}  // namespace gocean
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+using namespace gocean;
+
+extern "C" {
+
+const char* gocean_version(void) { return "gocean-mi355x 0.1 (gfx950)"; }
+
+const char* gocean_last_error(void) { return last_error(); }
+
+int gocean_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int gocean_tile_configs(const char** names, int max) {
+    for (int k = 0; k < kNumTileConfigs && k < max; ++k) names[k] = kTileConfigs[k].name;
+    return kNumTileConfigs;
+}
+
+int gocean_bathymetry_init(const gocean_domain_t* d, float* Hi, float* Hm, void* stream) {
+    clear_error();
+    if (int e = validate_domain(d)) return e;
+    if (!Hi || !Hm) return set_error(GOCEAN_EINVAL, "Hi/Hm is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    const int nsp = d->bc_north == kPeriodic && d->bc_south == kPeriodic;
+    const int ewp = d->bc_east == kPeriodic && d->bc_west == kPeriodic;
+    hipLaunchKernelGGL(k_hi_halo_ns, dim3((d->nx + 5 + 255) / 256), dim3(256), 0, s, Hi, d->nx,
+                       d->ny, d->hi_pitch, nsp);
+    hipLaunchKernelGGL(k_hi_halo_ew, dim3((d->ny + 5 + 255) / 256), dim3(256), 0, s, Hi, d->nx,
+                       d->ny, d->hi_pitch, ewp);
+    hipLaunchKernelGGL(k_init_bm, dim3((d->nx + 4 + 15) / 16, (d->ny + 4 + 15) / 16), dim3(16, 16),
+                       0, s, Hi, Hm, d->nx, d->ny, d->hi_pitch, d->pitch, d->land_value);
+    return check_launch("bathymetry init launch");
+}
+
+int gocean_periodic_bc(const gocean_domain_t* d, float* eta, float* hu, float* hv, void* stream) {
+    clear_error();
+    if (int e = validate_domain(d)) return e;
+    if (!eta || !hu || !hv) return set_error(GOCEAN_EINVAL, "state pointer is NULL");
+    return periodic_bc(d, eta, hu, hv, (hipStream_t)stream);
+}
+
+int gocean_max_dt(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
+                  const float* Hm, float courant, float* out_dev, void* stream) {
+    clear_error();
+    if (int e = validate_domain(d)) return e;
+    if (!eta || !hu || !hv || !Hm || !out_dev) return set_error(GOCEAN_EINVAL, "NULL pointer");
+    return max_dt(d, eta, hu, hv, Hm, courant, out_dev, (hipStream_t)stream);
+}
+
+int gocean_cdklm_substep(const gocean_domain_t* d, float dt, const float* dt_dev, int rk_step,
+                         const float* eta_in, const float* hu_in, const float* hv_in,
+                         const float* eta_qn, const float* hu_qn, const float* hv_qn,
+                         float* eta_out, float* hu_out, float* hv_out, const float* Hi,
+                         const gocean_fields_t* fields, const gocean_wind_t* wind,
+                         int tile_config, void* stream) {
+    clear_error();
+    if (int e = validate_domain(d)) return e;
+    if (!eta_in || !hu_in || !hv_in || !eta_out || !hu_out || !hv_out || !Hi)
+        return set_error(GOCEAN_EINVAL, "state/bathymetry pointer is NULL");
+    if (rk_step < 0 || rk_step >= d->rk_order)
+        return set_error(GOCEAN_EINVAL, "rk_step %d invalid for rk_order %d", rk_step, d->rk_order);
+    if (eta_in == eta_out || hu_in == hu_out || hv_in == hv_out)
+        return set_error(GOCEAN_EINVAL, "input and output state must not alias");
+    SubstepArgs A;
+    fill_args(d, A);
+    A.eta_in = eta_in; A.hu_in = hu_in; A.hv_in = hv_in;
+    A.eta_out = eta_out; A.hu_out = hu_out; A.hv_out = hv_out;
+    A.eta_qn = eta_qn ? eta_qn : eta_out;
+    A.hu_qn = hu_qn ? hu_qn : hu_out;
+    A.hv_qn = hv_qn ? hv_qn : hv_out;
+    A.Hi = Hi;
+    A.dt = dt;
+    A.dt_dev = dt_dev;
+    A.rk_step = rk_step;
+    bool field = false;
+    if (fields && (fields->f || fields->north_x || fields->north_y)) {
+        if (!fields->f || !fields->north_x || !fields->north_y)
+            return set_error(GOCEAN_EINVAL, "per-cell fields must give f, north_x and north_y together");
+        A.f_field = fields->f; A.nx_field = fields->north_x; A.ny_field = fields->north_y;
+        field = true;
+    }
+    if (wind) {
+        A.wind_t = wind->t; A.wx0 = wind->x0; A.wx1 = wind->x1; A.wy0 = wind->y0; A.wy1 = wind->y1;
+    }
+    return launch_substep(A, tile_config, field, (hipStream_t)stream);
+}
+
+int gocean_rk_steps(const gocean_domain_t* d, int n_steps, float dt, const float* dt_dev,
+                    float* eta0, float* hu0, float* hv0, float* eta1, float* hu1, float* hv1,
+                    float* eta2, float* hu2, float* hv2, const float* Hi,
+                    const gocean_fields_t* fields, const gocean_wind_t* wind, int tile_config,
+                    void* stream) {
+    clear_error();
+    if (int e = validate_domain(d)) return e;
+    if (n_steps < 0) return set_error(GOCEAN_EINVAL, "n_steps < 0");
+    if (d->rk_order == 3 && (!eta2 || !hu2 || !hv2))
+        return set_error(GOCEAN_EINVAL, "rk_order 3 needs the third buffer set");
+    if (d->rk_order == 1 && n_steps % 2 != 0)
+        return set_error(GOCEAN_EINVAL,
+                         "rk_order 1 swaps buffers every step; call with an even n_steps");
+    for (int n = 0; n < n_steps; ++n) {
+        int e = 0;
+        if (d->rk_order == 2) {
+            e = gocean_cdklm_substep(d, dt, dt_dev, 0, eta0, hu0, hv0, nullptr, nullptr, nullptr,
+                                     eta1, hu1, hv1, Hi, fields, wind, tile_config, stream);
+            if (!e)
+                e = gocean_cdklm_substep(d, dt, dt_dev, 1, eta1, hu1, hv1, nullptr, nullptr, nullptr,
+                                         eta0, hu0, hv0, Hi, fields, wind, tile_config, stream);
+        } else if (d->rk_order == 1) {
+            // Euler: h0 -> h1, then the caller's buffers swap; two steps
+            // bring the state back into h0.
+            if (n % 2 == 0)
+                e = gocean_cdklm_substep(d, dt, dt_dev, 0, eta0, hu0, hv0, nullptr, nullptr, nullptr,
+                                         eta1, hu1, hv1, Hi, fields, wind, tile_config, stream);
+            else
+                e = gocean_cdklm_substep(d, dt, dt_dev, 0, eta1, hu1, hv1, nullptr, nullptr, nullptr,
+                                         eta0, hu0, hv0, Hi, fields, wind, tile_config, stream);
+        } else {
+            // SSP-RK3; the middle stage goes to the third buffer instead of
+            // overwriting its own input as the reference does
+            e = gocean_cdklm_substep(d, dt, dt_dev, 0, eta0, hu0, hv0, nullptr, nullptr, nullptr,
+                                     eta1, hu1, hv1, Hi, fields, wind, tile_config, stream);
+            if (!e)
+                e = gocean_cdklm_substep(d, dt, dt_dev, 1, eta1, hu1, hv1, eta0, hu0, hv0, eta2, hu2,
+                                         hv2, Hi, fields, wind, tile_config, stream);
+            if (!e)
+                e = gocean_cdklm_substep(d, dt, dt_dev, 2, eta2, hu2, hv2, nullptr, nullptr, nullptr,
+                                         eta0, hu0, hv0, Hi, fields, wind, tile_config, stream);
+        }
+        if (e) return e;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,33 @@
+// gocean_internal.hpp -- declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+
+#include "../../include/gocean.h"
+#include "cdklm16_tile.hpp"
+
+namespace gocean {
+
+struct TileConfig {
+    int bw, bh, nt;
+    const char* name;
+};
+
+extern const TileConfig kTileConfigs[];
+extern const int kNumTileConfigs;
+extern const int kDefaultTileConfig;
+
+int set_error(int code, const char* fmt, ...);
+void clear_error();
+const char* last_error();
+int check_launch(const char* what);
+int validate_domain(const gocean_domain_t* d);
+void fill_args(const gocean_domain_t* d, SubstepArgs& A);
+int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s);
+int periodic_bc(const gocean_domain_t* d, float* e, float* u, float* v, hipStream_t s);
+int max_dt(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
+           const float* Hm, float courant, float* out, hipStream_t s);
+
+}  // namespace gocean

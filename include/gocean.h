@@ -1,0 +1,141 @@
+/*
+ * gocean.h -- C ABI of the MI355X (gfx950) CDKLM16 shallow-water step.
+ *
+ * Every entry point is extern "C", takes plain device pointers, element
+ * pitches and a hipStream_t passed as void*, returns 0 on success or a
+ * non-zero code whose message gocean_last_error() returns (thread-local).
+ * No C++ exception crosses this boundary.  All arrays are float32 device
+ * memory owned by the caller; the library owns only its small scratch.
+ *
+ * Layout (the reference's, gpu_ocean/SWESimulators/Common.py:503-546):
+ *   state arrays eta, hu, hv : (ny+4) rows x pitch floats, two ghost cells on
+ *                              every side, interior cells [2,ny+2) x [2,nx+2)
+ *   corner depth Hi          : (ny+5) rows x hi_pitch floats, Hi[j][i] is the
+ *                              south-west corner of cell (j,i)
+ *   cell depth Hm            : (ny+4) rows x pitch floats
+ * Land is marked in Hi by land_value (1e20, Common.py:1331).
+ */
+#ifndef GOCEAN_H
+#define GOCEAN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GOCEAN_BC_WALL 1
+#define GOCEAN_BC_PERIODIC 2
+
+/* Return codes: 0 success, > 0 a hipError_t passed through, < 0 below. */
+#define GOCEAN_OK 0
+#define GOCEAN_EINVAL (-1)   /* invalid argument (message says which) */
+#define GOCEAN_ENOMEM (-2)   /* device allocation failed */
+#define GOCEAN_ESTATE (-3)   /* call out of order (e.g. replay before capture) */
+
+/* Scheme constants of one domain.  The reference bakes these into the kernel
+ * with #defines at JIT time (gpu_ocean/SWESimulators/CDKLM16.py:179-191);
+ * here they are run-time arguments so that one launch can serve domains of
+ * different sizes. */
+typedef struct gocean_domain {
+    int32_t nx, ny;          /* interior cells */
+    int32_t pitch;           /* floats per row of state / Hm arrays, >= nx+4 */
+    int32_t hi_pitch;        /* floats per row of Hi, >= nx+5 */
+    float dx, dy, g, r;      /* spacing, gravity, quadratic bottom friction */
+    float theta;             /* minmod theta (1.3) */
+    float desing_eps;        /* KPSIMULATOR_DESING_EPS (0.1) */
+    float flux_slope_eps;    /* KPSIMULATOR_FLUX_SLOPE_EPS (0.1) */
+    float depth_cutoff;      /* KPSIMULATOR_DEPTH_CUTOFF (1e-5) */
+    float land_value;        /* 1e20 */
+    float f;                 /* constant Coriolis parameter */
+    float north_x, north_y;  /* constant north vector (sin a, cos a) */
+    int32_t bc_north, bc_east, bc_south, bc_west;   /* GOCEAN_BC_* */
+    int32_t rk_order;        /* 1, 2 or 3 */
+} gocean_domain_t;
+
+/* Uniform wind stress (tau/rho_water, m^2/s^2) at the two bracketing times
+ * and the interpolation weight t in [0,1] (Simulator.py:216-223,
+ * gpu_kernels/common.cu:616-650 with 1x1 textures). */
+typedef struct gocean_wind {
+    float t, x0, x1, y0, y1;
+} gocean_wind_t;
+
+/* Optional per-cell fields, (ny+4) x pitch floats each; NULL selects the
+ * constant f / north vector of the domain.  Replace the angle_tex and
+ * coriolis_f_tex textures (gpu_kernels/CDKLM16_kernel.cu:30-77). */
+typedef struct gocean_fields {
+    const float* f;
+    const float* north_x;
+    const float* north_y;
+} gocean_fields_t;
+
+/* Library version string. */
+const char* gocean_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gocean_last_error(void);
+
+/* Number of visible HIP devices (0 when there is none). */
+int gocean_device_count(void);
+
+/* Fill the two-deep halo of Hi (walls mirror, periodic wraps) and compute
+ * Hm = mean of the wet corners.  Replaces Common.Bathymetry's device work:
+ * _boundaryConditions + initBm (gpu_ocean/SWESimulators/Common.py:1376-1389,
+ * 1436-1459 -> gpu_kernels/boundary_kernels.cu:121-246,
+ * gpu_kernels/initBm_kernel.cu:38-107). */
+int gocean_bathymetry_init(const gocean_domain_t* d, float* Hi, float* Hm,
+                           void* stream);
+
+/* One Runge-Kutta substep for all interior cells (cdklm_swe_2D,
+ * gpu_ocean/SWESimulators/gpu_kernels/CDKLM16_kernel.cu:429-1006, launched by
+ * CDKLM16.callKernel, CDKLM16.py:613-639).  Reads the stencil from *_in,
+ * reads Q^n from *_qn when rk_step >= 1 (pass the output arrays, as the
+ * reference does, or NULL for "same as output"), writes *_out.  Wall and
+ * periodic boundaries are applied while loading the tile, so no separate
+ * ghost-cell launch is needed; for periodic sides the kernel also refreshes
+ * the output's ghost cells.  dt_dev, when not NULL, is a device float that
+ * overrides dt (device-resident time step). */
+int gocean_cdklm_substep(const gocean_domain_t* d, float dt, const float* dt_dev,
+                         int rk_step,
+                         const float* eta_in, const float* hu_in, const float* hv_in,
+                         const float* eta_qn, const float* hu_qn, const float* hv_qn,
+                         float* eta_out, float* hu_out, float* hv_out,
+                         const float* Hi, const gocean_fields_t* fields,
+                         const gocean_wind_t* wind, int tile_config,
+                         void* stream);
+
+/* n full RK steps at fixed dt (rk_order substeps each), h0 -> h0 with h1 as
+ * the intermediate buffer; the sequence of CDKLM16.step's loop body
+ * (CDKLM16.py:508-593) for a run of equal local_dt. */
+int gocean_rk_steps(const gocean_domain_t* d, int n_steps, float dt,
+                    const float* dt_dev,
+                    float* eta0, float* hu0, float* hv0,
+                    float* eta1, float* hu1, float* hv1,
+                    float* eta2, float* hu2, float* hv2,   /* RK3 only, else NULL */
+                    const float* Hi, const gocean_fields_t* fields,
+                    const gocean_wind_t* wind, int tile_config, void* stream);
+
+/* Periodic ghost-cell exchange, north/south then east/west
+ * (BoundaryConditionsArakawaA.boundaryCondition, Common.py:1219-1239 ->
+ * gpu_kernels/boundary_kernels.cu:35-110).  Wall sides are left untouched. */
+int gocean_periodic_bc(const gocean_domain_t* d, float* eta, float* hu, float* hv,
+                       void* stream);
+
+/* CFL time step (per_block_max_dt + max_dt_reduction,
+ * gpu_kernels/max_dt.cu:42-194, CDKLM16.updateDt, CDKLM16.py:712-739).
+ * Writes out_dev[0] = min over wet interior cells of
+ * min(0.25dx/|u+-c|, 0.25dy/|v+-c|) (1e5 if none), and, when courant > 0,
+ * out_dev[1] = courant*out_dev[0] rounded to float.  No host sync. */
+int gocean_max_dt(const gocean_domain_t* d, const float* eta, const float* hu,
+                  const float* hv, const float* Hm, float courant, float* out_dev,
+                  void* stream);
+
+/* Names of the tile configurations the library is built with; returns the
+ * count and writes up to max entries of "BWxBH/threads" into names. */
+int gocean_tile_configs(const char** names, int max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GOCEAN_H */

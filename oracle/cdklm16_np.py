@@ -1,0 +1,718 @@
+"""CPU oracle for the CDKLM16 shallow-water step (TEST INFRASTRUCTURE ONLY).
+
+This module is the checker, never the product: only ``tests/``, the smoke
+check in ``__graft_entry__.py`` and the ``cpu_baseline`` leg of ``bench.py``
+import it.  The shipped path is the HIP library under ``gpu-ocean_amd/csrc``;
+it never falls back to anything in this directory.
+
+It restates, vectorised over the whole ghost-padded grid with NumPy, the
+arithmetic of the reference implementation (paths relative to
+``gpu_ocean/SWESimulators/``):
+
+* ``gpu_kernels/CDKLM16_kernel.cu:80-1006``  -- one RK substep (``cdklm_swe_2D``)
+* ``gpu_kernels/common.cu:532-551,616-650,822-824`` -- minmod, wind, desingularize
+* ``gpu_kernels/max_dt.cu:42-194``          -- CFL time step
+* ``gpu_kernels/initBm_kernel.cu:38-107``   -- corner depth -> cell depth
+* ``gpu_kernels/boundary_kernels.cu:35-246`` -- periodic / closed ghost cells
+* ``CDKLM16.py:494-598,712-759``            -- host step loop, dt update
+
+Every operation keeps the reference's operand order and rounding points
+(the reference compiles with ``--fmad=false``), so in float32 it yields the
+IEEE-correctly-rounded value of each expression, and the same code run in
+float64 is the error-growth oracle.
+
+Tiling.  The reference evaluates the stencil per thread block on a shared
+memory tile whose wall ghost cells are mirrored inside the tile
+(``CDKLM16_kernel.cu:373-424``).  For every in-domain cell that is the same as
+mirroring the ghost rows/columns of the padded array, which is what this
+module does.  The only exception is a reference quirk: when ``(ny-1)`` is a
+multiple of the block height (or ``(nx-1)`` of the block width) one block reads
+the unmirrored ghost row from global memory.  ``tiled_reference_substep`` in
+``oracle/cdklm16_tiled.py`` emulates the block structure literally and is used
+by the tests to pin this vectorised form on small grids.
+
+Parity pins (see tests/test_oracle_*.py): the 13 ``timestep50`` golden runs of
+the reference's own test-suite, the literal tiled emulation, the exact
+``_getMaxTimestepHost`` closed form, lake-at-rest well-balancing, mass
+conservation, mirror/transpose symmetry and periodic roll equivariance.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+from typing import Optional, Tuple
+
+import numpy as np
+
+# CDKLM16_kernel.cu:37-38, max_dt.cu:31-36
+DRY_FLAG = 1.0e-30
+DRY_EPS = 1.0e-3
+# max_dt.cu:29 -- sentinel used for dry/land cells in the dt reduction
+DT_SENTINEL = 100000.0
+LAND_VALUE = 1.0e20  # Common.py:1331 (Bathymetry.mask_value)
+
+WALL, PERIODIC = 1, 2
+
+
+@dataclasses.dataclass
+class Params:
+    """Compile-time constants of ``cdklm_swe_2D`` (CDKLM16.py:179-191) plus the
+    texture-bound fields (Coriolis f, north vector, wind stress)."""
+
+    nx: int
+    ny: int
+    dx: float
+    dy: float
+    g: float = 9.81
+    r: float = 0.0
+    theta: float = 1.3
+    rk_order: int = 2
+    desing_eps: float = 0.1
+    flux_slope_eps: float = 0.1
+    depth_cutoff: float = 1.0e-5
+    # boundary conditions, north, east, south, west (Common.py:829-843)
+    bc: Tuple[int, int, int, int] = (WALL, WALL, WALL, WALL)
+    # Coriolis parameter: scalar or per-cell (ny+4, nx+4) array
+    f: object = 0.0
+    # north vector (sin(angle), cos(angle)): scalars or per-cell arrays
+    north_x: object = 0.0
+    north_y: object = 1.0
+    # uniform wind stress (already divided by rho_water), current and next
+    wind_x: Tuple[float, float] = (0.0, 0.0)
+    wind_y: Tuple[float, float] = (0.0, 0.0)
+    land_value: float = LAND_VALUE
+
+    @property
+    def bc_north(self):
+        return self.bc[0]
+
+    @property
+    def bc_east(self):
+        return self.bc[1]
+
+    @property
+    def bc_south(self):
+        return self.bc[2]
+
+    @property
+    def bc_west(self):
+        return self.bc[3]
+
+
+class _C:
+    """The reference's ``#define`` constants rounded to the working dtype."""
+
+    def __init__(self, p: Params, dt):
+        t = dt
+        self.t = t
+        self.DX = t(p.dx)
+        self.DY = t(p.dy)
+        self.GRAV = t(p.g)
+        self.FRIC = t(p.r)
+        self.THETA = t(p.theta)
+        self.DESING = t(p.desing_eps)
+        self.FLUX_EPS = t(p.flux_slope_eps)
+        self.CUTOFF = t(p.depth_cutoff)
+        self.TWO_G = t(2.0) * self.GRAV
+        self.HALF_G = t(0.5) * self.GRAV
+        self.THETA_G = self.THETA * self.GRAV
+        self.HALF = t(0.5)
+        self.QUARTER = t(0.25)
+        self.ZERO = t(0.0)
+        self.ONE = t(1.0)
+        self.TWO = t(2.0)
+        self.DRY_FLAG = t(DRY_FLAG)
+        self.DRY_EPS = t(DRY_EPS)
+        self.LAND = t(p.land_value)
+
+
+def _as_field(v, shape, dtype):
+    a = np.asarray(v, dtype=dtype)
+    if a.ndim == 0:
+        return np.full(shape, a, dtype=dtype)
+    assert a.shape == shape, (a.shape, shape)
+    return a
+
+
+# ---------------------------------------------------------------------------
+# device helpers
+# ---------------------------------------------------------------------------
+
+def minmod_raw(b, c, f, C):
+    """common.cu:532-551 -- 0.25*s(b)*(s(b)+s(c))*(s(c)+s(f))*min(|b|,|c|,|f|)."""
+    one = C.ONE
+    sb = np.copysign(one, b)
+    sc = np.copysign(one, c)
+    sf = np.copysign(one, f)
+    m = np.minimum(np.minimum(np.abs(b), np.abs(c)), np.abs(f))
+    return ((C.QUARTER * sb) * (sb + sc)) * (sc + sf) * m
+
+
+def minmod_slope(left, center, right, C):
+    """common.cu:532-542"""
+    backward = (center - left) * C.THETA
+    central = (right - left) * C.HALF
+    forward = (right - center) * C.THETA
+    return minmod_raw(backward, central, forward, C)
+
+
+def desingularize(h, hu, C):
+    """common.cu:822-824.  ``0.5*eps`` is a double literal in the reference, so
+    the float32 path evaluates that one addition in double."""
+    t = C.t
+    a = (h * h) / (C.TWO * C.DESING)
+    if t is np.float32:
+        a = (a.astype(np.float64) + 0.5 * np.float64(C.DESING)).astype(np.float32)
+    else:
+        a = a + t(0.5) * C.DESING
+    den = np.maximum(np.minimum(a, C.DESING), np.abs(h))
+    return hu / den
+
+
+def flux_cu(hm, um, vm, hp, up, vp, C):
+    """CDKLM16_kernel.cu:80-140 (CDKLM16_F_func + CDKLM16_flux) for the
+    primitive face states Qm=(hm, um, vm), Qp=(hp, up, vp)."""
+    z = C.ZERO
+    wet_p = hp > C.CUTOFF
+    wet_m = hm > C.CUTOFF
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        Fp_x = np.where(wet_p, hp * up, z)
+        Fp_y = np.where(wet_p, (hp * up) * up + (C.HALF_G * hp) * hp, z)
+        Fp_z = np.where(wet_p, (hp * up) * vp, z)
+        u_p = np.where(wet_p, up, z)
+        c_p = np.where(wet_p, np.sqrt(C.GRAV * np.where(wet_p, hp, z)), z)
+        Fm_x = np.where(wet_m, hm * um, z)
+        Fm_y = np.where(wet_m, (hm * um) * um + (C.HALF_G * hm) * hm, z)
+        Fm_z = np.where(wet_m, (hm * um) * vm, z)
+        u_m = np.where(wet_m, um, z)
+        c_m = np.where(wet_m, np.sqrt(C.GRAV * np.where(wet_m, hm, z)), z)
+
+        am = np.minimum(np.minimum(u_m - c_m, u_p - c_p), z)
+        ap = np.maximum(np.maximum(u_m + c_m, u_p + c_p), z)
+        den = ap - am
+        zero_flux = np.abs(ap - am) < C.FLUX_EPS
+        den_s = np.where(zero_flux, C.ONE, den)
+        apam = ap * am
+        Fx = ((ap * Fm_x - am * Fp_x) + apam * (hp - hm)) / den_s
+        Fy = ((ap * Fm_y - am * Fp_y) + apam * (Fp_x - Fm_x)) / den_s
+        Fz = np.where(um + up > z, Fm_z, Fp_z)
+    Fx = np.where(zero_flux, z, Fx)
+    Fy = np.where(zero_flux, z, Fy)
+    Fz = np.where(zero_flux, z, Fz)
+    return Fx, Fy, Fz
+
+
+# ---------------------------------------------------------------------------
+# bathymetry and ghost cells
+# ---------------------------------------------------------------------------
+
+def bathymetry_halo(Hi, p: Params):
+    """Common.py:1436-1459 -> boundary_kernels.cu:121-246.  Fills the two-deep
+    halo of the corner depths ``Hi`` ((ny+5, nx+5)), north/south first."""
+    Hi = np.array(Hi, copy=True)
+    nx, ny = p.nx, p.ny
+    if p.bc_north == PERIODIC and p.bc_south == PERIODIC:
+        # periodic_boundary_intersections_NS :121-148
+        Hi[0:2, :] = Hi[ny:ny + 2, :]
+        Hi[ny + 2:ny + 5, :] = Hi[2:5, :]
+    else:
+        # closed_boundary_intersections_NS :215-246
+        for j in range(2):
+            Hi[j, :] = Hi[4 - j, :]
+        for j in range(2):
+            Hi[ny + 4 - j, :] = Hi[ny + j, :]
+    if p.bc_east == PERIODIC and p.bc_west == PERIODIC:
+        # periodic_boundary_intersections_EW :151-178
+        Hi[:, 0:2] = Hi[:, nx:nx + 2]
+        Hi[:, nx + 2:nx + 5] = Hi[:, 2:5]
+    else:
+        # closed_boundary_intersections_EW :187-212
+        for i in range(2):
+            Hi[:, i] = Hi[:, 4 - i]
+        for i in range(2):
+            Hi[:, nx + 4 - i] = Hi[:, nx + i]
+    return Hi
+
+
+def init_bm(Hi, land_value=LAND_VALUE, dtype=np.float32):
+    """initBm_kernel.cu:38-107: mean of the wet corners, summed ((a+b)+c)+d,
+    ``land_value`` where all four corners are land."""
+    t = dtype
+    Hi = np.asarray(Hi, dtype=t)
+    dry = np.abs(Hi - t(land_value)) <= t(DRY_EPS)
+    B = np.where(dry, t(DRY_FLAG), Hi)
+    a = B[:-1, :-1]
+    b = B[1:, :-1]
+    c = B[:-1, 1:]
+    d = B[1:, 1:]
+    wet = (t(4.0) - (a == t(DRY_FLAG)).astype(t) - (b == t(DRY_FLAG)).astype(t)
+           - (c == t(DRY_FLAG)).astype(t) - (d == t(DRY_FLAG)).astype(t))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        Hm = (((a + b) + c) + d) / wet
+    return np.where(wet == t(0.0), t(land_value), Hm).astype(t)
+
+
+def periodic_bc(eta, hu, hv, p: Params):
+    """BoundaryConditionsArakawaA.boundaryCondition (Common.py:1219-1239) with
+    periodicBoundary_NS/EW (boundary_kernels.cu:35-110); in place."""
+    nx, ny = p.nx, p.ny
+    for a in (eta, hu, hv):
+        if p.bc_north == PERIODIC:
+            a[0:2, :] = a[ny:ny + 2, :]
+            a[ny + 2:ny + 4, :] = a[2:4, :]
+        if p.bc_east == PERIODIC:
+            a[:, 0:2] = a[:, nx:nx + 2]
+            a[:, nx + 2:nx + 4] = a[:, 2:4]
+
+
+# ---------------------------------------------------------------------------
+# one RK substep
+# ---------------------------------------------------------------------------
+
+def cdklm_substep(dt, rk_step, eta_in, hu_in, hv_in, eta_out, hu_out, hv_out,
+                  Hi, Hm, p: Params, wind_t=0.0, dtype=np.float32):
+    """One call of ``cdklm_swe_2D`` (CDKLM16_kernel.cu:429-1006).
+
+    Returns the new (eta, hu, hv) padded arrays.  Like the kernel only the
+    interior cells are written; the rest is copied from ``*_out`` (or, for the
+    RK3 middle stage, which the reference writes into the *input* buffers,
+    from ``*_in``).  ``*_out`` holds Q^n on entry for rk_step >= 1.
+    """
+    t = dtype
+    C = _C(p, t)
+    nx, ny = p.nx, p.ny
+    NY4, NX4 = ny + 4, nx + 4
+    shape = (NY4, NX4)
+    dt = t(dt)
+
+    eta_in = np.asarray(eta_in, dtype=t)
+    hu_in = np.asarray(hu_in, dtype=t)
+    hv_in = np.asarray(hv_in, dtype=t)
+    Hi = np.asarray(Hi, dtype=t)
+    Hm = np.asarray(Hm, dtype=t)
+    fc = _as_field(p.f, shape, t)
+    nox = _as_field(p.north_x, shape, t)
+    noy = _as_field(p.north_y, shape, t)
+
+    R0 = eta_in.copy()
+    R1 = hu_in.copy()
+    R2 = hv_in.copy()
+
+    # --- wall BCs mirrored into the ghost cells (handleWallBC :373-424) ---
+    if p.bc_north == WALL:
+        R0[ny + 2, :] = R0[ny + 1, :]; R1[ny + 2, :] = R1[ny + 1, :]; R2[ny + 2, :] = -R2[ny + 1, :]
+        R0[ny + 3, :] = R0[ny, :];     R1[ny + 3, :] = R1[ny, :];     R2[ny + 3, :] = -R2[ny, :]
+    if p.bc_south == WALL:
+        R0[1, :] = R0[2, :]; R1[1, :] = R1[2, :]; R2[1, :] = -R2[2, :]
+        R0[0, :] = R0[3, :]; R1[0, :] = R1[3, :]; R2[0, :] = -R2[3, :]
+    if p.bc_east == WALL:
+        R0[:, nx + 2] = R0[:, nx + 1]; R1[:, nx + 2] = -R1[:, nx + 1]; R2[:, nx + 2] = R2[:, nx + 1]
+        R0[:, nx + 3] = R0[:, nx];     R1[:, nx + 3] = -R1[:, nx];     R2[:, nx + 3] = R2[:, nx]
+    if p.bc_west == WALL:
+        R0[:, 1] = R0[:, 2]; R1[:, 1] = -R1[:, 2]; R2[:, 1] = R2[:, 2]
+        R0[:, 0] = R0[:, 3]; R1[:, 0] = -R1[:, 3]; R2[:, 0] = R2[:, 3]
+
+    # --- corner depths with land flagged (:509-525) ---
+    Hif = np.where(np.abs(Hi - C.LAND) < C.DRY_EPS, C.DRY_FLAG, Hi)
+
+    # --- (eta, hu, hv) -> (eta, u, v) (:551-592) ---
+    h = R0 + Hm
+    land = np.abs(Hm - C.LAND) <= C.DRY_EPS
+    shallow = (~land) & (h < C.DESING)
+    dry = shallow & (h <= C.CUTOFF)
+    desing = shallow & ~dry
+    wetc = (~land) & ~shallow
+    with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+        u_w = R1 / h
+        v_w = R2 / h
+        u_d = desingularize(h, R1, C)
+        v_d = desingularize(h, R2, C)
+    U = np.where(wetc, u_w, np.where(desing, u_d, C.ZERO))
+    V = np.where(wetc, v_w, np.where(desing, v_d, C.ZERO))
+    E = np.where(land, C.DRY_FLAG, np.where(dry, (-Hm) + C.CUTOFF, R0))
+    U = U.astype(t)
+    V = V.astype(t)
+    E = E.astype(t)
+
+    I = slice(2, nx + 2)   # interior columns
+    J = slice(2, ny + 2)   # interior rows
+
+    # register Hm from the (land-flagged) corner depths (:547)
+    Hm_reg = C.QUARTER * (((Hif[2:ny + 2, 2:nx + 2] + Hif[3:ny + 3, 2:nx + 2])
+                            + Hif[2:ny + 2, 3:nx + 3]) + Hif[3:ny + 3, 3:nx + 3])
+    # momentum used by the update (:597-602)
+    Ec = E[J, I]
+    hc_reg = Ec + Hm_reg
+    with np.errstate(over="ignore", invalid="ignore"):
+        hu_c = np.where(hc_reg > C.CUTOFF, U[J, I] * hc_reg, C.ZERO).astype(t)
+        hv_c = np.where(hc_reg > C.CUTOFF, V[J, I] * hc_reg, C.ZERO).astype(t)
+
+    # ---------------- x direction ----------------
+    # slope centres: rows J, columns 1..nx+2 (:610-665)
+    Xc = slice(1, nx + 3)
+    Xl = slice(0, nx + 2)
+    Xr = slice(2, nx + 4)
+    ux = minmod_slope(U[J, Xl], U[J, Xc], U[J, Xr], C)
+    vx = minmod_slope(V[J, Xl], V[J, Xc], V[J, Xr], C)
+    # v of wall ghost columns is negated for the geostrophic term (:630-641)
+    VK = V.copy()
+    if p.bc_west == WALL:
+        VK[:, 0:2] = -VK[:, 0:2]
+    if p.bc_east == WALL:
+        VK[:, nx + 2:nx + 4] = -VK[:, nx + 2:nx + 4]
+    n_x = nox[J, Xc]
+    n_y = noy[J, Xc]
+    fv_l = (n_x * U[J, Xl] + n_y * VK[J, Xl]) * fc[J, Xl]
+    fv_c = (n_x * U[J, Xc] + n_y * VK[J, Xc]) * fc[J, Xc]
+    fv_r = (n_x * U[J, Xr] + n_y * VK[J, Xr]) * fc[J, Xr]
+    Vc = C.DX / C.TWO_G
+    el, ec, er = E[J, Xl], E[J, Xc], E[J, Xr]
+    backward = C.THETA_G * ((ec - el) - Vc * (fv_c + fv_l))
+    central = C.HALF_G * ((er - el) - Vc * ((fv_r + C.TWO * fv_c) + fv_l))
+    forward = C.THETA_G * ((er - ec) - Vc * (fv_c + fv_r))
+    Kx = minmod_raw(backward, central, forward, C)
+
+    # adjustSlopes_x (:148-191); v flipped only in the west ghost column 1
+    v_adj = V[J, Xc].copy()
+    if p.bc_west == WALL:
+        v_adj[:, 0] = -v_adj[:, 0]      # global column 1
+    dxfv = (C.DX * fc[J, Xc]) * v_adj
+    H_west = C.HALF * (Hif[2:ny + 2, 1:nx + 3] + Hif[3:ny + 3, 1:nx + 3])
+    H_east = C.HALF * (Hif[2:ny + 2, 2:nx + 4] + Hif[3:ny + 3, 2:nx + 4])
+    h_west = (ec + H_west) - ((Kx + dxfv) / C.TWO_G)
+    h_east = (ec + H_east) + ((Kx + dxfv) / C.TWO_G)
+    Kx = np.where(h_west > C.ZERO, Kx, (-dxfv) + C.TWO_G * (ec + H_west))
+    Kx = np.where(h_east > C.ZERO, Kx, (-dxfv) - C.TWO_G * (ec + H_east))
+
+    # face fluxes.  Slope arrays are indexed by column-1 (Xc starts at 1).
+    def f_face(cm, north_cells):
+        """Flux through the face between column cm and cm+1 (rows J), using the
+        north vector of ``north_cells`` (the evaluating thread's cell)
+        (computeFFaceFlux :246-304)."""
+        cp = cm + 1
+        eta_p = E[J, cp]; eta_m = E[J, cm]
+        up = U[J, cp]; um = U[J, cm]
+        vp = V[J, cp]; vm = V[J, cm]
+        Rp_x = up - C.HALF * ux[:, cp - 1]
+        Rp_y = vp - C.HALF * vx[:, cp - 1]
+        Rm_x = um + C.HALF * ux[:, cm - 1]
+        Rm_y = vm + C.HALF * vx[:, cm - 1]
+        H_face = C.HALF * (Hif[2:ny + 2, cp] + Hif[3:ny + 3, cp])
+        Kp = Kx[:, cp - 1]; Km = Kx[:, cm - 1]
+        vm_f = vm.copy(); vp_f = vp.copy()
+        if p.bc_west == WALL:
+            vm_f = np.where(cp == 2, -vm_f, vm_f)
+        if p.bc_east == WALL:
+            vp_f = np.where(cp == nx + 2, -vp_f, vp_f)
+        nxv = nox[J, north_cells]; nyv = noy[J, north_cells]
+        vp_north = up * nxv + vp_f * nyv
+        vm_north = um * nxv + vm_f * nyv
+        hp = np.maximum(C.ZERO, (eta_p + H_face) - ((Kp + (C.DX * fc[J, cp]) * vp_north) / C.TWO_G))
+        hm = np.maximum(C.ZERO, (eta_m + H_face) + ((Km + (C.DX * fc[J, cm]) * vm_north) / C.TWO_G))
+        F = flux_cu(hm, Rm_x, Rm_y, hp, Rp_x, Rp_y, C)
+        dryf = (eta_p == C.DRY_FLAG) | (eta_m == C.DRY_FLAG)
+        return tuple(np.where(dryf, C.ZERO, x) for x in F)
+
+    cols = np.arange(2, nx + 2)
+    # east face of each interior cell evaluated with that cell's north vector,
+    # west face likewise (:688-701)
+    Fe = f_face(cols, cols)
+    Fw = f_face(cols - 1, cols)
+    fdx = tuple((Fe[k] - Fw[k]) / C.DX for k in range(3))
+
+    # eta on west/east faces for the bathymetry source term (:706-707)
+    Kc = Kx[:, 1:nx + 1]
+    tmpx = (Kc + (C.DX * fc[J, I]) * V[J, I]) / C.TWO_G
+    eta_west = Ec - tmpx
+    eta_east = Ec + tmpx
+
+    # ---------------- y direction ----------------
+    Yc = slice(1, ny + 3)
+    Yl = slice(0, ny + 2)
+    Yu = slice(2, ny + 4)
+    uy = minmod_slope(U[Yl, I], U[Yc, I], U[Yu, I], C)
+    vy = minmod_slope(V[Yl, I], V[Yc, I], V[Yu, I], C)
+    UK = U.copy()
+    if p.bc_south == WALL:
+        UK[0:2, :] = -UK[0:2, :]
+    if p.bc_north == WALL:
+        UK[ny + 2:ny + 4, :] = -UK[ny + 2:ny + 4, :]
+    e_x = noy[Yc, I]
+    e_y = -nox[Yc, I]
+    fu_lo = (e_x * UK[Yl, I] + e_y * V[Yl, I]) * fc[Yl, I]
+    fu_c = (e_x * UK[Yc, I] + e_y * V[Yc, I]) * fc[Yc, I]
+    fu_up = (e_x * UK[Yu, I] + e_y * V[Yu, I]) * fc[Yu, I]
+    Uc = C.DY / C.TWO_G
+    elo, ecy, eup = E[Yl, I], E[Yc, I], E[Yu, I]
+    backward = C.THETA_G * ((ecy - elo) + Uc * (fu_c + fu_lo))
+    central = C.HALF_G * ((eup - elo) + Uc * ((fu_up + C.TWO * fu_c) + fu_lo))
+    forward = C.THETA_G * ((eup - ecy) + Uc * (fu_c + fu_up))
+    Ly = minmod_raw(backward, central, forward, C)
+
+    # adjustSlopes_y (:198-241); u flipped only in the south ghost row 1
+    u_adj = U[Yc, I].copy()
+    if p.bc_south == WALL:
+        u_adj[0, :] = -u_adj[0, :]
+    dyfu = (C.DY * fc[Yc, I]) * u_adj
+    H_south = C.HALF * (Hif[1:ny + 3, 2:nx + 2] + Hif[1:ny + 3, 3:nx + 3])
+    H_north = C.HALF * (Hif[2:ny + 4, 2:nx + 2] + Hif[2:ny + 4, 3:nx + 3])
+    h_south = (ecy + H_south) - ((Ly - dyfu) / C.TWO_G)
+    h_north = (ecy + H_north) + ((Ly - dyfu) / C.TWO_G)
+    Ly = np.where(h_south > C.ZERO, Ly, dyfu + C.TWO_G * (ecy + H_south))
+    Ly = np.where(h_north > C.ZERO, Ly, dyfu - C.TWO_G * (ecy + H_north))
+
+    def g_face(rm, north_rows):
+        """Flux through the face between row rm and rm+1 (columns I)
+        (computeGFaceFlux :309-370)."""
+        rp = rm + 1
+        eta_p = E[rp, I]; eta_m = E[rm, I]
+        up = U[rp, I]; um = U[rm, I]
+        vp = V[rp, I]; vm = V[rm, I]
+        Rp_x = up - C.HALF * uy[rp - 1, :]
+        Rp_y = vp - C.HALF * vy[rp - 1, :]
+        Rm_x = um + C.HALF * uy[rm - 1, :]
+        Rm_y = vm + C.HALF * vy[rm - 1, :]
+        H_face = C.HALF * (Hif[rp, 2:nx + 2] + Hif[rp, 3:nx + 3])
+        Lp = Ly[rp - 1, :]; Lm = Ly[rm - 1, :]
+        um_f = um.copy(); up_f = up.copy()
+        if p.bc_south == WALL:
+            um_f = np.where((rp == 2)[:, None], -um_f, um_f)
+        if p.bc_north == WALL:
+            up_f = np.where((rp == ny + 2)[:, None], -up_f, up_f)
+        ex = noy[north_rows, I]; ey = -nox[north_rows, I]
+        up_east = up_f * ex + vp * ey
+        um_east = um_f * ex + vm * ey
+        hp = np.maximum(C.ZERO, (eta_p + H_face) - ((Lp - (C.DY * fc[rp, I]) * up_east) / C.TWO_G))
+        hm = np.maximum(C.ZERO, (eta_m + H_face) + ((Lm - (C.DY * fc[rm, I]) * um_east) / C.TWO_G))
+        F = flux_cu(hm, Rm_y, Rm_x, hp, Rp_y, Rp_x, C)
+        dryf = (eta_p == C.DRY_FLAG) | (eta_m == C.DRY_FLAG)
+        Fx, Fy, Fz = (np.where(dryf, C.ZERO, x) for x in F)
+        return Fx, Fz, Fy
+
+    rows = np.arange(2, ny + 2)
+    Gn = g_face(rows, rows)
+    Gs = g_face(rows - 1, rows)
+    fd = tuple(fdx[k] + (Gn[k] - Gs[k]) / C.DY for k in range(3))
+
+    Lc = Ly[1:ny + 1, :]
+    tmpy = (Lc - (C.DY * fc[J, I]) * U[J, I]) / C.TWO_G
+    eta_south = Ec - tmpy
+    eta_north = Ec + tmpy
+
+    # ---------------- source terms (:817-867) ----------------
+    fcc = fc[J, I]
+    n_x = nox[J, I]; n_y = noy[J, I]
+    e_x = n_y; e_y = -n_x
+    wt = t(wind_t)
+    X = wt * t(p.wind_x[1]) + (C.ONE - wt) * t(p.wind_x[0])
+    Y = wt * t(p.wind_y[1]) + (C.ONE - wt) * t(p.wind_y[0])
+    RHxp = C.HALF * (Hif[2:ny + 2, 3:nx + 3] + Hif[3:ny + 3, 3:nx + 3])
+    RHxm = C.HALF * (Hif[2:ny + 2, 2:nx + 2] + Hif[3:ny + 3, 2:nx + 2])
+    RHyp = C.HALF * (Hif[3:ny + 3, 2:nx + 2] + Hif[3:ny + 3, 3:nx + 3])
+    RHym = C.HALF * (Hif[2:ny + 2, 2:nx + 2] + Hif[2:ny + 2, 3:nx + 3])
+    H_x = RHxp - RHxm
+    H_y = RHyp - RHym
+    eta_sn = C.HALF * (eta_north + eta_south)
+    eta_we = C.HALF * (eta_west + eta_east)
+    bathy1 = (C.GRAV * (eta_we + Hm_reg)) * H_x
+    bathy2 = (C.GRAV * (eta_sn + Hm_reg)) * H_y
+    hu_east = hu_c * e_x + hv_c * e_y
+    hv_north = hu_c * n_x + hv_c * n_y
+    hu_east_cor = fcc * hv_north
+    hv_north_cor = (-fcc) * hu_east
+    up_x = -n_x; up_y = n_y
+    right_x = up_y; right_y = -up_x
+    hu_cor = right_x * hu_east_cor + right_y * hv_north_cor
+    hv_cor = up_x * hu_east_cor + up_y * hv_north_cor
+    st1 = (X + hu_cor) + bathy1 / C.DX
+    st2 = (Y + hv_cor) + bathy2 / C.DY
+    hsrc = Ec + Hm_reg
+    active = (hsrc >= C.CUTOFF) & (Ec != C.DRY_FLAG)
+    st1 = np.where(active, st1, C.ZERO)
+    st2 = np.where(active, st2, C.ZERO)
+
+    L1 = -fd[0]
+    L2 = -fd[1] + st1
+    L3 = -fd[2] + st2
+
+    # ---------------- friction (:884-900) ----------------
+    if C.FRIC > 0:
+        with np.errstate(invalid="ignore", divide="ignore"):
+            uf = np.where(hsrc < C.DESING, desingularize(hsrc, hu_c, C), hu_c / hsrc)
+            vf = np.where(hsrc < C.DESING, desingularize(hsrc, hv_c, C), hv_c / hsrc)
+            sp = np.sqrt(uf * uf + vf * vf)
+            Cf_d = (dt * C.FRIC) * desingularize(hsrc, sp, C)
+            Cf_w = ((dt * C.FRIC) * sp) / hsrc
+        Cf = np.where(hsrc < C.DESING, Cf_d, Cf_w).astype(t)
+    else:
+        Cf = np.zeros_like(hsrc)
+
+    eta_o = np.asarray(eta_out, dtype=t)
+    hu_o = np.asarray(hu_out, dtype=t)
+    hv_o = np.asarray(hv_out, dtype=t)
+    rk = p.rk_order
+    if rk < 3:
+        if rk_step == 0:
+            ue = Ec + dt * L1
+            uu = (hu_c + dt * L2) / (C.ONE + Cf)
+            uv = (hv_c + dt * L3) / (C.ONE + Cf)
+        else:
+            ea = eta_o[J, I]; ha = hu_o[J, I]; va = hv_o[J, I]
+            eb = C.HALF * (ea + (Ec + dt * L1))
+            hb = C.HALF * (ha + (hu_c + dt * L2))
+            vb = C.HALF * (va + (hv_c + dt * L3))
+            ue = eb
+            uu = hb / (C.ONE + C.HALF * Cf)
+            uv = vb / (C.ONE + C.HALF * Cf)
+    else:
+        if rk_step == 0:
+            ue = Ec + dt * L1
+            uu = hu_c + dt * L2
+            uv = hv_c + dt * L3
+        elif rk_step == 1:
+            ea = eta_o[J, I]; ha = hu_o[J, I]; va = hv_o[J, I]
+            ue = t(0.75) * ea + t(0.25) * (Ec + dt * L1)
+            uu = t(0.75) * ha + t(0.25) * (hu_c + dt * L2)
+            uv = t(0.75) * va + t(0.25) * (hv_c + dt * L3)
+        else:
+            ea = eta_o[J, I]; ha = hu_o[J, I]; va = hv_o[J, I]
+            ue = (ea + C.TWO * (Ec + dt * L1)) / t(3.0)
+            uu = (ha + C.TWO * (hu_c + dt * L2)) / t(3.0)
+            uv = (va + C.TWO * (hv_c + dt * L3)) / t(3.0)
+
+    dry_after = (ue + Hm_reg) <= C.CUTOFF
+    ue = np.where(dry_after, (-Hm_reg) + C.CUTOFF, ue)
+    uu = np.where(dry_after, C.ZERO, uu)
+    uv = np.where(dry_after, C.ZERO, uv)
+    ue = np.maximum((-Hm_reg) + C.CUTOFF, ue)
+
+    if rk == 3 and rk_step == 1:
+        base = (eta_in, hu_in, hv_in)
+    else:
+        base = (eta_o, hu_o, hv_o)
+    res = [b.copy() for b in base]
+    res[0][J, I] = ue
+    res[1][J, I] = uu
+    res[2][J, I] = uv
+    return tuple(res)
+
+
+# ---------------------------------------------------------------------------
+# CFL time step
+# ---------------------------------------------------------------------------
+
+def max_dt(eta, hu, hv, Hm, p: Params, dtype=np.float32):
+    """Minimum over interior cells of per_block_max_dt's per-cell value
+    (max_dt.cu:42-141); the reduction (:147-194) is an exact minimum."""
+    t = dtype
+    nx, ny = p.nx, p.ny
+    J = slice(2, ny + 2); I = slice(2, nx + 2)
+    e = np.asarray(eta, t)[J, I]; H = np.asarray(Hm, t)[J, I]
+    qu = np.asarray(hu, t)[J, I]; qv = np.asarray(hv, t)[J, I]
+    h = e + H
+    skip = (h <= t(0.0)) | (np.abs(H - t(p.land_value)) < t(DRY_EPS))
+    with np.errstate(invalid="ignore", divide="ignore"):
+        u = qu / h
+        v = qv / h
+        gw = np.sqrt(t(p.g) * h)
+        qdx = t(0.25) * t(p.dx)
+        qdy = t(0.25) * t(p.dy)
+        d = qdx / np.abs(u + gw)
+        d = np.minimum(d, qdx / np.abs(u - gw))
+        d = np.minimum(d, qdy / np.abs(v + gw))
+        d = np.minimum(d, qdy / np.abs(v - gw))
+    d = np.where(skip, t(DT_SENTINEL), d)
+    if d.size == 0:
+        return t(DT_SENTINEL)
+    return t(np.min(d))
+
+
+def max_dt_host(eta, hu, hv, Hm, p: Params, courant_number=0.8):
+    """CDKLM16._getMaxTimestepHost (CDKLM16.py:741-759): the closed form the
+    reference's tests require updateDt() to reproduce exactly.  The final
+    products are evaluated in double, as NumPy 1.x did for ``float*float32``."""
+    J = slice(2, p.ny + 2); I = slice(2, p.nx + 2)
+    e = np.asarray(eta, np.float32)[J, I]
+    H = np.asarray(Hm, np.float32)[J, I]
+    h = e + H
+    gw = np.sqrt(np.float32(p.g) * h)
+    u = np.asarray(hu, np.float32)[J, I] / h
+    v = np.asarray(hv, np.float32)[J, I] / h
+    a = np.float32(p.dx) / np.max(np.abs(u) + gw)
+    b = np.float32(p.dy) / np.max(np.abs(v) + gw)
+    return float(courant_number) * (0.25 * float(min(a, b)))
+
+
+# ---------------------------------------------------------------------------
+# host driver
+# ---------------------------------------------------------------------------
+
+class OracleSim:
+    """Host loop of CDKLM16.step (CDKLM16.py:494-598) around the substep."""
+
+    def __init__(self, eta0, hu0, hv0, Hi, p: Params, dt, t=0.0,
+                 dtype=np.float32):
+        self.p = p
+        self.dtype = dtype
+        Hi_host = Hi
+        if np.ma.is_masked(Hi_host):
+            Hi_host = Hi_host.filled(p.land_value)
+        self.Hi = bathymetry_halo(np.asarray(Hi_host, dtype=np.float32), p).astype(dtype)
+        self.Hm = init_bm(self.Hi, p.land_value, dtype)
+        e0 = np.maximum(np.asarray(eta0, dtype), -self.Hm)      # CDKLM16.py:237-238
+        self.h0 = [e0.astype(dtype), np.asarray(hu0, dtype).copy(), np.asarray(hv0, dtype).copy()]
+        self.h1 = [x.copy() for x in self.h0]
+        self.t = t
+        self.dt = dt
+        self.num_iterations = 0
+        if self.dt <= 0:
+            self.update_dt()
+
+    def update_dt(self, courant_number=0.8):
+        """CDKLM16.updateDt (CDKLM16.py:712-739); product in double."""
+        m = max_dt(*self.h0, self.Hm, self.p, self.dtype)
+        self.dt = float(courant_number) * float(m)
+        return self.dt
+
+    def substep(self, rk_step, src, dst, local_dt):
+        out = cdklm_substep(local_dt, rk_step, *src, *dst, self.Hi, self.Hm,
+                            self.p, dtype=self.dtype)
+        return out
+
+    def step(self, t_end):
+        p = self.p
+        if self.t == 0:
+            periodic_bc(*self.h0, p)
+        t_now = 0.0
+        while t_now < t_end:
+            local_dt = np.float32(min(self.dt, np.float32(t_end - t_now)))
+            if p.rk_order == 2:
+                self.h1 = list(self.substep(0, self.h0, self.h1, local_dt))
+                periodic_bc(*self.h1, p)
+                self.h0 = list(self.substep(1, self.h1, self.h0, local_dt))
+            elif p.rk_order == 1:
+                self.h1 = list(self.substep(0, self.h0, self.h1, local_dt))
+                self.h0, self.h1 = self.h1, self.h0
+            else:
+                self.h1 = list(self.substep(0, self.h0, self.h1, local_dt))
+                periodic_bc(*self.h1, p)
+                # middle stage reads Q^n from h0 and writes into h1 (the input)
+                self.h1 = list(cdklm_substep(local_dt, 1, *self.h1, *self.h0, self.Hi, self.Hm,
+                                             p, dtype=self.dtype))
+                periodic_bc(*self.h1, p)
+                self.h0 = list(self.substep(2, self.h1, self.h0, local_dt))
+            periodic_bc(*self.h0, p)
+            self.t += np.float64(local_dt)
+            t_now += np.float64(local_dt)
+            self.num_iterations += 1
+        return self.t
+
+    def step_n(self, n, dt=None):
+        """n fixed-dt RK steps (the fixed-dt loop used by the batched path)."""
+        d = self.dt if dt is None else dt
+        for _ in range(n):
+            self.step(np.float32(d))
+        return self.t
+
+    def download(self):
+        return tuple(x.copy() for x in self.h0)

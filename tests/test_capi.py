@@ -1,0 +1,104 @@
+"""The C-ABI library loads, exports every symbol ``include/gocean.h`` declares,
+and rejects invalid arguments with a message -- all without a GPU (the
+argument checks run before any HIP call)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from SWESimulators import _gocean as G
+from SWESimulators import Common
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(inc, fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(gocean_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_built():
+    assert G.library_exists(), "run __graft_entry__.build() first"
+
+
+def test_every_declared_symbol_is_exported_and_bound():
+    lib = G.load()
+    decl = declared_symbols()
+    assert len(decl) >= 9
+    bound = {n for n, _, _ in G.PROTOTYPES}
+    for name in decl:
+        assert hasattr(lib, name), name
+        assert name in bound, f"{name} has no ctypes prototype"
+    assert bound <= decl
+
+
+def test_version_and_configs():
+    lib = G.load()
+    assert b"gfx950" in lib.gocean_version()
+    names = (ctypes.c_char_p * 32)()
+    n = lib.gocean_tile_configs(names, 32)
+    assert n >= 4
+    assert all(b"/" in names[i] for i in range(n))
+
+
+def _dom(**kw):
+    args = dict(nx=32, ny=20, dx=100.0, dy=100.0, g=9.81, r=0.0, theta=1.3, rk_order=2,
+                bc=(1, 1, 1, 1))
+    args.update(kw)
+    return Common.make_domain(**args)
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(nx=1), "too small"),
+    (dict(pitch=10), "pitch"),
+    (dict(bc=(3, 1, 1, 1)), "not supported"),
+    (dict(bc=(2, 1, 1, 1)), "paired"),
+    (dict(rk_order=4), "rk_order"),
+    (dict(rk_order=3, r=0.1), "r == 0"),
+    (dict(dx=0.0), "positive"),
+])
+def test_invalid_domain_rejected(kw, msg):
+    lib = G.load()
+    rc = lib.gocean_periodic_bc(_dom(**kw), 16, 16, 16, None)
+    assert rc == -1
+    assert msg in lib.gocean_last_error().decode()
+
+
+def test_null_and_alias_rejected():
+    lib = G.load()
+    d = _dom()
+    assert lib.gocean_periodic_bc(d, None, None, None, None) == -1
+    assert lib.gocean_bathymetry_init(d, None, None, None) == -1
+    assert lib.gocean_max_dt(d, None, None, None, None, 0.7, None, None) == -1
+    rc = lib.gocean_cdklm_substep(d, 1.0, None, 0, 16, 16, 16, None, None, None,
+                                  16, 32, 48, 64, None, None, -1, None)
+    assert rc == -1 and "alias" in lib.gocean_last_error().decode()
+    rc = lib.gocean_cdklm_substep(d, 1.0, None, 2, 16, 16, 16, None, None, None,
+                                  32, 32, 32, 64, None, None, -1, None)
+    assert rc == -1 and "rk_step" in lib.gocean_last_error().decode()
+    f = G.Fields()
+    f.f = 16
+    rc = lib.gocean_cdklm_substep(d, 1.0, None, 0, 16, 16, 16, None, None, None,
+                                  32, 32, 32, 64, f, None, -1, None)
+    assert rc == -1 and "together" in lib.gocean_last_error().decode()
+    rc = lib.gocean_cdklm_substep(d, 1.0, None, 0, 16, 16, 16, None, None, None,
+                                  32, 32, 32, 64, None, None, 99, None)
+    assert rc == -1 and "tile configuration" in lib.gocean_last_error().decode()
+
+
+def test_check_raises_with_message():
+    lib = G.load()
+    with pytest.raises(RuntimeError, match="too small"):
+        G.check(lib.gocean_periodic_bc(_dom(nx=1), 16, 16, 16, None), "periodic")
+
+
+def test_no_gpu_means_zero_devices_or_more():
+    assert G.load().gocean_device_count() >= 0
