@@ -87,6 +87,20 @@ def cpu_baseline(inp, n_steps=1):
     return inp.nx * inp.ny * n_steps / s / 1e6, s
 
 
+def measured_traffic(cells):
+    """HBM bytes per substep launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy), when
+    one exists for this domain size; else None."""
+    p = os.path.join(ROOT, "profiles", "cdklm16_substep_2048_traffic.json")
+    try:
+        d = json.load(open(p))
+    except (OSError, ValueError):
+        return None
+    if d.get("cells") != cells:
+        return None
+    return d.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,6 +109,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=None)
     ap.add_argument("--n", type=int, default=2048, help="domain side (headline: 2048)")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {a.gpus}); the CDKLM16 path "
@@ -109,7 +125,21 @@ def main():
     inp = B.config2(nx=a.n, ny=a.n)
     log(f"[bench] workload {inp.name}: {inp.nx}x{inp.ny}, steps {a.steps}, warmup {a.warmup}")
     sim = make_sim(inp, ctx, tile_config=a.tile)
+    if a.tile is None:
+        from SWESimulators import autotune
+        best, times = autotune.tune_simulator(sim)
+        names = autotune.tile_config_names()
+        log("[bench] autotune ms/step: " + ", ".join(f"{names[c]}={t:.4f}" for c, t in times.items())
+            + f" -> {names[best]}")
     wall, ev, launches = time_plain(sim, a.steps, a.warmup)
+    if a.calib_copy_mib:
+        n = a.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {a.calib_copy_mib} MiB read + {a.calib_copy_mib} MiB written per launch")
     cells = inp.nx * inp.ny
     value = cells * a.steps / wall / 1e6
     avg_launch = ev / launches
@@ -143,7 +173,8 @@ def main():
                    "nx": inp.nx, "ny": inp.ny, "cells": cells, "rk_order": 2,
                    "tile_config": sim.tile_config},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(cells)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1,0 +1,62 @@
+"""Tile-shape autotuner for the CDKLM16 substep kernels.
+
+Times every tile configuration the library is built with on the caller's own
+domain (a few fixed-dt steps each, HIP events on the simulator's stream),
+restores the state afterwards, and returns the fastest.  Results are cached
+per (nx, ny, boundary conditions) for the life of the process.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+from . import _gocean as G
+
+_cache = {}
+
+
+def tile_config_names():
+    lib = G.load()
+    names = (ctypes.c_char_p * 64)()
+    n = lib.gocean_tile_configs(names, 64)
+    return [names[i].decode() for i in range(n)]
+
+
+def tune_simulator(sim, steps=8, warmup=2, configs=None, use_cache=True):
+    """Return (best_config, {config: ms_per_step}) for ``sim`` (a CDKLM16).
+    The simulator's state, time and iteration count are left unchanged; its
+    ``tile_config`` is set to the winner."""
+    import torch
+    key = (int(sim.nx), int(sim.ny), tuple(sim.boundary_conditions.codes()),
+           sim.field_tensors is not None, (sim.north_const[0], sim.north_const[1]))
+    if use_cache and key in _cache:
+        sim.tile_config = _cache[key][0]
+        return _cache[key]
+    if configs is None:
+        configs = list(range(len(tile_config_names())))
+    gd = sim.gpu_data
+    arrays = [gd.h0, gd.hu0, gd.hv0, gd.h1, gd.hu1, gd.hv1]
+    st = sim.gpu_stream
+    with torch.cuda.stream(st):
+        saved = [a.tensor.clone() for a in arrays]
+    t_saved, it_saved = sim.t, sim.num_iterations
+    times = {}
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    for c in configs:
+        sim.tile_config = c
+        sim.step_n(warmup)
+        ev0.record(st)
+        sim.step_n(steps)
+        ev1.record(st)
+        ev1.synchronize()
+        times[c] = ev0.elapsed_time(ev1) / steps
+    with torch.cuda.stream(st):
+        for a, s in zip(arrays, saved):
+            a.tensor.copy_(s)
+    st.synchronize()
+    sim.t, sim.num_iterations = t_saved, it_saved
+    best = min(times, key=times.get)
+    sim.tile_config = best
+    _cache[key] = (best, times)
+    return best, times

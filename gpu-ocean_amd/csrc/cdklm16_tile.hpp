@@ -3,9 +3,9 @@
 // Device code for gfx950.  The arithmetic restates
 // gpu_ocean/SWESimulators/gpu_kernels/CDKLM16_kernel.cu:80-1006 and
 // common.cu:532-551,822-824 with the reference's operand order and rounding
-// points (the reference builds with --fmad=false; this file must be built with
-// -ffp-contract=off and IEEE division / square root), so that a float32 result
-// equals the reference's on the same inputs.
+// points (the reference builds with --fmad=false; this file must be built
+// with -ffp-contract=off), and every division and square root is correctly
+// rounded, so that a float32 result equals the reference's on the same inputs.
 //
 // The structure is this library's own:
 //   * one workgroup owns a BW x BH tile of output cells; the state tile with a
@@ -77,10 +77,48 @@ __device__ __forceinline__ float desingularize(float h, float hu, float eps) {
     return hu / fmaxf(fminf(b, eps), fabsf(h));
 }
 
+// Correctly rounded float division and square root without the range
+// scaling of the generic library sequences.  The scaled forms only differ for
+// operands near the overflow / denormal boundaries, which this scheme never
+// produces (depths >= 1e-5, speeds and fluxes far inside the normal range).
+//
+// rcp_refined: hardware reciprocal plus one Newton step, the same reciprocal
+// the generic division sequence builds.
+__device__ __forceinline__ float rcp_refined(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    const float e = fmaf(-d, r, 1.0f);
+    return fmaf(e, r, r);
+}
+
+// n/d from a reciprocal r of d that is RN(1/d) or rcp_refined(d): two
+// residual corrections (Markstein), identical to the core of the generic
+// sequence, give the correctly rounded quotient.
+__device__ __forceinline__ float div_r(float n, float d, float r) {
+    float q = n * r;
+    float e = fmaf(-d, q, n);
+    q = fmaf(e, r, q);
+    e = fmaf(-d, q, n);
+    return fmaf(e, r, q);
+}
+
+// correctly rounded sqrt for x == 0 or x >= 2^-96: hardware estimate, then
+// pick among the two neighbours by the sign of the exact residual.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+    const float vm = fmaf(-sm, s, x);
+    const float vp = fmaf(-sp, s, x);
+    s = (vm <= 0.0f) ? sm : s;
+    s = (vp > 0.0f) ? sp : s;
+    return s;
+}
+
 struct Consts {
     float g, half_g, two_g, theta, theta_g;
     float dx, dy, vdx, udy;        // vdx = DX/(2g), udy = DY/(2g)
     float desing, flux_eps, cutoff, land;
+    float r_two_g, r_dx, r_dy;     // RN(1/(2g)), RN(1/dx), RN(1/dy)
 };
 
 // Central-upwind flux of primitive face states Qm=(hm,um,vm), Qp=(hp,up,vp)
@@ -96,7 +134,7 @@ __device__ __forceinline__ void cu_flux(float hm, float um, float vm,
         Fpy = q * up + (K.half_g * hp) * hp;
         Fpz = q * vp;
         u_p = up;
-        c_p = sqrtf(K.g * hp);
+        c_p = sqrt_rn(K.g * hp);
     }
     float Fmx = 0.0f, Fmy = 0.0f, Fmz = 0.0f, u_m = 0.0f, c_m = 0.0f;
     if (hm > K.cutoff) {
@@ -105,7 +143,7 @@ __device__ __forceinline__ void cu_flux(float hm, float um, float vm,
         Fmy = q * um + (K.half_g * hm) * hm;
         Fmz = q * vm;
         u_m = um;
-        c_m = sqrtf(K.g * hm);
+        c_m = sqrt_rn(K.g * hm);
     }
     const float am = fminf(fminf(u_m - c_m, u_p - c_p), 0.0f);
     const float ap = fmaxf(fmaxf(u_m + c_m, u_p + c_p), 0.0f);
@@ -115,25 +153,40 @@ __device__ __forceinline__ void cu_flux(float hm, float um, float vm,
         return;
     }
     const float apam = ap * am;
-    Fx = ((ap * Fmx - am * Fpx) + apam * (hp - hm)) / den;
-    Fy = ((ap * Fmy - am * Fpy) + apam * (Fpx - Fmx)) / den;
+    const float rden = rcp_refined(den);
+    Fx = div_r((ap * Fmx - am * Fpx) + apam * (hp - hm), den, rden);
+    Fy = div_r((ap * Fmy - am * Fpy) + apam * (Fpx - Fmx), den, rden);
     Fz = (um + up > 0.0f) ? Fmz : Fpz;
 }
 
 // ---------------------------------------------------------------------------
 // LDS layout of one tile
 // ---------------------------------------------------------------------------
-template <int BW, int BH, bool FIELD>
+// MODE selects how the Coriolis parameter and the north vector enter:
+//   kModeZero  constant f, north = (0, 1) (zero grid rotation): with this
+//              north vector the reference's face term (K + dx*f*v_north)/2g
+//              equals the cell's source-term expression (K + dx*f*v)/2g, so it
+//              is computed once per cell (T, S below) instead of per face;
+//   kModeConst constant f and any constant north vector;
+//   kModeField per-cell f and north (the reference's textures).  The
+//              reference evaluates each face with the north vector of the
+//              thread's own cell, so the two neighbours of a face may disagree;
+//              both evaluations are kept.
+enum { kModeZero = 0, kModeConst = 1, kModeField = 2 };
+
+template <int BW, int BH, int MODE>
 struct TileSmem {
+    static constexpr bool FIELD = MODE == kModeField;
     float Hi[BH + 5][BW + 5];          // corner depths, land flagged
     float E[BH + 4][BW + 4];           // eta (dry / land adjusted)
     float U[BH + 4][BW + 4];           // u
     float V[BH + 4][BW + 4];           // v
+    // x slopes of u, v and K (kModeZero: T = (K + dx f v_flip)/2g instead of K)
     float Ux[BH][BW + 2], Vx[BH][BW + 2], Kx[BH][BW + 2];
+    // y slopes of u, v and L (kModeZero: S = (L - dy f u_flip)/2g instead of L)
     float Uy[BH + 2][BW], Vy[BH + 2][BW], Ly[BH + 2][BW];
-    float F[3][BH][BW + 1];            // x faces (evaluated with the west cell's north)
+    float F[3][BH][BW + 1];            // x faces (west cell's north vector)
     float G[3][BH + 1][BW];            // y faces (south cell's east vector)
-    // per-cell Coriolis / north fields and the second face evaluation they need
     float Fc[FIELD ? BH + 4 : 1][FIELD ? BW + 4 : 1];
     float Nx[FIELD ? BH + 4 : 1][FIELD ? BW + 4 : 1];
     float Ny[FIELD ? BH + 4 : 1][FIELD ? BW + 4 : 1];
@@ -150,17 +203,7 @@ __device__ __forceinline__ int xcd_band(int b, int nb) {
     return (xcd < rem) ? xcd * (q + 1) + idx : rem * (q + 1) + (xcd - rem) * q + idx;
 }
 
-// ---------------------------------------------------------------------------
-// the tile
-// ---------------------------------------------------------------------------
-template <int BW, int BH, int NT, bool FIELD>
-__device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int tile_y,
-                                           TileSmem<BW, BH, FIELD>& S) {
-    const int tid = threadIdx.x;
-    const int nx = A.nx, ny = A.ny, pitch = A.pitch;
-    const int bx = tile_x * BW;     // padded column of the tile's first halo column
-    const int by = tile_y * BH;     // padded row of the tile's first halo row
-
+__device__ __forceinline__ Consts make_consts(const SubstepArgs& A) {
     Consts K;
     K.g = A.g;
     K.half_g = 0.5f * A.g;
@@ -175,6 +218,25 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
     K.flux_eps = A.flux_eps;
     K.cutoff = A.cutoff;
     K.land = A.land;
+    K.r_two_g = 1.0f / K.two_g;
+    K.r_dx = 1.0f / A.dx;
+    K.r_dy = 1.0f / A.dy;
+    return K;
+}
+
+// ---------------------------------------------------------------------------
+// the tile
+// ---------------------------------------------------------------------------
+template <int BW, int BH, int NT, int MODE>
+__device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int tile_y,
+                                           TileSmem<BW, BH, MODE>& S) {
+    constexpr bool FIELD = MODE == kModeField;
+    constexpr bool ZERO = MODE == kModeZero;
+    const int tid = threadIdx.x;
+    const int nx = A.nx, ny = A.ny, pitch = A.pitch;
+    const int bx = tile_x * BW;     // padded column of the tile's first halo column
+    const int by = tile_y * BH;     // padded row of the tile's first halo row
+    const Consts K = make_consts(A);
 
     const bool wN = A.bc_n == kWall, wS = A.bc_s == kWall;
     const bool wE = A.bc_e == kWall, wW = A.bc_w == kWall;
@@ -232,7 +294,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
         const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag)
                         - (float)(c == kDryFlag) - (float)(d == kDryFlag);
         const float sum = ((a + b) + c) + d;
-        const float Hm = (wet == 0.0f) ? K.land : (wet == 4.0f ? sum * 0.25f : sum / wet);
+        const float Hm = (wet == 4.0f) ? sum * 0.25f : (wet == 0.0f ? K.land : sum / wet);
         const float h = eta + Hm;
         float E, U, V;
         if (fabsf(Hm - K.land) <= kDryEps) {
@@ -246,12 +308,15 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                 V = desingularize(h, hv, K.desing);
             }
         } else {
-            E = eta; U = hu / h; V = hv / h;
+            E = eta;
+            const float rh = rcp_refined(h);
+            U = div_r(hu, h, rh);
+            V = div_r(hv, h, rh);
         }
         S.E[rr][cc] = E;
         S.U[rr][cc] = U;
         S.V[rr][cc] = V;
-        if (FIELD) {
+        if constexpr (FIELD) {
             const size_t o = (size_t)R * pitch + C;   // textures are read unremapped
             S.Fc[rr][cc] = A.f_field[o];
             S.Nx[rr][cc] = A.nx_field[o];
@@ -261,15 +326,15 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
     __syncthreads();
 
     auto fcell = [&](int rr, int cc) -> float {
-        if (FIELD) return S.Fc[rr][cc];
+        if constexpr (FIELD) return S.Fc[rr][cc];
         return A.f;
     };
     auto nxcell = [&](int rr, int cc) -> float {
-        if (FIELD) return S.Nx[rr][cc];
+        if constexpr (FIELD) return S.Nx[rr][cc];
         return A.north_x;
     };
     auto nycell = [&](int rr, int cc) -> float {
-        if (FIELD) return S.Ny[rr][cc];
+        if constexpr (FIELD) return S.Ny[rr][cc];
         return A.north_y;
     };
     // geostrophic sign flip of the tangential velocity in wall ghost cells
@@ -295,14 +360,21 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             S.Ux[j][i] = minmod_slope(ul, uc, ur, K.theta);
             S.Vx[j][i] = minmod_slope(vl, vc, vr, K.theta);
             const float vc_raw = vc;
+            const bool flip_c = ghost_col_flip(C);
             if (ghost_col_flip(C - 1)) vl = -vl;
-            if (ghost_col_flip(C)) vc = -vc;
+            if (flip_c) vc = -vc;
             if (ghost_col_flip(C + 1)) vr = -vr;
-            const float n_x = nxcell(rr, cc), n_y = nycell(rr, cc);
-            const float fl = fcell(rr, cc - 1), fc = fcell(rr, cc), fr = fcell(rr, cc + 1);
-            const float fv_l = (n_x * ul + n_y * vl) * fl;
-            const float fv_c = (n_x * uc + n_y * vc) * fc;
-            const float fv_r = (n_x * ur + n_y * vr) * fr;
+            const float fc = fcell(rr, cc);
+            float fv_l, fv_c, fv_r;
+            if constexpr (ZERO) {
+                // (0*u + 1*v)*f == v*f up to the sign of a zero
+                fv_l = vl * fc; fv_c = vc * fc; fv_r = vr * fc;
+            } else {
+                const float n_x = nxcell(rr, cc), n_y = nycell(rr, cc);
+                fv_l = (n_x * ul + n_y * vl) * fcell(rr, cc - 1);
+                fv_c = (n_x * uc + n_y * vc) * fc;
+                fv_r = (n_x * ur + n_y * vr) * fcell(rr, cc + 1);
+            }
             const float bw_ = K.theta_g * ((ec - el) - K.vdx * (fv_c + fv_l));
             const float ce_ = K.half_g * ((er - el) - K.vdx * ((fv_r + 2.0f * fv_c) + fv_l));
             const float fw_ = K.theta_g * ((er - ec) - K.vdx * (fv_c + fv_r));
@@ -310,16 +382,27 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             // positivity adjustment; the reference flips v only in west ghost
             // column 1 here (its east test, bx+k > NX+2, never hits a column
             // an interior cell reads)
-            const float v_adj = (wW && C < 2) ? -vc_raw : vc_raw;
-            const float dxfv = (K.dx * fc) * v_adj;
+            const bool flip_a = wW && C < 2;
+            const float dfc = K.dx * fc;
+            const float dxfv = dfc * (flip_a ? -vc_raw : vc_raw);
             const float H_west = 0.5f * (S.Hi[rr][cc] + S.Hi[rr + 1][cc]);
             const float H_east = 0.5f * (S.Hi[rr][cc + 1] + S.Hi[rr + 1][cc + 1]);
-            const float t = (Kx + dxfv) / K.two_g;
+            const float t = div_r(Kx + dxfv, K.two_g, K.r_two_g);
             const float h_west = (ec + H_west) - t;
             const float h_east = (ec + H_east) + t;
-            Kx = (h_west > 0.0f) ? Kx : (-dxfv) + K.two_g * (ec + H_west);
-            Kx = (h_east > 0.0f) ? Kx : (-dxfv) - K.two_g * (ec + H_east);
-            S.Kx[j][i] = Kx;
+            bool adj = false;
+            if (!(h_west > 0.0f)) { Kx = (-dxfv) + K.two_g * (ec + H_west); adj = true; }
+            if (!(h_east > 0.0f)) { Kx = (-dxfv) - K.two_g * (ec + H_east); adj = true; }
+            if constexpr (ZERO) {
+                // T = (K + dx f v)/2g with the face's wall flip; equals t
+                // unless the adjustment fired or the flips differ
+                float T = t;
+                if (adj || flip_c != flip_a)
+                    T = div_r(Kx + dfc * (flip_c ? -vc_raw : vc_raw), K.two_g, K.r_two_g);
+                S.Kx[j][i] = T;
+            } else {
+                S.Kx[j][i] = Kx;
+            }
         } else {
             const int k = it - kSx;
             const int j = k / BW, i = k - j * BW;
@@ -331,29 +414,45 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             S.Uy[j][i] = minmod_slope(ul, uc, uu, K.theta);
             S.Vy[j][i] = minmod_slope(vl, vc, vu, K.theta);
             const float uc_raw = uc;
+            const bool flip_c = ghost_row_flip(R);
             if (ghost_row_flip(R - 1)) ul = -ul;
-            if (ghost_row_flip(R)) uc = -uc;
+            if (flip_c) uc = -uc;
             if (ghost_row_flip(R + 1)) uu = -uu;
-            const float n_x = nxcell(rr, cc), n_y = nycell(rr, cc);
-            const float e_x = n_y, e_y = -n_x;
-            const float fl = fcell(rr - 1, cc), fc = fcell(rr, cc), fu = fcell(rr + 1, cc);
-            const float fu_l = (e_x * ul + e_y * vl) * fl;
-            const float fu_c = (e_x * uc + e_y * vc) * fc;
-            const float fu_u = (e_x * uu + e_y * vu) * fu;
+            const float fc = fcell(rr, cc);
+            float fu_l, fu_c, fu_u;
+            if constexpr (ZERO) {
+                // east = (1, -0): (1*u + (-0)*v)*f == u*f up to a zero's sign
+                fu_l = ul * fc; fu_c = uc * fc; fu_u = uu * fc;
+            } else {
+                const float n_x = nxcell(rr, cc), n_y = nycell(rr, cc);
+                const float e_x = n_y, e_y = -n_x;
+                fu_l = (e_x * ul + e_y * vl) * fcell(rr - 1, cc);
+                fu_c = (e_x * uc + e_y * vc) * fc;
+                fu_u = (e_x * uu + e_y * vu) * fcell(rr + 1, cc);
+            }
             const float bw_ = K.theta_g * ((ec - el) + K.udy * (fu_c + fu_l));
             const float ce_ = K.half_g * ((eu - el) + K.udy * ((fu_u + 2.0f * fu_c) + fu_l));
             const float fw_ = K.theta_g * ((eu - ec) + K.udy * (fu_c + fu_u));
             float Ly = minmod_raw(bw_, ce_, fw_);
-            const float u_adj = (wS && R < 2) ? -uc_raw : uc_raw;
-            const float dyfu = (K.dy * fc) * u_adj;
+            const bool flip_a = wS && R < 2;
+            const float dfc = K.dy * fc;
+            const float dyfu = dfc * (flip_a ? -uc_raw : uc_raw);
             const float H_south = 0.5f * (S.Hi[rr][cc] + S.Hi[rr][cc + 1]);
             const float H_north = 0.5f * (S.Hi[rr + 1][cc] + S.Hi[rr + 1][cc + 1]);
-            const float t = (Ly - dyfu) / K.two_g;
+            const float t = div_r(Ly - dyfu, K.two_g, K.r_two_g);
             const float h_south = (ec + H_south) - t;
             const float h_north = (ec + H_north) + t;
-            Ly = (h_south > 0.0f) ? Ly : dyfu + K.two_g * (ec + H_south);
-            Ly = (h_north > 0.0f) ? Ly : dyfu - K.two_g * (ec + H_north);
-            S.Ly[j][i] = Ly;
+            bool adj = false;
+            if (!(h_south > 0.0f)) { Ly = dyfu + K.two_g * (ec + H_south); adj = true; }
+            if (!(h_north > 0.0f)) { Ly = dyfu - K.two_g * (ec + H_north); adj = true; }
+            if constexpr (ZERO) {
+                float Sv = t;
+                if (adj || flip_c != flip_a)
+                    Sv = div_r(Ly - dfc * (flip_c ? -uc_raw : uc_raw), K.two_g, K.r_two_g);
+                S.Ly[j][i] = Sv;
+            } else {
+                S.Ly[j][i] = Ly;
+            }
         }
     }
     __syncthreads();
@@ -376,29 +475,35 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                 const float Rmx = um + 0.5f * S.Ux[j][i];
                 const float Rmy = vm + 0.5f * S.Vx[j][i];
                 const float H_face = 0.5f * (S.Hi[rr][cp] + S.Hi[rr + 1][cp]);
-                const float Kp = S.Kx[j][i + 1], Km = S.Kx[j][i];
-                if (wW && Cp == 2) vm = -vm;
-                if (wE && Cp == nx + 2) vp = -vp;
-                const float dfp = K.dx * fcell(rr, cp), dfm = K.dx * fcell(rr, cm);
-                {
-                    const float n_x = nxcell(rr, cm), n_y = nycell(rr, cm);
-                    const float vpn = up * n_x + vp * n_y;
-                    const float vmn = um * n_x + vm * n_y;
-                    const float hp = fmaxf(0.0f, (eta_p + H_face) - ((Kp + dfp * vpn) / K.two_g));
-                    const float hm = fmaxf(0.0f, (eta_m + H_face) + ((Km + dfm * vmn) / K.two_g));
+                if constexpr (ZERO) {
+                    const float hp = fmaxf(0.0f, (eta_p + H_face) - S.Kx[j][i + 1]);
+                    const float hm = fmaxf(0.0f, (eta_m + H_face) + S.Kx[j][i]);
                     cu_flux(hm, Rmx, Rmy, hp, Rpx, Rpy, K, Fx, Fy, Fz);
-                }
-                if (FIELD) {
-                    const float n_x = nxcell(rr, cp), n_y = nycell(rr, cp);
-                    const float vpn = up * n_x + vp * n_y;
-                    const float vmn = um * n_x + vm * n_y;
-                    const float hp = fmaxf(0.0f, (eta_p + H_face) - ((Kp + dfp * vpn) / K.two_g));
-                    const float hm = fmaxf(0.0f, (eta_m + H_face) + ((Km + dfm * vmn) / K.two_g));
-                    cu_flux(hm, Rmx, Rmy, hp, Rpx, Rpy, K, Fx2, Fy2, Fz2);
+                } else {
+                    const float Kp = S.Kx[j][i + 1], Km = S.Kx[j][i];
+                    if (wW && Cp == 2) vm = -vm;
+                    if (wE && Cp == nx + 2) vp = -vp;
+                    const float dfp = K.dx * fcell(rr, cp), dfm = K.dx * fcell(rr, cm);
+                    {
+                        const float n_x = nxcell(rr, cm), n_y = nycell(rr, cm);
+                        const float vpn = up * n_x + vp * n_y;
+                        const float vmn = um * n_x + vm * n_y;
+                        const float hp = fmaxf(0.0f, (eta_p + H_face) - div_r(Kp + dfp * vpn, K.two_g, K.r_two_g));
+                        const float hm = fmaxf(0.0f, (eta_m + H_face) + div_r(Km + dfm * vmn, K.two_g, K.r_two_g));
+                        cu_flux(hm, Rmx, Rmy, hp, Rpx, Rpy, K, Fx, Fy, Fz);
+                    }
+                    if constexpr (FIELD) {
+                        const float n_x = nxcell(rr, cp), n_y = nycell(rr, cp);
+                        const float vpn = up * n_x + vp * n_y;
+                        const float vmn = um * n_x + vm * n_y;
+                        const float hp = fmaxf(0.0f, (eta_p + H_face) - div_r(Kp + dfp * vpn, K.two_g, K.r_two_g));
+                        const float hm = fmaxf(0.0f, (eta_m + H_face) + div_r(Km + dfm * vmn, K.two_g, K.r_two_g));
+                        cu_flux(hm, Rmx, Rmy, hp, Rpx, Rpy, K, Fx2, Fy2, Fz2);
+                    }
                 }
             }
             S.F[0][j][i] = Fx; S.F[1][j][i] = Fy; S.F[2][j][i] = Fz;
-            if (FIELD) { S.F2[0][j][i] = Fx2; S.F2[1][j][i] = Fy2; S.F2[2][j][i] = Fz2; }
+            if constexpr (FIELD) { S.F2[0][j][i] = Fx2; S.F2[1][j][i] = Fy2; S.F2[2][j][i] = Fz2; }
         } else {
             const int k = it - kFx;
             const int j = k / BW, i = k - j * BW;
@@ -414,33 +519,41 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                 const float Rmx = um + 0.5f * S.Uy[j][i];
                 const float Rmy = vm + 0.5f * S.Vy[j][i];
                 const float H_face = 0.5f * (S.Hi[rp][cc] + S.Hi[rp][cc + 1]);
-                const float Lp = S.Ly[j + 1][i], Lm = S.Ly[j][i];
-                if (wS && Rp == 2) um = -um;
-                if (wN && Rp == ny + 2) up = -up;
-                const float dfp = K.dy * fcell(rp, cc), dfm = K.dy * fcell(rm, cc);
-                {
-                    const float e_x = nycell(rm, cc), e_y = -nxcell(rm, cc);
-                    const float upe = up * e_x + vp * e_y;
-                    const float ume = um * e_x + vm * e_y;
-                    const float hp = fmaxf(0.0f, (eta_p + H_face) - ((Lp - dfp * upe) / K.two_g));
-                    const float hm = fmaxf(0.0f, (eta_m + H_face) + ((Lm - dfm * ume) / K.two_g));
+                if constexpr (ZERO) {
+                    const float hp = fmaxf(0.0f, (eta_p + H_face) - S.Ly[j + 1][i]);
+                    const float hm = fmaxf(0.0f, (eta_m + H_face) + S.Ly[j][i]);
                     float a, b, c;
                     cu_flux(hm, Rmy, Rmx, hp, Rpy, Rpx, K, a, b, c);
                     Gx = a; Gy = c; Gz = b;
-                }
-                if (FIELD) {
-                    const float e_x = nycell(rp, cc), e_y = -nxcell(rp, cc);
-                    const float upe = up * e_x + vp * e_y;
-                    const float ume = um * e_x + vm * e_y;
-                    const float hp = fmaxf(0.0f, (eta_p + H_face) - ((Lp - dfp * upe) / K.two_g));
-                    const float hm = fmaxf(0.0f, (eta_m + H_face) + ((Lm - dfm * ume) / K.two_g));
-                    float a, b, c;
-                    cu_flux(hm, Rmy, Rmx, hp, Rpy, Rpx, K, a, b, c);
-                    Gx2 = a; Gy2 = c; Gz2 = b;
+                } else {
+                    const float Lp = S.Ly[j + 1][i], Lm = S.Ly[j][i];
+                    if (wS && Rp == 2) um = -um;
+                    if (wN && Rp == ny + 2) up = -up;
+                    const float dfp = K.dy * fcell(rp, cc), dfm = K.dy * fcell(rm, cc);
+                    {
+                        const float e_x = nycell(rm, cc), e_y = -nxcell(rm, cc);
+                        const float upe = up * e_x + vp * e_y;
+                        const float ume = um * e_x + vm * e_y;
+                        const float hp = fmaxf(0.0f, (eta_p + H_face) - div_r(Lp - dfp * upe, K.two_g, K.r_two_g));
+                        const float hm = fmaxf(0.0f, (eta_m + H_face) + div_r(Lm - dfm * ume, K.two_g, K.r_two_g));
+                        float a, b, c;
+                        cu_flux(hm, Rmy, Rmx, hp, Rpy, Rpx, K, a, b, c);
+                        Gx = a; Gy = c; Gz = b;
+                    }
+                    if constexpr (FIELD) {
+                        const float e_x = nycell(rp, cc), e_y = -nxcell(rp, cc);
+                        const float upe = up * e_x + vp * e_y;
+                        const float ume = um * e_x + vm * e_y;
+                        const float hp = fmaxf(0.0f, (eta_p + H_face) - div_r(Lp - dfp * upe, K.two_g, K.r_two_g));
+                        const float hm = fmaxf(0.0f, (eta_m + H_face) + div_r(Lm - dfm * ume, K.two_g, K.r_two_g));
+                        float a, b, c;
+                        cu_flux(hm, Rmy, Rmx, hp, Rpy, Rpx, K, a, b, c);
+                        Gx2 = a; Gy2 = c; Gz2 = b;
+                    }
                 }
             }
             S.G[0][j][i] = Gx; S.G[1][j][i] = Gy; S.G[2][j][i] = Gz;
-            if (FIELD) { S.G2[0][j][i] = Gx2; S.G2[1][j][i] = Gy2; S.G2[2][j][i] = Gz2; }
+            if constexpr (FIELD) { S.G2[0][j][i] = Gx2; S.G2[1][j][i] = Gy2; S.G2[2][j][i] = Gz2; }
         }
     }
     __syncthreads();
@@ -450,6 +563,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
     const float wt = A.wind_t;
     const float X = wt * A.wx1 + (1.0f - wt) * A.wx0;
     const float Y = wt * A.wy1 + (1.0f - wt) * A.wy0;
+    const bool friction = A.r > 0.0f;
     for (int it = tid; it < BW * BH; it += NT) {
         const int j = it / BW, i = it - j * BW;
         const int rr = j + 2, cc = i + 2;
@@ -467,23 +581,27 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
         float fex, fey, fez, fwx, fwy, fwz, gnx, gny, gnz, gsx, gsy, gsz;
         fex = S.F[0][j][i + 1]; fey = S.F[1][j][i + 1]; fez = S.F[2][j][i + 1];
         gnx = S.G[0][j + 1][i]; gny = S.G[1][j + 1][i]; gnz = S.G[2][j + 1][i];
-        if (FIELD) {
+        if constexpr (FIELD) {
             fwx = S.F2[0][j][i]; fwy = S.F2[1][j][i]; fwz = S.F2[2][j][i];
             gsx = S.G2[0][j][i]; gsy = S.G2[1][j][i]; gsz = S.G2[2][j][i];
         } else {
             fwx = S.F[0][j][i]; fwy = S.F[1][j][i]; fwz = S.F[2][j][i];
             gsx = S.G[0][j][i]; gsy = S.G[1][j][i]; gsz = S.G[2][j][i];
         }
-        const float fd0 = (fex - fwx) / K.dx + (gnx - gsx) / K.dy;
-        const float fd1 = (fey - fwy) / K.dx + (gny - gsy) / K.dy;
-        const float fd2 = (fez - fwz) / K.dx + (gnz - gsz) / K.dy;
+        const float fd0 = div_r(fex - fwx, K.dx, K.r_dx) + div_r(gnx - gsx, K.dy, K.r_dy);
+        const float fd1 = div_r(fey - fwy, K.dx, K.r_dx) + div_r(gny - gsy, K.dy, K.r_dy);
+        const float fd2 = div_r(fez - fwz, K.dx, K.r_dx) + div_r(gnz - gsz, K.dy, K.r_dy);
 
         const float fc = fcell(rr, cc);
-        const float n_x = nxcell(rr, cc), n_y = nycell(rr, cc);
-        const float e_x = n_y, e_y = -n_x;
-        const float tx = (S.Kx[j][i + 1] + (K.dx * fc) * vc) / K.two_g;
+        float tx, ty;
+        if constexpr (ZERO) {
+            tx = S.Kx[j][i + 1];
+            ty = S.Ly[j + 1][i];
+        } else {
+            tx = div_r(S.Kx[j][i + 1] + (K.dx * fc) * vc, K.two_g, K.r_two_g);
+            ty = div_r(S.Ly[j + 1][i] - (K.dy * fc) * uc, K.two_g, K.r_two_g);
+        }
         const float eta_west = ec - tx, eta_east = ec + tx;
-        const float ty = (S.Ly[j + 1][i] - (K.dy * fc) * uc) / K.two_g;
         const float eta_south = ec - ty, eta_north = ec + ty;
 
         float st1 = 0.0f, st2 = 0.0f;
@@ -498,23 +616,32 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             const float eta_we = 0.5f * (eta_west + eta_east);
             const float b1 = (K.g * (eta_we + Hm)) * H_x;
             const float b2 = (K.g * (eta_sn + Hm)) * H_y;
-            const float hu_east = hu * e_x + hv * e_y;
-            const float hv_north = hu * n_x + hv * n_y;
-            const float hu_east_cor = fc * hv_north;
-            const float hv_north_cor = (-fc) * hu_east;
-            const float up_x = -n_x, up_y = n_y;
-            const float right_x = up_y, right_y = -up_x;
-            const float hu_cor = right_x * hu_east_cor + right_y * hv_north_cor;
-            const float hv_cor = up_x * hu_east_cor + up_y * hv_north_cor;
-            st1 = (X + hu_cor) + b1 / K.dx;
-            st2 = (Y + hv_cor) + b2 / K.dy;
+            float hu_cor, hv_cor;
+            if constexpr (ZERO) {
+                // north (0,1): the rotations reduce to f*hv and (-f)*hu
+                hu_cor = fc * hv;
+                hv_cor = (-fc) * hu;
+            } else {
+                const float n_x = nxcell(rr, cc), n_y = nycell(rr, cc);
+                const float e_x = n_y, e_y = -n_x;
+                const float hu_east = hu * e_x + hv * e_y;
+                const float hv_north = hu * n_x + hv * n_y;
+                const float hu_east_cor = fc * hv_north;
+                const float hv_north_cor = (-fc) * hu_east;
+                const float up_x = -n_x, up_y = n_y;
+                const float right_x = up_y, right_y = -up_x;
+                hu_cor = right_x * hu_east_cor + right_y * hv_north_cor;
+                hv_cor = up_x * hu_east_cor + up_y * hv_north_cor;
+            }
+            st1 = (X + hu_cor) + div_r(b1, K.dx, K.r_dx);
+            st2 = (Y + hv_cor) + div_r(b2, K.dy, K.r_dy);
         }
         const float L1 = -fd0;
         const float L2 = -fd1 + st1;
         const float L3 = -fd2 + st2;
 
         float Cf = 0.0f;
-        if (A.r > 0.0f) {   // quadratic bottom friction (:884-900)
+        if (friction) {   // quadratic bottom friction (:884-900)
             const float h = hreg;
             if (h < K.desing) {
                 const float u = desingularize(h, hu, K.desing);
@@ -531,13 +658,21 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
         if (A.rk_order < 3) {
             if (A.rk_step == 0) {
                 ue = ec + dt * L1;
-                uu = (hu + dt * L2) / (1.0f + Cf);
-                uv = (hv + dt * L3) / (1.0f + Cf);
+                uu = hu + dt * L2;
+                uv = hv + dt * L3;
+                if (friction) {   // x / (1 + 0) == x, so r == 0 skips it
+                    uu = uu / (1.0f + Cf);
+                    uv = uv / (1.0f + Cf);
+                }
             } else {
                 const float ea = A.eta_qn[o], ha = A.hu_qn[o], va = A.hv_qn[o];
                 ue = 0.5f * (ea + (ec + dt * L1));
-                uu = (0.5f * (ha + (hu + dt * L2))) / (1.0f + 0.5f * Cf);
-                uv = (0.5f * (va + (hv + dt * L3))) / (1.0f + 0.5f * Cf);
+                uu = 0.5f * (ha + (hu + dt * L2));
+                uv = 0.5f * (va + (hv + dt * L3));
+                if (friction) {
+                    uu = uu / (1.0f + 0.5f * Cf);
+                    uv = uv / (1.0f + 0.5f * Cf);
+                }
             }
         } else {
             if (A.rk_step == 0) {

@@ -91,37 +91,45 @@ const TileConfig kTileConfigs[] = {
 const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 const int kDefaultTileConfig = 1;
 
-template <int BW, int BH, int NT, bool FIELD>
+template <int BW, int BH, int NT, int MODE>
 __global__ __launch_bounds__(NT) void k_substep(SubstepArgs A) {
-    __shared__ TileSmem<BW, BH, FIELD> S;
+    __shared__ TileSmem<BW, BH, MODE> S;
     const int t = xcd_band(blockIdx.x, gridDim.x);
     const int ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
-    cdklm_tile<BW, BH, NT, FIELD>(A, tx, ty, S);
+    cdklm_tile<BW, BH, NT, MODE>(A, tx, ty, S);
 }
 
-template <int BW, int BH, int NT, bool FIELD>
+template <int BW, int BH, int NT, int MODE>
 static void launch_substep_t(SubstepArgs A, hipStream_t s) {
     A.tiles_x = (A.nx + BW - 1) / BW;
     A.tiles_y = (A.ny + BH - 1) / BH;
     const int nb = A.tiles_x * A.tiles_y;
-    hipLaunchKernelGGL((k_substep<BW, BH, NT, FIELD>), dim3(nb), dim3(NT), 0, s, A);
+    hipLaunchKernelGGL((k_substep<BW, BH, NT, MODE>), dim3(nb), dim3(NT), 0, s, A);
+}
+
+int substep_mode(const SubstepArgs& A, bool field) {
+    if (field) return kModeField;
+    return (A.north_x == 0.0f && A.north_y == 1.0f) ? kModeZero : kModeConst;
 }
 
 int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s) {
     if (cfg < 0) cfg = kDefaultTileConfig;
     if (cfg >= kNumTileConfigs)
         return set_error(GOCEAN_EINVAL, "tile configuration %d out of range [0,%d)", cfg, kNumTileConfigs);
-    if (field) {
+    const int mode = substep_mode(A, field);
+    if (mode == kModeField) {
         // per-cell Coriolis / north fields: one general configuration
-        launch_substep_t<32, 8, 256, true>(A, s);
+        launch_substep_t<32, 8, 256, kModeField>(A, s);
+    } else if (mode == kModeConst) {
+        launch_substep_t<32, 8, 256, kModeConst>(A, s);
     } else {
         switch (cfg) {
-            case 0: launch_substep_t<32, 8, 256, false>(A, s); break;
-            case 1: launch_substep_t<32, 16, 256, false>(A, s); break;
-            case 2: launch_substep_t<64, 8, 256, false>(A, s); break;
-            case 3: launch_substep_t<16, 16, 256, false>(A, s); break;
-            case 4: launch_substep_t<64, 4, 256, false>(A, s); break;
-            case 5: launch_substep_t<32, 16, 512, false>(A, s); break;
+            case 0: launch_substep_t<32, 8, 256, kModeZero>(A, s); break;
+            case 1: launch_substep_t<32, 16, 256, kModeZero>(A, s); break;
+            case 2: launch_substep_t<64, 8, 256, kModeZero>(A, s); break;
+            case 3: launch_substep_t<16, 16, 256, kModeZero>(A, s); break;
+            case 4: launch_substep_t<64, 4, 256, kModeZero>(A, s); break;
+            case 5: launch_substep_t<32, 16, 512, kModeZero>(A, s); break;
         }
     }
     return check_launch("cdklm substep launch");
