@@ -87,6 +87,62 @@ def cpu_baseline(inp, n_steps=1):
     return inp.nx * inp.ny * n_steps / s / 1e6, s
 
 
+def bench_ensemble(ctx, members=100, n=512, steps=50, warmup=5, per_member_steps=5):
+    """C3: ``members`` domains of n^2 advanced by the batched launch (eager
+    and HIP-graph replay), and the same members stepped one member per launch
+    from a Python loop (the reference's ensemble pattern)."""
+    import torch
+    from SWESimulators import BathymetryAndICs as B, CDKLM16Ensemble
+    inps = B.config3(members=members, n=n)
+    cells = sum(i.nx * i.ny for i in inps)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, inps, courant_number=0.7)
+    st = ens.gpu_stream
+    ens.step_n(warmup)
+    st.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    ens.step_n(steps)
+    e1.record(st)
+    st.synchronize()
+    wall = time.perf_counter() - t0
+    eager = cells * steps / wall / 1e6
+    kernel_s = e0.elapsed_time(e1) / 1000.0 / (2 * steps)
+    ens.capture(steps)
+    ens.replay()
+    st.synchronize()
+    t0 = time.perf_counter()
+    ens.replay()
+    st.synchronize()
+    gwall = time.perf_counter() - t0
+    graph = cells * steps / gwall / 1e6
+    ens.cleanUp()
+    del ens
+    # one member per launch (plain drop-in class, Python loop over members)
+    sims = [make_sim(i, ctx) for i in inps]
+    for sm in sims:
+        sm.step_n(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(per_member_steps):
+        for sm in sims:
+            sm.step_n(1)
+    torch.cuda.synchronize()
+    pwall = time.perf_counter() - t0
+    plain = cells * per_member_steps / pwall / 1e6
+    for sm in sims:
+        sm.cleanUp()
+    return {"workload": f"{members} members x {n}x{n}, walls, random depth, CFL 0.7 per member, SSP-RK2",
+            "cells": cells,
+            "batched_eager_mcell_steps_s": round(eager, 1),
+            "batched_graph_mcell_steps_s": round(graph, 1),
+            "plain_per_member_mcell_steps_s": round(plain, 1),
+            "batched_vs_plain": round(max(eager, graph) / plain, 2),
+            "batched_substep_launch_us": round(kernel_s * 1e6, 1),
+            "batched_hbm_frac_68B": round(max(eager, graph) * 1e6 * BYTES_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9), 4),
+            "steps": steps, "per_member_steps": per_member_steps}
+
+
 def measured_traffic(cells):
     """HBM bytes per substep launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy), when
@@ -109,6 +165,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile", type=int, default=None)
     ap.add_argument("--n", type=int, default=2048, help="domain side (headline: 2048)")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -146,6 +203,12 @@ def main():
     achieved = 0.5 * BYTES_PER_CELL_STEP * cells / avg_launch / 1e9
     log(f"[bench] plain: {value:.1f} Mcell-steps/s, event time/launch {avg_launch*1e6:.1f} us")
 
+    extra = {}
+    if not a.no_ensemble:
+        log("[bench] batched ensemble C3")
+        extra["ensemble"] = bench_ensemble(ctx)
+        log(f"[bench] ensemble: {extra['ensemble']}")
+
     cpu = None
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle, 1 core, 1 RK2 step)")
@@ -176,6 +239,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(cells)},
         "cpu_baseline": cpu,
+        "extra": extra,
     }
     print(json.dumps(out), flush=True)
 

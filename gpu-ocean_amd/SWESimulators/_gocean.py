@@ -68,6 +68,39 @@ PROTOTYPES = [
      [ctypes.POINTER(Domain), c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
 ]
 
+class Member(ctypes.Structure):
+    """gocean_member_t"""
+    _fields_ = [("domain", Domain), ("state_offset", ctypes.c_int64),
+                ("hi_offset", ctypes.c_int64), ("hm_offset", ctypes.c_int64),
+                ("field_offset", ctypes.c_int64), ("wind", Wind)]
+
+
+PROTOTYPES.extend([
+    ("gocean_ensemble_create", ctypes.c_int,
+     [ctypes.POINTER(Member), ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("gocean_ensemble_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("gocean_ensemble_info", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+      ctypes.POINTER(ctypes.c_int)]),
+    ("gocean_ensemble_bathymetry_init", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("gocean_ensemble_periodic_bc", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("gocean_ensemble_max_dt", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]),
+    ("gocean_ensemble_substep", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("gocean_ensemble_rk_steps", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("gocean_ensemble_capture", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+      ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    ("gocean_ensemble_replay", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+])
+
+
+
 _lib = None
 _lock = threading.Lock()
 

@@ -134,6 +134,66 @@ int gocean_max_dt(const gocean_domain_t* d, const float* eta, const float* hu,
  * count and writes up to max entries of "BWxBH/threads" into names. */
 int gocean_tile_configs(const char** names, int max);
 
+/* ------------------------------------------------------------------------
+ * Batched ensembles.  The reference steps ensemble members one at a time
+ * from Python (OceanModelEnsemble.modelStep,
+ * gpu_ocean/SWESimulators/OceanModelEnsemble.py:105-117;
+ * EnsembleFromFiles.stepToObservation, EnsembleFromFiles.py:302-325;
+ * CDKLM16.dataAssimilationStep, CDKLM16.py:650-704).  Here all members, of
+ * any sizes, advance together: one launch per RK substep covers every
+ * member's tiles.  The caller owns flat device buffers:
+ *   state buffers (two, three for RK3): member m's eta at state_offset,
+ *       hu at state_offset + plane, hv at state_offset + 2*plane,
+ *       plane = (ny+4)*pitch;
+ *   Hi buffer (member m at hi_offset), Hm buffer (hm_offset);
+ *   optional field buffer: f at field_offset, north_x at +plane,
+ *       north_y at +2*plane (field_offset -1 = constant f / north);
+ *   dt: one device float per member.
+ * The handle owns the member table, the tile -> member map and an optional
+ * captured graph.  Calls on one handle must not overlap.
+ * ------------------------------------------------------------------------ */
+typedef struct gocean_member {
+    gocean_domain_t domain;
+    int64_t state_offset;
+    int64_t hi_offset;
+    int64_t hm_offset;
+    int64_t field_offset;
+    gocean_wind_t wind;
+} gocean_member_t;
+
+typedef struct gocean_ensemble gocean_ensemble_t;
+
+/* Build the member table (all members share rk_order). */
+int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_config,
+                           gocean_ensemble_t** out);
+int gocean_ensemble_destroy(gocean_ensemble_t* ens);
+/* n_members, total tiles and kernel mode (0 zero angle, 1 constant north,
+ * 2 per-cell fields). */
+int gocean_ensemble_info(const gocean_ensemble_t* ens, int* n_members, int* n_tiles, int* mode);
+/* Hi halo + Hm for every member (gocean_bathymetry_init per member). */
+int gocean_ensemble_bathymetry_init(gocean_ensemble_t* ens, float* Hi, float* Hm, void* stream);
+/* Periodic ghost exchange for every member (gocean_periodic_bc per member). */
+int gocean_ensemble_periodic_bc(gocean_ensemble_t* ens, float* state, void* stream);
+/* dt_out[m] = courant * CFL minimum of member m (raw minimum if courant <= 0),
+ * on the device; the per-member form of gocean_max_dt. */
+int gocean_ensemble_max_dt(gocean_ensemble_t* ens, const float* state, const float* Hm,
+                           float courant, float* dt_out, void* stream);
+/* One substep of every member (gocean_cdklm_substep per member), dt per
+ * member from the device array. */
+int gocean_ensemble_substep(gocean_ensemble_t* ens, int rk_step, const float* state_in,
+                            const float* state_qn, float* state_out, const float* Hi,
+                            const float* fields, const float* dt, void* stream);
+/* n full RK steps of every member (gocean_rk_steps per member). */
+int gocean_ensemble_rk_steps(gocean_ensemble_t* ens, int n_steps, float* state0, float* state1,
+                             float* state2, const float* Hi, const float* fields,
+                             const float* dt, void* stream);
+/* Capture n full RK steps of every member into a HIP graph held by the
+ * handle (no host synchronisation inside; dt is read from the device), and
+ * replay it on a stream. */
+int gocean_ensemble_capture(gocean_ensemble_t* ens, int n_steps, float* state0, float* state1,
+                            float* state2, const float* Hi, const float* fields, const float* dt);
+int gocean_ensemble_replay(gocean_ensemble_t* ens, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
