@@ -708,10 +708,16 @@ class OracleSim:
         return self.t
 
     def step_n(self, n, dt=None):
-        """n fixed-dt RK steps (the fixed-dt loop used by the batched path)."""
-        d = self.dt if dt is None else dt
-        for _ in range(n):
-            self.step(np.float32(d))
+        """Exactly n RK steps of dt (the fixed-dt loop used by the batched
+        path): step(t_end=dt) with the simulator's own dt set to dt."""
+        d = np.float32(self.dt if dt is None else dt)
+        saved = self.dt
+        self.dt = float(d)
+        try:
+            for _ in range(n):
+                self.step(d)
+        finally:
+            self.dt = saved
         return self.t
 
     def download(self):
