@@ -97,6 +97,15 @@ PROTOTYPES.extend([
      [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
       ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("gocean_ensemble_replay", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("gocean_adjoint_scratch_floats", ctypes.c_longlong, [ctypes.POINTER(Domain)]),
+    ("gocean_adjoint_substep", c_int,
+     [ctypes.POINTER(Domain), c_float, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_float, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gocean_adjoint_rk2_step", c_int,
+     [ctypes.POINTER(Domain), c_float, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 ])
 
 

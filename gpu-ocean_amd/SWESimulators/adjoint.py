@@ -1,0 +1,236 @@
+"""Reverse mode of the fixed-dt CDKLM16 RK2 step: checkpointed adjoint driver
+and a ``torch.autograd.Function``.
+
+The reference has no gradients; callers that do variational data
+assimilation or parameter estimation around gpu-ocean's step
+(CDKLM16.step, gpu_ocean/SWESimulators/CDKLM16.py:508-593 with
+rk_order 2, CDKLM16.py:533-544) need d(misfit)/d(initial state).  The
+Jacobian-transpose products are hand-written HIP kernels
+(``gocean_adjoint_rk2_step`` in ``lib/libgocean_hip.so``); this module only
+schedules them:
+
+* forward: n steps of the HIP forward, a copy of the state kept every
+  ``checkpoint_every`` steps (revolve-style uniform checkpoints);
+* backward: for each segment, last first, re-run the forward from its
+  checkpoint keeping Q^n and Q* of every step of the segment, then apply the
+  step adjoints in reverse.
+
+Memory: (n / K + 1) checkpoints of 3 planes plus 6 (K + 1) planes of segment
+trajectory, plane = (ny + 4) * pitch floats.  Work: one extra forward run plus
+the adjoint kernels.
+
+Scope (enforced by the library): north = (0, 1), constant f, r == 0,
+rk_order 2, fixed dt, constant wind.  Ghost cells: an output ghost cell the
+forward rewrites (periodic images) passes its adjoint to the interior cell it
+copies; one it leaves untouched (wall sides) is the input's ghost value, so
+its adjoint passes to the same input ghost cell.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import Common
+from . import _gocean as G
+
+
+def _torch():
+    return Common._torch()
+
+
+class RK2Adjoint(object):
+    """Checkpointed forward / reverse runs of one domain.
+
+    ``domain``: a ``gocean_domain_t`` (``Common.make_domain``), rk_order 2.
+    ``Hi``: device tensor of corner depths, halo completed
+    (``gocean_bathymetry_init``).  ``wind``: ``gocean_wind_t`` or None.
+    """
+
+    def __init__(self, domain, Hi, stream=None, wind=None, checkpoint_every=50, tile_config=-1):
+        torch = _torch()
+        self.lib = G.load()
+        self.d = domain
+        if domain.rk_order != 2:
+            raise ValueError("the adjoint supports rk_order 2 only")
+        self.Hi = Hi
+        self.dev = Hi.device
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.dev)
+        self.wind = wind if wind is not None else G.Wind()
+        self.fields = G.Fields()
+        self.K = max(1, int(checkpoint_every))
+        self.tile_config = int(tile_config)
+        self.shape = (domain.ny + 4, domain.pitch)
+        n_scratch = int(self.lib.gocean_adjoint_scratch_floats(domain))
+        with torch.cuda.stream(self.stream):
+            self.scratch = torch.empty(n_scratch, dtype=torch.float32, device=self.dev)
+        self._ghost_maps()
+        self.checkpoints = None
+
+    # ------------------------------------------------------------------
+    def _ghost_maps(self):
+        """Index arrays for the output-ghost rules (see module docstring)."""
+        torch = _torch()
+        d = self.d
+        nx, ny, pitch = d.nx, d.ny, d.pitch
+        pNS = d.bc_north == G_BC_PERIODIC
+        pEW = d.bc_east == G_BC_PERIODIC
+        R = np.arange(ny + 4, dtype=np.int64)[:, None]
+        C = np.arange(nx + 4, dtype=np.int64)[None, :]
+        gr = (R < 2) | (R > ny + 1)
+        gc = (C < 2) | (C > nx + 1)
+        ghost = gr | gc
+        written = ghost & (~gr | pNS) & (~gc | pEW)
+        Ri = np.where(R < 2, R + ny, np.where(R > ny + 1, R - ny, R))
+        Ci = np.where(C < 2, C + nx, np.where(C > nx + 1, C - nx, C))
+        o = R * pitch + C
+        mk = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=self.dev)
+        self.wrap_src = mk(o[written])
+        self.wrap_dst = mk((Ri * pitch + Ci)[written])
+        self.keep = mk(o[ghost & ~written])
+
+    def _alloc3(self):
+        torch = _torch()
+        return tuple(torch.empty(self.shape, dtype=torch.float32, device=self.dev) for _ in range(3))
+
+    def _substep(self, rk_step, qin, qn, qout, dt):
+        G.check(self.lib.gocean_cdklm_substep(
+            self.d, float(dt), None, int(rk_step),
+            *(G.ptr(a) for a in qin), *(G.ptr(a) for a in qn) if qn is not None else (None,) * 3,
+            *(G.ptr(a) for a in qout), G.ptr(self.Hi), self.fields, self.wind,
+            self.tile_config, G.stream_handle(self.stream)), "gocean_cdklm_substep")
+
+    def _rk_steps(self, q, tmp, n, dt):
+        G.check(self.lib.gocean_rk_steps(
+            self.d, int(n), float(dt), None, *(G.ptr(a) for a in q), *(G.ptr(a) for a in tmp),
+            None, None, None, G.ptr(self.Hi), self.fields, self.wind, self.tile_config,
+            G.stream_handle(self.stream)), "gocean_rk_steps")
+
+    # ------------------------------------------------------------------
+    def forward(self, q0, n_steps, dt):
+        """Run ``n_steps`` fixed-dt RK2 steps from padded (eta, hu, hv) device
+        tensors; returns new tensors and keeps the checkpoints."""
+        torch = _torch()
+        self.n_steps, self.dt = int(n_steps), float(np.float32(dt))
+        with torch.cuda.stream(self.stream):
+            q = tuple(a.detach().to(torch.float32).contiguous().clone() for a in q0)
+            tmp = self._alloc3()
+            self.checkpoints = []
+            done = 0
+            while done < self.n_steps:
+                self.checkpoints.append(tuple(a.clone() for a in q))
+                k = min(self.K, self.n_steps - done)
+                self._rk_steps(q, tmp, k, self.dt)
+                done += k
+            self.final = q
+        return q
+
+    def backward(self, lam):
+        """Adjoint of the last ``forward`` run: ``lam`` = d(loss)/d(output)
+        (padded tensors); returns d(loss)/d(input)."""
+        torch = _torch()
+        if self.checkpoints is None:
+            raise RuntimeError("backward() needs a preceding forward()")
+        with torch.cuda.stream(self.stream):
+            lam = [a.detach().to(torch.float32).contiguous().clone() for a in lam]
+            keep = [a.reshape(-1)[self.keep].clone() for a in lam]
+            for a in lam:
+                f = a.reshape(-1)
+                f.index_add_(0, self.wrap_dst, f[self.wrap_src])
+            cur = tuple(lam)
+            nxt = self._alloc3()
+            n_seg = len(self.checkpoints)
+            traj_n = [self._alloc3() for _ in range(self.K)]
+            traj_s = [self._alloc3() for _ in range(self.K)]
+            for s in range(n_seg - 1, -1, -1):
+                start = s * self.K
+                k = min(self.K, self.n_steps - start)
+                end_state = self.checkpoints[s + 1] if s + 1 < n_seg else self.final
+                # recompute the segment: Q^n and Q* of each of its steps
+                for a, b in zip(traj_n[0], self.checkpoints[s]):
+                    a.copy_(b)
+                for i in range(k):
+                    self._substep(0, traj_n[i], None, traj_s[i], self.dt)
+                    if i + 1 < k:
+                        self._substep(1, traj_s[i], traj_n[i], traj_n[i + 1], self.dt)
+                for i in range(k - 1, -1, -1):
+                    eta_next = traj_n[i + 1][0] if i + 1 < k else end_state[0]
+                    G.check(self.lib.gocean_adjoint_rk2_step(
+                        self.d, self.dt, None,
+                        *(G.ptr(a) for a in traj_n[i]), *(G.ptr(a) for a in traj_s[i]),
+                        G.ptr(eta_next), *(G.ptr(a) for a in cur), *(G.ptr(a) for a in nxt),
+                        G.ptr(self.Hi), G.ptr(self.scratch), G.stream_handle(self.stream)),
+                        "gocean_adjoint_rk2_step")
+                    cur, nxt = nxt, cur
+            out = tuple(a.clone() for a in cur)
+            for a, kv in zip(out, keep):
+                a.reshape(-1)[self.keep] += kv
+        return out
+
+
+G_BC_PERIODIC = 2
+
+
+class CDKLM16RK2(object):
+    """``torch.autograd.Function`` wrapper: ``CDKLM16RK2.apply(eta, hu, hv,
+    adjoint, n_steps, dt)`` returns the state after ``n_steps`` RK2 steps and
+    backpropagates through the hand-written HIP adjoint."""
+
+    _fn = None
+
+    @classmethod
+    def function(cls):
+        if cls._fn is None:
+            torch = _torch()
+
+            class _Fn(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, eta, hu, hv, adj, n_steps, dt):
+                    ctx.adj = adj
+                    return adj.forward((eta, hu, hv), n_steps, dt)
+
+                @staticmethod
+                def backward(ctx, g_eta, g_hu, g_hv):
+                    z = lambda g, like: g if g is not None else torch.zeros_like(like)
+                    fin = ctx.adj.final
+                    lam = ctx.adj.backward((z(g_eta, fin[0]), z(g_hu, fin[1]), z(g_hv, fin[2])))
+                    return lam[0], lam[1], lam[2], None, None, None
+
+            cls._fn = _Fn
+        return cls._fn
+
+    @classmethod
+    def apply(cls, eta, hu, hv, adj, n_steps, dt):
+        return cls.function().apply(eta, hu, hv, adj, n_steps, dt)
+
+
+def simulator_adjoint(sim, checkpoint_every=50):
+    """An ``RK2Adjoint`` on a CDKLM16 simulator's domain, bathymetry, stream
+    and current wind."""
+    return RK2Adjoint(sim.domain, sim.bathymetry.Bi.tensor, stream=sim.gpu_stream,
+                      wind=sim._wind(), checkpoint_every=checkpoint_every,
+                      tile_config=sim.tile_config)
+
+
+def bilinear_eta(eta, obs_x, obs_y, dx, dy):
+    """eta bilinearly interpolated at physical points (x, y), with cell (i, j)
+    of the interior centred at ((i + 0.5) dx, (j + 0.5) dy); ``eta`` is a
+    padded (ny + 4, nx + 4 or pitch) tensor.  Differentiable torch ops; points
+    must lie between interior cell centres."""
+    torch = _torch()
+    fx = torch.as_tensor(obs_x, dtype=eta.dtype, device=eta.device) / dx - 0.5
+    fy = torch.as_tensor(obs_y, dtype=eta.dtype, device=eta.device) / dy - 0.5
+    i0 = torch.floor(fx).to(torch.int64)
+    j0 = torch.floor(fy).to(torch.int64)
+    wx = fx - i0.to(eta.dtype)
+    wy = fy - j0.to(eta.dtype)
+    r, c = j0 + 2, i0 + 2
+    e00, e01 = eta[r, c], eta[r, c + 1]
+    e10, e11 = eta[r + 1, c], eta[r + 1, c + 1]
+    return (1 - wy) * ((1 - wx) * e00 + wx * e01) + wy * ((1 - wx) * e10 + wx * e11)
+
+
+def point_misfit(eta, obs_x, obs_y, y, dx, dy):
+    """J = 0.5 * sum_k (eta(x_k, y_k) - y_k)^2 with bilinear interpolation."""
+    torch = _torch()
+    d = bilinear_eta(eta, obs_x, obs_y, dx, dy) - torch.as_tensor(y, dtype=eta.dtype, device=eta.device)
+    return 0.5 * (d * d).sum()

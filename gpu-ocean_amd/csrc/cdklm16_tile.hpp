@@ -57,21 +57,22 @@ struct SubstepArgs {
 // minmod of (b, c, f): 0.25*s(b)*(s(b)+s(c))*(s(c)+s(f))*min(|b|,|c|,|f|)
 // (common.cu:544-551).  The sign product is exactly +1, -1 or 0, so the value
 // is min(|b|,|c|,|f|) with b's sign when all three sign bits agree, else zero.
-__device__ __forceinline__ float minmod_raw(float b, float c, float f) {
+__host__ __device__ __forceinline__ float minmod_raw(float b, float c, float f) {
     const float m = fminf(fminf(fabsf(b), fabsf(c)), fabsf(f));
-    const uint32_t ub = __float_as_uint(b), uc = __float_as_uint(c), uf = __float_as_uint(f);
+    const uint32_t ub = __builtin_bit_cast(uint32_t, b), uc = __builtin_bit_cast(uint32_t, c);
+    const uint32_t uf = __builtin_bit_cast(uint32_t, f);
     const bool agree = (((ub ^ uc) | (uc ^ uf)) & 0x80000000u) == 0u;
     return agree ? copysignf(m, b) : 0.0f;
 }
 
 // common.cu:532-542
-__device__ __forceinline__ float minmod_slope(float l, float c, float r, float theta) {
+__host__ __device__ __forceinline__ float minmod_slope(float l, float c, float r, float theta) {
     return minmod_raw((c - l) * theta, (r - l) * 0.5f, (r - c) * theta);
 }
 
 // common.cu:822-824.  The reference's "0.5*eps" is a double literal, so that
 // one addition is evaluated in double there; this does the same.
-__device__ __forceinline__ float desingularize(float h, float hu, float eps) {
+__host__ __device__ __forceinline__ float desingularize(float h, float hu, float eps) {
     const float a = (h * h) / (2.0f * eps);
     const float b = (float)((double)a + 0.5 * (double)eps);
     return hu / fmaxf(fminf(b, eps), fabsf(h));

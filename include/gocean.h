@@ -194,6 +194,45 @@ int gocean_ensemble_capture(gocean_ensemble_t* ens, int n_steps, float* state0, 
                             float* state2, const float* Hi, const float* fields, const float* dt);
 int gocean_ensemble_replay(gocean_ensemble_t* ens, void* stream);
 
+/* ---- adjoint (reverse mode) of the fixed-dt RK2 step ----
+ * No counterpart in the reference, which has no gradients: this serves
+ * variational data assimilation / parameter estimation callers that need
+ * d(misfit)/d(initial state) of gocean_rk_steps with a fixed dt.  Scope:
+ * north = (0, 1), constant f, r == 0, rk_order == 2; other domains return
+ * GOCEAN_EINVAL.  All state / adjoint arrays are padded (ny+4) x pitch
+ * planes; ghost entries of adjoint inputs are ignored and those of adjoint
+ * outputs are zero (their sensitivity is folded into the interior cells the
+ * ghosts mirror or wrap).  Branches (minmod candidate, wave-speed extrema,
+ * upwinding, positivity adjustment, wet / desingularised / dry cells) follow
+ * the forward evaluation.
+ *
+ * Scratch: gocean_adjoint_scratch_floats(d) floats of device memory, owned
+ * by the caller. */
+long long gocean_adjoint_scratch_floats(const gocean_domain_t* d);
+
+/* o = alpha * J^T w + gamma * x, with J the Jacobian of the substep map
+ * Q -> base(Q) + dt * L(Q) (the rk_step 0 update, CDKLM16_kernel.cu:905-1001)
+ * at Q = (eta, hu, hv).  w_clamp / x_clamp (nullable): forward eta outputs;
+ * cells they show clamped to the dry state pass no w / x.  x may be NULL. */
+int gocean_adjoint_substep(const gocean_domain_t* d, float dt, const float* dt_dev,
+                           const float* eta, const float* hu, const float* hv,
+                           const float* w_eta, const float* w_hu, const float* w_hv,
+                           const float* w_clamp, const float* x_eta, const float* x_hu,
+                           const float* x_hv, const float* x_clamp, float alpha, float gamma,
+                           float* o_eta, float* o_hu, float* o_hv, const float* Hi,
+                           float* scratch, void* stream);
+
+/* lambda^n = (dQ^{n+1}/dQ^n)^T lambda^{n+1} for one SSP-RK2 step
+ * Q* = P(Q^n), Q^{n+1} = (Q^n + P(Q*)) / 2 (CDKLM16.py:533-544).  Needs the
+ * step's Q^n and Q* (and, nullable, eta of Q^{n+1} for the dry-clamp mask);
+ * o must not alias l. */
+int gocean_adjoint_rk2_step(const gocean_domain_t* d, float dt, const float* dt_dev,
+                            const float* eta_n, const float* hu_n, const float* hv_n,
+                            const float* eta_s, const float* hu_s, const float* hv_s,
+                            const float* eta_next, const float* l_eta, const float* l_hu,
+                            const float* l_hv, float* o_eta, float* o_hu, float* o_hv,
+                            const float* Hi, float* scratch, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

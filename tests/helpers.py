@@ -201,3 +201,53 @@ class DeviceDomain:
                      "max_dt")
         self.torch.cuda.synchronize()
         return self.scratch[:2].cpu().numpy()
+
+
+def random_case(nx, ny, bc, seed, land=False, shallow=False, fvar=False, angvar=False, r=0.0,
+          wind=(0.0, 0.0), dx=200.0, dy=150.0, dyadic=False):
+    """Random smooth-bathymetry case with random state (no flat regions, so
+    branch ties are improbable); optional shallow strip, land block, Coriolis /
+    rotation fields, friction and wind.  Returns (p, Hi, Hm, eta, hu, hv)."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:ny + 5, 0:nx + 5]
+    Hi = (30 + 10 * np.sin(x / 3.0) + 8 * np.cos(y / 2.5)).astype(np.float32)
+    if shallow:
+        w = (nx + 5) // 3
+        Hi[:, :w] = (0.05 + 0.02 * rng.random((ny + 5, w))).astype(np.float32)
+    if dyadic:
+        # multiples of 2^-12: sums of four corners are exact in float32 and
+        # float64, so a ghost cell's depth equals its mirror image's in both
+        # precisions and wall-face upwind ties resolve the same way in both
+        Hi = (np.round(Hi.astype(np.float64) * 4096.0) / 4096.0).astype(np.float32)
+    if land:
+        Hi[ny // 2:ny // 2 + 3, nx // 2:nx // 2 + 2] = O.LAND_VALUE
+    f = np.float32(1.2e-4)
+    if fvar:
+        f = (1e-4 + 2e-5 * rng.random((ny + 4, nx + 4))).astype(np.float32)
+    nxv, nyv = np.float32(0.0), np.float32(1.0)
+    if angvar:
+        ang = (0.3 * rng.random((ny + 4, nx + 4))).astype(np.float32)
+        nxv, nyv = np.sin(ang).astype(np.float32), np.cos(ang).astype(np.float32)
+    p = O.Params(nx=nx, ny=ny, dx=dx, dy=dy, bc=bc, f=f, north_x=nxv, north_y=nyv, r=r,
+                 wind_x=(wind[0], wind[0] * 1.5), wind_y=(wind[1], wind[1] * 0.5))
+    Hi = O.bathymetry_halo(Hi, p)
+    Hm = O.init_bm(Hi)
+    eta = (0.3 * rng.standard_normal((ny + 4, nx + 4))).astype(np.float32)
+    eta = np.maximum(eta, -Hm)
+    hu = (2 * rng.standard_normal((ny + 4, nx + 4))).astype(np.float32)
+    hv = (2 * rng.standard_normal((ny + 4, nx + 4))).astype(np.float32)
+    if shallow:
+        w = (nx + 4) // 3
+        eta[:, :w] = -Hm[:, :w] + (0.08 * rng.random((ny + 4, w))).astype(np.float32)
+        hu[:, :w] *= 0.001
+        hv[:, :w] *= 0.001
+    O.periodic_bc(eta, hu, hv, p)
+    return p, Hi, Hm, eta, hu, hv
+
+
+def hi64(Hi):
+    """Corner depths in float64 with land kept at exactly the land value: the
+    float32 land marker 1e20 widens to 1.00000002e20, which the float64
+    land test |H - 1e20| < 1e-3 would no longer recognise."""
+    Hi = np.asarray(Hi)
+    return np.where(Hi == np.float32(O.LAND_VALUE), float(O.LAND_VALUE), Hi.astype(np.float64))

@@ -102,3 +102,34 @@ def test_check_raises_with_message():
 
 def test_no_gpu_means_zero_devices_or_more():
     assert G.load().gocean_device_count() >= 0
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(rk_order=1), "rk_order 2"),
+    (dict(r=0.01), "r == 0"),
+    (dict(north_x=0.1, north_y=0.99), "rotation"),
+])
+def test_adjoint_scope_rejected(kw, msg):
+    lib = G.load()
+    d = _dom(**kw)
+    rc = lib.gocean_adjoint_rk2_step(d, 1.0, None, *([16] * 6), None, 16, 16, 16, 32, 32, 32,
+                                     64, 128, None)
+    assert rc == -1 and msg in lib.gocean_last_error().decode()
+    rc = lib.gocean_adjoint_substep(d, 1.0, None, *([16] * 6), None, None, None, None, None,
+                                    1.0, 0.0, 32, 32, 32, 64, 128, None)
+    assert rc == -1 and msg in lib.gocean_last_error().decode()
+
+
+def test_adjoint_arguments_rejected():
+    lib = G.load()
+    d = _dom()
+    assert lib.gocean_adjoint_scratch_floats(d) == 27 * (d.ny + 4) * d.pitch
+    rc = lib.gocean_adjoint_rk2_step(d, 1.0, None, *([16] * 6), None, 16, 16, 16, 16, 32, 32,
+                                     64, 128, None)
+    assert rc == -1 and "alias" in lib.gocean_last_error().decode()
+    rc = lib.gocean_adjoint_substep(d, 1.0, None, *([16] * 6), None, 16, None, None, None,
+                                    1.0, 0.0, 32, 32, 32, 64, 128, None)
+    assert rc == -1 and "extra" in lib.gocean_last_error().decode()
+    rc = lib.gocean_adjoint_substep(d, 1.0, None, *([16] * 6), None, None, None, None, None,
+                                    1.0, 0.0, 32, 32, 32, 64, None, None)
+    assert rc == -1 and "NULL" in lib.gocean_last_error().decode()
