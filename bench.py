@@ -143,6 +143,64 @@ def bench_ensemble(ctx, members=100, n=512, steps=50, warmup=5, per_member_steps
             "steps": steps, "per_member_steps": per_member_steps}
 
 
+def bench_adjoint(ctx, n=512, steps=500, K=50, n_obs=64, seed=44):
+    """C4: gradient of J = 0.5 sum_k (eta(x_k, T) - y_k)^2 (64 point
+    observations, bilinear) with respect to the initial eta, hu, hv of one
+    n^2 domain after ``steps`` fixed-dt RK2 steps, checkpoint every K.  The
+    observations come from a twin run started from a perturbed eta plus
+    N(0, 0.01 m) noise.  Reports the forward time (plain step loop) and the
+    forward + adjoint time of the autograd wrapper, and their ratio."""
+    import torch
+    from SWESimulators import BathymetryAndICs as B, adjoint as A
+    rng = np.random.default_rng(seed)
+    inp = B.config4(n=n)
+    sim = make_sim(inp, ctx)
+    dt = sim.dt
+    st = sim.gpu_stream
+    ox = rng.uniform(2.0, n - 2.0, n_obs) * inp.dx
+    oy = rng.uniform(2.0, n - 2.0, n_obs) * inp.dy
+    q0 = [a.tensor.clone() for a in (sim.gpu_data.h0, sim.gpu_data.hu0, sim.gpu_data.hv0)]
+    adj = A.simulator_adjoint(sim, checkpoint_every=K)
+    # twin run: perturbed initial eta
+    pert = 0.05 * B.gaussian_random_field((n + 4, n + 4), 16.0, rng).astype(np.float32)
+    twin0 = [q0[0] + torch.from_numpy(pert).to(q0[0].device), q0[1], q0[2]]
+    with torch.no_grad():
+        tw = adj.forward(twin0, steps, dt)
+        y = A.bilinear_eta(tw[0], ox, oy, inp.dx, inp.dy) + torch.from_numpy(
+            rng.normal(0.0, 0.01, n_obs).astype(np.float32)).to(q0[0].device)
+    # forward alone: the plain step loop
+    sim.step_n(5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step_n(steps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+
+    def fwd_adj():
+        q = [a.clone().requires_grad_(True) for a in q0]
+        out = A.CDKLM16RK2.apply(*q, adj, steps, dt)
+        J = A.point_misfit(out[0], ox, oy, y, inp.dx, inp.dy)
+        J.backward()
+        return J, q
+
+    fwd_adj()                       # warm-up (allocations, first launches)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, q = fwd_adj()
+    torch.cuda.synchronize()
+    t_tot = time.perf_counter() - t0
+    cells = n * n
+    g = [float(x.grad.norm()) for x in q]
+    sim.cleanUp()
+    return {"workload": f"{n}x{n} walls, random depth, 8 eddies, fixed dt {dt:.3f} s (CFL 0.7), "
+                        f"{steps} RK2 steps, checkpoint every {K}, {n_obs} point observations",
+            "fwd_s": round(t_fwd, 4), "fwd_adj_s": round(t_tot, 4),
+            "fwd_adj_over_fwd": round(t_tot / t_fwd, 2),
+            "fwd_mcell_steps_s": round(cells * steps / t_fwd / 1e6, 1),
+            "fwd_adj_mcell_steps_s": round(cells * steps / t_tot / 1e6, 1),
+            "J": float(J), "grad_norms_eta_hu_hv": [round(v, 6) for v in g]}
+
+
 def measured_traffic(cells):
     """HBM bytes per substep launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy), when
@@ -166,6 +224,7 @@ def main():
     ap.add_argument("--tile", type=int, default=None)
     ap.add_argument("--n", type=int, default=2048, help="domain side (headline: 2048)")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -208,6 +267,11 @@ def main():
         log("[bench] batched ensemble C3")
         extra["ensemble"] = bench_ensemble(ctx)
         log(f"[bench] ensemble: {extra['ensemble']}")
+
+    if not a.no_adjoint:
+        log("[bench] adjoint C4")
+        extra["adjoint"] = bench_adjoint(ctx)
+        log(f"[bench] adjoint: {extra['adjoint']}")
 
     cpu = None
     if not a.no_cpu:

@@ -88,6 +88,22 @@ class RK2Adjoint(object):
         self.wrap_dst = mk((Ri * pitch + Ci)[written])
         self.keep = mk(o[ghost & ~written])
 
+    def _enter(self):
+        """Order this driver's stream after the caller's current stream."""
+        torch = _torch()
+        cur = torch.cuda.current_stream(self.dev)
+        if cur != self.stream:
+            self.stream.wait_stream(cur)
+        return cur
+
+    def _leave(self, cur, outs):
+        """Order the caller's stream after this driver's work; mark returned
+        tensors as used there so their memory is not reused early."""
+        if cur != self.stream:
+            cur.wait_stream(self.stream)
+            for t in outs:
+                t.record_stream(cur)
+
     def _alloc3(self):
         torch = _torch()
         return tuple(torch.empty(self.shape, dtype=torch.float32, device=self.dev) for _ in range(3))
@@ -111,6 +127,7 @@ class RK2Adjoint(object):
         tensors; returns new tensors and keeps the checkpoints."""
         torch = _torch()
         self.n_steps, self.dt = int(n_steps), float(np.float32(dt))
+        cur = self._enter()
         with torch.cuda.stream(self.stream):
             q = tuple(a.detach().to(torch.float32).contiguous().clone() for a in q0)
             tmp = self._alloc3()
@@ -122,6 +139,7 @@ class RK2Adjoint(object):
                 self._rk_steps(q, tmp, k, self.dt)
                 done += k
             self.final = q
+        self._leave(cur, q)
         return q
 
     def backward(self, lam):
@@ -130,6 +148,7 @@ class RK2Adjoint(object):
         torch = _torch()
         if self.checkpoints is None:
             raise RuntimeError("backward() needs a preceding forward()")
+        cur_stream = self._enter()
         with torch.cuda.stream(self.stream):
             lam = [a.detach().to(torch.float32).contiguous().clone() for a in lam]
             keep = [a.reshape(-1)[self.keep].clone() for a in lam]
@@ -164,6 +183,7 @@ class RK2Adjoint(object):
             out = tuple(a.clone() for a in cur)
             for a, kv in zip(out, keep):
                 a.reshape(-1)[self.keep] += kv
+        self._leave(cur_stream, out)
         return out
 
 
