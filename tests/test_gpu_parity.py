@@ -227,3 +227,38 @@ def test_device_dt_pointer():
     D.rk_steps(3, 123.0, b0, b1, dt_dev=dt_dev)
     for x, y in zip(D.host(a0), D.host(b0)):
         assert np.array_equal(x, y)
+
+
+def test_config1_fp64_error_growth():
+    """C1 on the GPU against the float64 oracle: max |d| / max |ref| per
+    field after 1, 10, 50, 100 and 200 RK2 steps, printed; bounded by 1e-5
+    for eta after one step and 1e-3 for every field at 200 steps."""
+    from SWESimulators import BathymetryAndICs as B
+    _say("config1 fp64 error growth")
+    c1 = B.config1()
+    p = H.oracle_params(c1)
+    sim = O.OracleSim(c1.eta, c1.hu, c1.hv, c1.Hi, p, dt=-1)
+    D = H.DeviceDomain(c1)
+    D.bathymetry_init()
+    h0 = D.state(*sim.h0)
+    h1 = D.state(*sim.h1)
+    dt = np.float32(np.float32(0.7) * np.float32(O.max_dt(*sim.h0, sim.Hm, p)))
+    Hi64 = H.hi64(sim.Hi)
+    Hm64 = O.init_bm(Hi64, dtype=np.float64)
+    q = [a.astype(np.float64) for a in sim.h0]
+    done = 0
+    for target in (1, 10, 50, 100, 200):
+        D.rk_steps(target - done, dt, h0, h1)
+        for _ in range(target - done):
+            e, u, v = q
+            s = [np.array(a) for a in O.cdklm_substep(float(dt), 0, e, u, v, e.copy(), u.copy(), v.copy(),
+                                                      Hi64, Hm64, p, wind_t=0.0, dtype=np.float64)]
+            q = [np.array(a) for a in O.cdklm_substep(float(dt), 1, *s, e.copy(), u.copy(), v.copy(),
+                                                      Hi64, Hm64, p, wind_t=0.0, dtype=np.float64)]
+        done = target
+        errs = [H.max_rel(g[2:-2, 2:-2], r_[2:-2, 2:-2]) for g, r_ in zip(D.host(h0), q)]
+        _say("   steps", target, "max rel error vs fp64 (eta, hu, hv):", ["%.2e" % e for e in errs])
+        if target == 1:
+            assert errs[0] <= 1e-5
+        if target == 200:
+            assert max(errs) <= 1e-3
