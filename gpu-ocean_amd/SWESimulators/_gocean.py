@@ -106,6 +106,11 @@ PROTOTYPES.extend([
      [ctypes.POINTER(Domain), c_float, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gocean_adjoint_segment_work_floats", ctypes.c_longlong, [ctypes.POINTER(Domain), c_int]),
+    ("gocean_adjoint_segment", c_int,
+     [ctypes.POINTER(Domain), c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, ctypes.c_longlong, c_void_p]),
 ])
 
 

@@ -11,13 +11,14 @@ schedules them:
 
 * forward: n steps of the HIP forward, a copy of the state kept every
   ``checkpoint_every`` steps (revolve-style uniform checkpoints);
-* backward: for each segment, last first, re-run the forward from its
-  checkpoint keeping Q^n and Q* of every step of the segment, then apply the
-  step adjoints in reverse.
+* backward: for each segment, last first, one ``gocean_adjoint_segment``
+  call re-runs the forward from its checkpoint in the recording variant
+  (keeping Q^n, Q* and the branch record of every substep) and applies the
+  substep adjoints in reverse.
 
-Memory: (n / K + 1) checkpoints of 3 planes plus 6 (K + 1) planes of segment
-trajectory, plane = (ny + 4) * pitch floats.  Work: one extra forward run plus
-the adjoint kernels.
+Memory: (n / K + 1) checkpoints of 3 planes plus (16 K + 9) planes of segment
+work, plane = (ny + 4) * pitch floats.  Work: one extra forward run plus the
+adjoint kernels.
 
 Scope (enforced by the library): north = (0, 1), constant f, r == 0,
 rk_order 2, fixed dt, constant wind.  Ghost cells: an output ghost cell the
@@ -65,6 +66,7 @@ class RK2Adjoint(object):
             self.scratch = torch.empty(n_scratch, dtype=torch.float32, device=self.dev)
         self._ghost_maps()
         self.checkpoints = None
+        self.work = None
 
     # ------------------------------------------------------------------
     def _ghost_maps(self):
@@ -107,13 +109,6 @@ class RK2Adjoint(object):
     def _alloc3(self):
         torch = _torch()
         return tuple(torch.empty(self.shape, dtype=torch.float32, device=self.dev) for _ in range(3))
-
-    def _substep(self, rk_step, qin, qn, qout, dt):
-        G.check(self.lib.gocean_cdklm_substep(
-            self.d, float(dt), None, int(rk_step),
-            *(G.ptr(a) for a in qin), *(G.ptr(a) for a in qn) if qn is not None else (None,) * 3,
-            *(G.ptr(a) for a in qout), G.ptr(self.Hi), self.fields, self.wind,
-            self.tile_config, G.stream_handle(self.stream)), "gocean_cdklm_substep")
 
     def _rk_steps(self, q, tmp, n, dt):
         G.check(self.lib.gocean_rk_steps(
@@ -158,28 +153,17 @@ class RK2Adjoint(object):
             cur = tuple(lam)
             nxt = self._alloc3()
             n_seg = len(self.checkpoints)
-            traj_n = [self._alloc3() for _ in range(self.K)]
-            traj_s = [self._alloc3() for _ in range(self.K)]
+            if self.work is None:
+                nw = int(self.lib.gocean_adjoint_segment_work_floats(self.d, self.K))
+                self.work = torch.empty(nw, dtype=torch.float32, device=self.dev)
             for s in range(n_seg - 1, -1, -1):
-                start = s * self.K
-                k = min(self.K, self.n_steps - start)
-                end_state = self.checkpoints[s + 1] if s + 1 < n_seg else self.final
-                # recompute the segment: Q^n and Q* of each of its steps
-                for a, b in zip(traj_n[0], self.checkpoints[s]):
-                    a.copy_(b)
-                for i in range(k):
-                    self._substep(0, traj_n[i], None, traj_s[i], self.dt)
-                    if i + 1 < k:
-                        self._substep(1, traj_s[i], traj_n[i], traj_n[i + 1], self.dt)
-                for i in range(k - 1, -1, -1):
-                    eta_next = traj_n[i + 1][0] if i + 1 < k else end_state[0]
-                    G.check(self.lib.gocean_adjoint_rk2_step(
-                        self.d, self.dt, None,
-                        *(G.ptr(a) for a in traj_n[i]), *(G.ptr(a) for a in traj_s[i]),
-                        G.ptr(eta_next), *(G.ptr(a) for a in cur), *(G.ptr(a) for a in nxt),
-                        G.ptr(self.Hi), G.ptr(self.scratch), G.stream_handle(self.stream)),
-                        "gocean_adjoint_rk2_step")
-                    cur, nxt = nxt, cur
+                k = min(self.K, self.n_steps - s * self.K)
+                G.check(self.lib.gocean_adjoint_segment(
+                    self.d, k, self.dt, None, *(G.ptr(a) for a in self.checkpoints[s]),
+                    *(G.ptr(a) for a in cur), *(G.ptr(a) for a in nxt), G.ptr(self.Hi),
+                    G.ptr(self.work), self.work.numel(), G.stream_handle(self.stream)),
+                    "gocean_adjoint_segment")
+                cur, nxt = nxt, cur
             out = tuple(a.clone() for a in cur)
             for a, kv in zip(out, keep):
                 a.reshape(-1)[self.keep] += kv

@@ -48,6 +48,10 @@ struct SubstepArgs {
     int bc_n, bc_e, bc_s, bc_w;
     int rk_order, rk_step;
     int tiles_x, tiles_y;
+    // branch record of the substep (written only by the recording variant):
+    // per padded cell u, v, the x / y face depth terms and a code word
+    float* tape_u; float* tape_v; float* tape_t; float* tape_s;
+    uint32_t* tape_code;
 };
 
 // ---------------------------------------------------------------------------
@@ -69,6 +73,27 @@ __host__ __device__ __forceinline__ float minmod_raw(float b, float c, float f) 
 __host__ __device__ __forceinline__ float minmod_slope(float l, float c, float r, float theta) {
     return minmod_raw((c - l) * theta, (r - l) * 0.5f, (r - c) * theta);
 }
+
+// which candidate minmod_raw(b, c, f) returns: 0, 1, 2, or 3 when the sign
+// bits disagree (result zero).  Ties pick the first; the value is the same.
+__host__ __device__ __forceinline__ uint32_t minmod_sel(float b, float c, float f) {
+    const uint32_t ub = __builtin_bit_cast(uint32_t, b), uc = __builtin_bit_cast(uint32_t, c);
+    const uint32_t uf = __builtin_bit_cast(uint32_t, f);
+    if ((((ub ^ uc) | (uc ^ uf)) & 0x80000000u) != 0u) return 3u;
+    const float ab = fabsf(b), ac = fabsf(c), af = fabsf(f);
+    if (ab <= ac && ab <= af) return 0u;
+    if (ac <= af) return 1u;
+    return 2u;
+}
+
+// Branch record of one cell (tape code word):
+//   bits  0- 7: x slope centre: u slope sel, v slope sel, K sel, adjustment
+//   bits  8-15: y slope centre: u slope sel, v slope sel, L sel, adjustment
+//   bits 16-20: flux of the cell's east face, bits 21-25: of its north face
+// sel as minmod_sel; adjustment 0 none, 1 low side, 2 high side (overrides 1);
+// flux: a+ source (0 zero, 1 minus side, 2 plus side), a- source (same), and
+// bit 4 when |a+ - a-| < flux_eps (zero flux).
+constexpr int kCodeX = 0, kCodeY = 8, kCodeFx = 16, kCodeFy = 21;
 
 // common.cu:822-824.  The reference's "0.5*eps" is a double literal, so that
 // one addition is evaluated in double there; this does the same.
@@ -124,10 +149,12 @@ struct Consts {
 
 // Central-upwind flux of primitive face states Qm=(hm,um,vm), Qp=(hp,up,vp)
 // (CDKLM16_F_func + CDKLM16_flux, CDKLM16_kernel.cu:80-140).
+template <bool REC = false>
 __device__ __forceinline__ void cu_flux(float hm, float um, float vm,
                                         float hp, float up, float vp,
                                         const Consts& K,
-                                        float& Fx, float& Fy, float& Fz) {
+                                        float& Fx, float& Fy, float& Fz,
+                                        uint32_t* code = nullptr) {
     float Fpx = 0.0f, Fpy = 0.0f, Fpz = 0.0f, u_p = 0.0f, c_p = 0.0f;
     if (hp > K.cutoff) {
         const float q = hp * up;
@@ -149,6 +176,12 @@ __device__ __forceinline__ void cu_flux(float hm, float um, float vm,
     const float am = fminf(fminf(u_m - c_m, u_p - c_p), 0.0f);
     const float ap = fmaxf(fmaxf(u_m + c_m, u_p + c_p), 0.0f);
     const float den = ap - am;
+    if constexpr (REC) {
+        const float b1 = u_m + c_m, b2 = u_p + c_p, a1 = u_m - c_m, a2 = u_p - c_p;
+        const uint32_t cp = (fmaxf(b1, b2) > 0.0f) ? ((b1 >= b2) ? 1u : 2u) : 0u;
+        const uint32_t cm = (fminf(a1, a2) < 0.0f) ? ((a1 <= a2) ? 1u : 2u) : 0u;
+        *code = cp | (cm << 2) | ((fabsf(den) < K.flux_eps) ? 16u : 0u);
+    }
     if (fabsf(den) < K.flux_eps) {
         Fx = 0.0f; Fy = 0.0f; Fz = 0.0f;
         return;
@@ -175,9 +208,14 @@ __device__ __forceinline__ void cu_flux(float hm, float um, float vm,
 //              both evaluations are kept.
 enum { kModeZero = 0, kModeConst = 1, kModeField = 2 };
 
-template <int BW, int BH, int MODE>
+template <int BW, int BH, int MODE, bool TAPE = false>
 struct TileSmem {
     static constexpr bool FIELD = MODE == kModeField;
+    // branch codes of the recording variant (see kCodeX ...)
+    uint32_t CodeX[TAPE ? BH : 1][TAPE ? BW + 2 : 1];
+    uint32_t CodeY[TAPE ? BH + 2 : 1][TAPE ? BW : 1];
+    uint32_t CodeF[TAPE ? BH : 1][TAPE ? BW + 1 : 1];
+    uint32_t CodeG[TAPE ? BH + 1 : 1][TAPE ? BW : 1];
     float Hi[BH + 5][BW + 5];          // corner depths, land flagged
     float E[BH + 4][BW + 4];           // eta (dry / land adjusted)
     float U[BH + 4][BW + 4];           // u
@@ -228,9 +266,10 @@ __device__ __forceinline__ Consts make_consts(const SubstepArgs& A) {
 // ---------------------------------------------------------------------------
 // the tile
 // ---------------------------------------------------------------------------
-template <int BW, int BH, int NT, int MODE>
+template <int BW, int BH, int NT, int MODE, bool TAPE = false>
 __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int tile_y,
-                                           TileSmem<BW, BH, MODE>& S) {
+                                           TileSmem<BW, BH, MODE, TAPE>& S) {
+    static_assert(!TAPE || MODE == kModeZero, "the branch record is defined for zero rotation");
     constexpr bool FIELD = MODE == kModeField;
     constexpr bool ZERO = MODE == kModeZero;
     const int tid = threadIdx.x;
@@ -392,6 +431,14 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             const float h_west = (ec + H_west) - t;
             const float h_east = (ec + H_east) + t;
             bool adj = false;
+            if constexpr (TAPE) {
+                S.CodeX[j][i] = (minmod_sel((uc - ul) * K.theta, (ur - ul) * 0.5f, (ur - uc) * K.theta)
+                                 | (minmod_sel((vc_raw - S.V[rr][cc - 1]) * K.theta,
+                                               (S.V[rr][cc + 1] - S.V[rr][cc - 1]) * 0.5f,
+                                               (S.V[rr][cc + 1] - vc_raw) * K.theta) << 2)
+                                 | (minmod_sel(bw_, ce_, fw_) << 4)
+                                 | ((!(h_east > 0.0f)) ? (2u << 6) : (!(h_west > 0.0f)) ? (1u << 6) : 0u));
+            }
             if (!(h_west > 0.0f)) { Kx = (-dxfv) + K.two_g * (ec + H_west); adj = true; }
             if (!(h_east > 0.0f)) { Kx = (-dxfv) - K.two_g * (ec + H_east); adj = true; }
             if constexpr (ZERO) {
@@ -444,6 +491,14 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             const float h_south = (ec + H_south) - t;
             const float h_north = (ec + H_north) + t;
             bool adj = false;
+            if constexpr (TAPE) {
+                S.CodeY[j][i] = (minmod_sel((uc_raw - S.U[rr - 1][cc]) * K.theta,
+                                            (S.U[rr + 1][cc] - S.U[rr - 1][cc]) * 0.5f,
+                                            (S.U[rr + 1][cc] - uc_raw) * K.theta)
+                                 | (minmod_sel((vc - vl) * K.theta, (vu - vl) * 0.5f, (vu - vc) * K.theta) << 2)
+                                 | (minmod_sel(bw_, ce_, fw_) << 4)
+                                 | ((!(h_north > 0.0f)) ? (2u << 6) : (!(h_south > 0.0f)) ? (1u << 6) : 0u));
+            }
             if (!(h_south > 0.0f)) { Ly = dyfu + K.two_g * (ec + H_south); adj = true; }
             if (!(h_north > 0.0f)) { Ly = dyfu - K.two_g * (ec + H_north); adj = true; }
             if constexpr (ZERO) {
@@ -468,6 +523,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             const int Cp = bx + cp;
             const float eta_p = S.E[rr][cp], eta_m = S.E[rr][cm];
             float Fx = 0.0f, Fy = 0.0f, Fz = 0.0f, Fx2 = 0.0f, Fy2 = 0.0f, Fz2 = 0.0f;
+            uint32_t fcode = 0u;
             if (!(eta_p == kDryFlag || eta_m == kDryFlag)) {
                 const float up = S.U[rr][cp], um = S.U[rr][cm];
                 float vp = S.V[rr][cp], vm = S.V[rr][cm];
@@ -479,7 +535,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                 if constexpr (ZERO) {
                     const float hp = fmaxf(0.0f, (eta_p + H_face) - S.Kx[j][i + 1]);
                     const float hm = fmaxf(0.0f, (eta_m + H_face) + S.Kx[j][i]);
-                    cu_flux(hm, Rmx, Rmy, hp, Rpx, Rpy, K, Fx, Fy, Fz);
+                    cu_flux<TAPE>(hm, Rmx, Rmy, hp, Rpx, Rpy, K, Fx, Fy, Fz, &fcode);
                 } else {
                     const float Kp = S.Kx[j][i + 1], Km = S.Kx[j][i];
                     if (wW && Cp == 2) vm = -vm;
@@ -504,6 +560,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                 }
             }
             S.F[0][j][i] = Fx; S.F[1][j][i] = Fy; S.F[2][j][i] = Fz;
+            if constexpr (TAPE) S.CodeF[j][i] = fcode;
             if constexpr (FIELD) { S.F2[0][j][i] = Fx2; S.F2[1][j][i] = Fy2; S.F2[2][j][i] = Fz2; }
         } else {
             const int k = it - kFx;
@@ -512,6 +569,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             const int Rp = by + rp;
             const float eta_p = S.E[rp][cc], eta_m = S.E[rm][cc];
             float Gx = 0.0f, Gy = 0.0f, Gz = 0.0f, Gx2 = 0.0f, Gy2 = 0.0f, Gz2 = 0.0f;
+            uint32_t gcode = 0u;
             if (!(eta_p == kDryFlag || eta_m == kDryFlag)) {
                 float up = S.U[rp][cc], um = S.U[rm][cc];
                 const float vp = S.V[rp][cc], vm = S.V[rm][cc];
@@ -524,7 +582,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                     const float hp = fmaxf(0.0f, (eta_p + H_face) - S.Ly[j + 1][i]);
                     const float hm = fmaxf(0.0f, (eta_m + H_face) + S.Ly[j][i]);
                     float a, b, c;
-                    cu_flux(hm, Rmy, Rmx, hp, Rpy, Rpx, K, a, b, c);
+                    cu_flux<TAPE>(hm, Rmy, Rmx, hp, Rpy, Rpx, K, a, b, c, &gcode);
                     Gx = a; Gy = c; Gz = b;
                 } else {
                     const float Lp = S.Ly[j + 1][i], Lm = S.Ly[j][i];
@@ -554,10 +612,60 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                 }
             }
             S.G[0][j][i] = Gx; S.G[1][j][i] = Gy; S.G[2][j][i] = Gz;
+            if constexpr (TAPE) S.CodeG[j][i] = gcode;
             if constexpr (FIELD) { S.G2[0][j][i] = Gx2; S.G2[1][j][i] = Gy2; S.G2[2][j][i] = Gz2; }
         }
     }
     __syncthreads();
+
+    // ---- branch record (recording variant only) ----
+    // Each padded cell is written by one tile: interior cells by the tile
+    // that owns them; the ghost columns / rows the reverse pass reads (u, v
+    // of the two ghost cells, the depth term and code of the first one) by
+    // the boundary tile next to them.
+    if constexpr (TAPE) {
+        const bool own_w = tile_x == 0, own_s = tile_y == 0;
+        const bool own_e = nx + 1 >= bx + 2 && nx + 1 < bx + BW + 2;
+        const bool own_n = ny + 1 >= by + 2 && ny + 1 < by + BH + 2;
+        for (int it = tid; it < kConv; it += NT) {
+            int rr, cc;
+            if (it < kMid) {
+                rr = 2 + it / (BW + 4);
+                cc = it - (rr - 2) * (BW + 4);
+            } else {
+                const int k = it - kMid;
+                const int band = k / BW;
+                rr = band < 2 ? band : BH + band;
+                cc = 2 + (k - band * BW);
+            }
+            const int R = by + rr, C = bx + cc;
+            if (R > ny + 3 || C > nx + 3) continue;
+            const bool ri = R >= 2 && R <= ny + 1, ci = C >= 2 && C <= nx + 1;
+            bool own;
+            if (ri && ci) own = rr >= 2 && rr < BH + 2 && cc >= 2 && cc < BW + 2;
+            else if (ri) own = rr >= 2 && rr < BH + 2 && (C < 2 ? own_w : own_e);
+            else if (ci) own = cc >= 2 && cc < BW + 2 && (R < 2 ? own_s : own_n);
+            else own = false;
+            if (!own) continue;
+            const size_t o = (size_t)R * pitch + C;
+            A.tape_u[o] = S.U[rr][cc];
+            A.tape_v[o] = S.V[rr][cc];
+            uint32_t code = 0u;
+            // x slope centre (interior rows, tile columns 1..BW+2)
+            if (ri && cc >= 1 && cc <= BW + 2 && C >= 1 && C <= nx + 2) {
+                A.tape_t[o] = S.Kx[rr - 2][cc - 1];
+                code |= S.CodeX[rr - 2][cc - 1] << kCodeX;
+                if (C <= nx + 1 && cc <= BW + 1) code |= S.CodeF[rr - 2][cc - 1] << kCodeFx;
+            }
+            // y slope centre (interior columns, tile rows 1..BH+2)
+            if (ci && rr >= 1 && rr <= BH + 2 && R >= 1 && R <= ny + 2) {
+                A.tape_s[o] = S.Ly[rr - 1][cc - 2];
+                code |= S.CodeY[rr - 1][cc - 2] << kCodeY;
+                if (R <= ny + 1 && rr <= BH + 1) code |= S.CodeG[rr - 1][cc - 2] << kCodeFy;
+            }
+            A.tape_code[o] = code;
+        }
+    }
 
     // ---- phase E: sources, RK update, store (:802-1004) ----
     const float dt = A.dt_dev ? *A.dt_dev : A.dt;

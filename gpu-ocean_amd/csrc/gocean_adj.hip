@@ -2,28 +2,31 @@
 // CDKLM16 substep, for the fixed-dt SSP-RK2 step.
 //
 // The forward substep is  out = P(Q) = base(Q) + dt*L(Q)  (rk_step 0) and
-// out = 0.5*(Q^n + P(Q*))  (rk_step 1), where L is the flux divergence plus
-// sources of gpu_ocean/SWESimulators/gpu_kernels/CDKLM16_kernel.cu:429-1004
-// and base(Q) = (E, u*h_reg, v*h_reg) the converted state the update starts
-// from (:597-602, :905-920).  gocean_adjoint_substep returns
-//     alpha * J_P(Q)^T w  +  gamma * extra
-// with every branch (minmod candidate, wave-speed min/max, upwinding,
-// positivity adjustment, wet/dry/desingularised cells) fixed to the forward
-// choice.  Scope: zero grid rotation, constant f, walls/periodic, no
-// friction.  Cells the forward update clamped to the dry state are recognised
-// from the forward outputs (optional arguments) and pass no adjoint.
+// out = 0.5*(Q^n + P(Q*))  (rk_step 1), then the dry clamp, where L is the
+// flux divergence plus sources of
+// gpu_ocean/SWESimulators/gpu_kernels/CDKLM16_kernel.cu:429-1004 and
+// base(Q) = (E, u*h_reg, v*h_reg) the converted state the update starts from
+// (:597-602, :905-920).  The reverse pass applies
+//     o = alpha * J_P(Q)^T w  +  gamma * x
+// with every branch fixed to the forward's choice.  Scope: zero grid
+// rotation, constant f, walls/periodic, no friction, RK2.
 //
-// Structure (all kernels one thread per face or cell, deterministic gathers,
-// no atomics):
-//   k_adj_xface / k_adj_yface  recompute the four cells around a face
-//       (conversion, slopes, adjustment, face depth term), evaluate the face
-//       flux, and apply its VJP with lambda_F = dt*(w_p - w_m)/dx back to the
-//       converted (E, u, v) of those four cells -> 12 scratch planes each;
-//   k_adj_cell   gathers the face contributions of a cell, adds the base and
-//       source-term VJP (output cells), applies the conversion VJP and writes
-//       the cell's adjoint, ghost cells included;
-//   k_adj_fold_ns / k_adj_fold_ew  map ghost-cell adjoints back through the
-//       wall mirror / periodic wrap into interior cells and clear the ghosts.
+// Two launches per substep adjoint:
+//   * the forward substep re-run in its recording variant writes a branch
+//     record ("tape": u, v, the x / y face depth terms, and a code word with
+//     every limiter / adjustment / wave-speed choice) for each cell;
+//   * k_adj_tile: one workgroup per output tile.  Cell data of the tile and a
+//     3-cell halo go to LDS; every face of the tile is differentiated once
+//     (flux VJP), the slope / depth-term VJPs are applied once per slope
+//     centre, and each output cell gathers its contributions in a fixed
+//     order, so the result is deterministic.  Ghost cells get their own
+//     adjoint entries;
+//   * k_adj_fold: each interior cell near a boundary adds the adjoints of the
+//     ghost cells that are its images (wall mirror with the normal momentum
+//     negated, periodic wrap) and clears them.
+// Branch-deciding quantities are either read from the record or recomputed
+// with the forward's exact operations; values that only scale derivatives use
+// fast reciprocal / square root.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -34,465 +37,609 @@
 
 namespace gocean {
 
-struct AdjArgs {
-    const float* eta; const float* hu; const float* hv;     // substep input Q
-    const float* w_eta; const float* w_hu; const float* w_hv;
-    const float* x_eta; const float* x_hu; const float* x_hv;
-    const float* w_clamp;   // forward output (eta) whose dry clamp masks w
-    const float* x_clamp;   // forward output (eta) whose dry clamp masks x
+struct AdjTileArgs {
+    const float* eta;                     // substep input eta (images read for ghosts)
+    const float* tU; const float* tV; const float* tT; const float* tS;
+    const uint32_t* tC;                   // branch record of the substep
+    const float* w_eta; const float* w_hu; const float* w_hv; const float* w_clamp;
+    const float* x_eta; const float* x_hu; const float* x_hv; const float* x_clamp;
     float* o_eta; float* o_hu; float* o_hv;
     const float* Hi;
-    float* xc;       // 12 planes: x-face contributions
-    float* yc;       // 12 planes: y-face contributions
     const float* dt_dev;
     float dt, alpha, gamma;
     int nx, ny, pitch, hi_pitch;
-    long long plane;
     float dx, dy, g, theta, desing, flux_eps, cutoff, land, f;
+    int bc_n, bc_e, bc_s, bc_w;
+    int tiles_x, tiles_y;
+};
+
+__host__ __device__ inline float adj_rcp(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
+__host__ __device__ inline float adj_sqrt(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+
+// value of the minmod candidate `sel` (the forward's minmod_slope result,
+// bit for bit) and its derivative pattern
+__host__ __device__ inline float slope_val(uint32_t sel, float l, float c, float r, float theta) {
+    if (sel == 0u) return (c - l) * theta;
+    if (sel == 1u) return (r - l) * 0.5f;
+    if (sel == 2u) return (r - c) * theta;
+    return 0.0f;
+}
+
+// d(candidate sel)/d(l, c, r): (c-l)*theta, (r-l)/2, (r-c)*theta, or 0
+__host__ __device__ inline void slope_coef(uint32_t sel, float theta, float& kl, float& kc, float& kr) {
+    kl = sel == 0u ? -theta : (sel == 1u ? -0.5f : 0.0f);
+    kc = sel == 0u ? theta : (sel == 2u ? -theta : 0.0f);
+    kr = sel == 1u ? 0.5f : (sel == 2u ? theta : 0.0f);
+}
+
+// VJP of the face depth term T = (K - sig*Dd*s_c*t_c)/2g of one slope centre
+// (sig = -1 for x faces, +1 for y faces; t = tangential velocity, v for x and
+// u for y; s_* the wall ghost sign flips, s_a the adjustment's own flip) with
+// K the minmod candidate `ks` of
+//   theta g ((E_c - E_l) + sig D f (s_c t_c + s_l t_l)),
+//   g/2 ((E_r - E_l) + sig D f ((s_r t_r + 2 s_c t_c) + s_l t_l)),
+//   theta g ((E_r - E_c) + sig D f (s_c t_c + s_r t_r)),
+// or the positivity adjustment sig Dd s_a t_c +- 2g (E_c + H).  Adds
+// d/d(E, t) of cells l, c, r times gT.
+__host__ __device__ inline void depth_term_vjp(uint32_t ks, uint32_t adj, float sig, float theta, float g,
+                                               float f, float Dd, float D, float sl, float sc, float sr,
+                                               float sa, float gT, float* gE, float* gt) {
+    const float two_g = 2.0f * g;
+    const float gK = gT / two_g;
+    float el, ec, er, tl, tc, tr;
+    if (adj != 0u) {
+        el = 0.0f; er = 0.0f; ec = adj == 1u ? two_g : -two_g;
+        tl = 0.0f; tr = 0.0f; tc = sig * Dd * sa;
+    } else {
+        const float q = ks == 1u ? 0.5f * g : (ks == 3u ? 0.0f : theta * g);
+        // E pattern of the candidate: (c - l), (r - l), (r - c)
+        el = (ks == 0u || ks == 1u) ? -q : 0.0f;
+        ec = ks == 0u ? q : (ks == 2u ? -q : 0.0f);
+        er = (ks == 1u || ks == 2u) ? q : 0.0f;
+        const float m = q * sig * D * f;
+        // f*t pattern: (l, c) for 0, (l, 2c, r) for 1, (c, r) for 2
+        tl = (ks == 0u || ks == 1u) ? m * sl : 0.0f;
+        tc = ks == 1u ? 2.0f * m * sc : (ks == 3u ? 0.0f : m * sc);
+        tr = (ks == 1u || ks == 2u) ? m * sr : 0.0f;
+    }
+    tc += -sig * Dd * sc;                       // T's own tangential term
+    gE[0] += el * gK; gE[1] += ec * gK; gE[2] += er * gK;
+    gt[0] += tl * gK; gt[1] += tc * gK; gt[2] += tr * gK;
+}
+
+// VJP of the central-upwind flux in (depth, normal velocity, tangential
+// velocity) form: mass = F0, normal momentum = F1, tangential momentum = F2
+// (CDKLM16_flux, CDKLM16_kernel.cu:101-140), branch choices from `code`.
+__host__ __device__ inline void flux_vjp(float hm, float nm, float tm, float hp, float np_, float tp,
+                                         uint32_t code, float g, float cutoff,
+                                         float l0, float l1, float l2,
+                                         float& ghm, float& gnm, float& gtm,
+                                         float& ghp, float& gnp, float& gtp) {
+    ghm = gnm = gtm = ghp = gnp = gtp = 0.0f;
+    if (code & 16u) return;                       // |a+ - a-| < flux_eps: zero flux
+    const bool wp = hp > cutoff, wm = hm > cutoff;
+    float Fpx = 0.0f, Fpy = 0.0f, c_p = 0.0f, u_p = 0.0f;
+    if (wp) { Fpx = hp * np_; Fpy = hp * np_ * np_ + 0.5f * g * hp * hp; u_p = np_; c_p = adj_sqrt(g * hp); }
+    float Fmx = 0.0f, Fmy = 0.0f, c_m = 0.0f, u_m = 0.0f;
+    if (wm) { Fmx = hm * nm; Fmy = hm * nm * nm + 0.5f * g * hm * hm; u_m = nm; c_m = adj_sqrt(g * hm); }
+    const uint32_t cap = code & 3u, cam = (code >> 2) & 3u;
+    const float ap = cap == 1u ? u_m + c_m : (cap == 2u ? u_p + c_p : 0.0f);
+    const float am = cam == 1u ? u_m - c_m : (cam == 2u ? u_p - c_p : 0.0f);
+    const float rden = adj_rcp(ap - am);
+    const float apam = ap * am;
+    const float F0 = ((ap * Fmx - am * Fpx) + apam * (hp - hm)) * rden;
+    const float F1 = ((ap * Fmy - am * Fpy) + apam * (Fpx - Fmx)) * rden;
+    const float g0 = l0 * rden, g1 = l1 * rden;
+    const float gden = -(l0 * F0 + l1 * F1) * rden;
+    const float gap = g0 * (Fmx + am * (hp - hm)) + g1 * (Fmy + am * (Fpx - Fmx)) + gden;
+    const float gam = g0 * (-Fpx + ap * (hp - hm)) + g1 * (-Fpy + ap * (Fpx - Fmx)) - gden;
+    const float gFmx = g0 * ap - g1 * apam;
+    const float gFpx = -g0 * am + g1 * apam;
+    const float gFmy = g1 * ap;
+    const float gFpy = -g1 * am;
+    ghp += g0 * apam;
+    ghm -= g0 * apam;
+    // tangential momentum upwinded on the sign of nm + np (recomputed exactly)
+    const float gFmz = (nm + np_ > 0.0f) ? l2 : 0.0f;
+    const float gFpz = (nm + np_ > 0.0f) ? 0.0f : l2;
+    float gum = 0.0f, gcm = 0.0f, gup = 0.0f, gcp = 0.0f;
+    if (cap == 1u) { gum += gap; gcm += gap; } else if (cap == 2u) { gup += gap; gcp += gap; }
+    if (cam == 1u) { gum += gam; gcm -= gam; } else if (cam == 2u) { gup += gam; gcp -= gam; }
+    if (wp) {
+        ghp += gFpx * np_ + gFpy * (np_ * np_ + g * hp) + gFpz * np_ * tp + gcp * g * 0.5f * adj_rcp(c_p);
+        gnp += gFpx * hp + gFpy * 2.0f * hp * np_ + gFpz * hp * tp + gup;
+        gtp += gFpz * hp * np_;
+    }
+    if (wm) {
+        ghm += gFmx * nm + gFmy * (nm * nm + g * hm) + gFmz * nm * tm + gcm * g * 0.5f * adj_rcp(c_m);
+        gnm += gFmx * hm + gFmy * 2.0f * hm * nm + gFmz * hm * tm + gum;
+        gtm += gFmz * hm * nm;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS of one adjoint tile.  Cell (R, C) sits at [R - by + 1][C - bx + 1]
+// (a 3-cell halo around the BW x BH block of interior cells).
+// ---------------------------------------------------------------------------
+template <int BW, int BH>
+struct AdjSmem {
+    static constexpr int W = BW + 6, H = BH + 6;
+    static constexpr int NF = (BH * W > H * BW) ? BH * W : H * BW;
+    float Hi[H + 1][W + 1];
+    float E[H][W], U[H][W], V[H][W], T[H][W], S[H][W], h[H][W];
+    uint32_t code[H][W];                 // record word; bits 28-29: conversion branch
+    float W0[H][W], W1[H][W], W2[H][W];  // lambda at interior cells, clamp-masked
+    float A0[H][W], A1[H][W], A2[H][W];  // gathered d/d(E, u, v) of output cells
+    float FD[6][NF];                     // face -> (E, u, v) of its minus and plus cell
+    float FS[6][NF];                     // face -> (T, u slope, v slope) of minus / plus
+    float CE[9][NF];                     // slope centre -> (E, u, v) of cells -1, 0, +1
+};
+
+enum { kBrLand = 0, kBrDry = 1, kBrDesing = 2, kBrWet = 3 };
+
+template <int BW, int BH>
+struct AdjTile {
+    using Smem = AdjSmem<BW, BH>;
+    static constexpr int W = Smem::W, H = Smem::H;
+    const AdjTileArgs& A;
+    Smem& S;
+    int bx, by;                  // padded index of the block's first interior cell is (by+2, bx+2)
+    int C0, C1, R0, R1;          // output cell range (ghost cells included)
+    int fx0, fx1, fy0, fy1;      // x faces (minus cell column) / y faces (minus cell row) computed
+    float dt;
+
+    __host__ __device__ AdjTile(const AdjTileArgs& a, Smem& s, int tx, int ty) : A(a), S(s) {
+        bx = tx * BW;
+        by = ty * BH;
+        const bool own_e = A.nx + 1 >= bx + 2 && A.nx + 1 < bx + BW + 2;
+        const bool own_n = A.ny + 1 >= by + 2 && A.ny + 1 < by + BH + 2;
+        C0 = tx == 0 ? 0 : bx + 2;
+        C1 = own_e ? A.nx + 3 : bx + BW + 1;
+        R0 = ty == 0 ? 0 : by + 2;
+        R1 = own_n ? A.ny + 3 : by + BH + 1;
+        fx0 = C0 - 2 > 1 ? C0 - 2 : 1;
+        fx1 = C1 + 1 < A.nx + 1 ? C1 + 1 : A.nx + 1;
+        fy0 = R0 - 2 > 1 ? R0 - 2 : 1;
+        fy1 = R1 + 1 < A.ny + 1 ? R1 + 1 : A.ny + 1;
+        dt = A.dt_dev ? *A.dt_dev : A.dt;
+    }
+
+    __host__ __device__ int Rof(int lr) const { return by - 1 + lr; }
+    __host__ __device__ int Cof(int lc) const { return bx - 1 + lc; }
+    __host__ __device__ bool ghost_r(int R) const { return R < 2 || R > A.ny + 1; }
+    __host__ __device__ bool ghost_c(int C) const { return C < 2 || C > A.nx + 1; }
+    __host__ __device__ bool is_output(int R, int C) const {
+        return R >= R0 && R <= R1 && C >= C0 && C <= C1 && !(ghost_r(R) && ghost_c(C));
+    }
+    __host__ __device__ float hflag(float v) const {
+        return (fabsf(v - A.land) < kDryEps) ? kDryFlag : v;
+    }
+    // tangential-velocity sign of a wall ghost column / row in the K / L terms
+    __host__ __device__ float col_flip(int C) const {
+        return ((A.bc_w == kWall && C < 2) || (A.bc_e == kWall && C > A.nx + 1)) ? -1.0f : 1.0f;
+    }
+    __host__ __device__ float row_flip(int R) const {
+        return ((A.bc_s == kWall && R < 2) || (A.bc_n == kWall && R > A.ny + 1)) ? -1.0f : 1.0f;
+    }
+    __host__ __device__ bool clamped(const float* out_eta, int lr, int lc, size_t o) const {
+        if (!out_eta) return false;
+        const float Hreg = 0.25f * (((S.Hi[lr][lc] + S.Hi[lr + 1][lc]) + S.Hi[lr][lc + 1]) + S.Hi[lr + 1][lc + 1]);
+        return out_eta[o] <= -Hreg + A.cutoff;
+    }
+
+    // ---- stage 0: corner depths, land flagged
+    __host__ __device__ void stage_hi(int it) {
+        const int lr = it / (W + 1), lc = it - lr * (W + 1);
+        const int R = Rof(lr), C = Cof(lc);
+        float v = 0.0f;
+        if (R >= 0 && R <= A.ny + 4 && C >= 0 && C <= A.nx + 4) v = hflag(A.Hi[(size_t)R * A.hi_pitch + C]);
+        S.Hi[lr][lc] = v;
+    }
+    static constexpr int kHiItems = (H + 1) * (W + 1);
+
+    // ---- stage 1: cell data
+    __host__ __device__ void stage_cells(int it) {
+        const int lr = it / W, lc = it - lr * W;
+        const int R = Rof(lr), C = Cof(lc);
+        const int nx = A.nx, ny = A.ny;
+        float E = 0.0f, U = 0.0f, V = 0.0f, T = 0.0f, Sv = 0.0f, h = 0.0f;
+        float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
+        uint32_t code = 0u;
+        const bool in_band = (lr >= 3 && lr < BH + 3) || (lc >= 3 && lc < BW + 3);
+        if (in_band && R >= 0 && R <= ny + 3 && C >= 0 && C <= nx + 3 && !(ghost_r(R) && ghost_c(C))) {
+            // eta of the image cell (wall mirror / periodic wrap)
+            int sr = R, sc = C;
+            if (R < 2) sr = (A.bc_s == kWall) ? 3 - R : R + ny;
+            else if (R > ny + 1) sr = (A.bc_n == kWall) ? 2 * ny + 3 - R : R - ny;
+            if (C < 2) sc = (A.bc_w == kWall) ? 3 - C : C + nx;
+            else if (C > nx + 1) sc = (A.bc_e == kWall) ? 2 * nx + 3 - C : C - nx;
+            const float eta = A.eta[(size_t)sr * A.pitch + sc];
+            const float a = S.Hi[lr][lc], b = S.Hi[lr + 1][lc], c = S.Hi[lr][lc + 1], d = S.Hi[lr + 1][lc + 1];
+            const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag) - (float)(c == kDryFlag)
+                              - (float)(d == kDryFlag);
+            const float sum = ((a + b) + c) + d;
+            const float Hm = (wet == 4.0f) ? sum * 0.25f : (wet == 0.0f ? A.land : sum / wet);
+            h = eta + Hm;
+            uint32_t br;
+            if (fabsf(Hm - A.land) <= kDryEps) { E = kDryFlag; br = kBrLand; }
+            else if (h < A.desing) {
+                if (h <= A.cutoff) { E = -Hm + A.cutoff; br = kBrDry; }
+                else { E = eta; br = kBrDesing; }
+            } else { E = eta; br = kBrWet; }
+            const size_t o = (size_t)R * A.pitch + C;
+            U = A.tU[o];
+            V = A.tV[o];
+            const bool ri = !ghost_r(R), ci = !ghost_c(C);
+            code = br << 28;
+            if (ri && C >= 1 && C <= nx + 2) { T = A.tT[o]; code |= A.tC[o] & 0x001F00FFu; }
+            if (ci && R >= 1 && R <= ny + 2) { Sv = A.tS[o]; code |= A.tC[o] & 0x03E0FF00u; }
+            if (ri && ci) {
+                if (!clamped(A.w_clamp, lr, lc, o)) { w0 = A.w_eta[o]; w1 = A.w_hu[o]; w2 = A.w_hv[o]; }
+            }
+        }
+        S.E[lr][lc] = E; S.U[lr][lc] = U; S.V[lr][lc] = V; S.T[lr][lc] = T; S.S[lr][lc] = Sv;
+        S.h[lr][lc] = h; S.code[lr][lc] = code;
+        S.W0[lr][lc] = w0; S.W1[lr][lc] = w1; S.W2[lr][lc] = w2;
+        S.A0[lr][lc] = 0.0f; S.A1[lr][lc] = 0.0f; S.A2[lr][lc] = 0.0f;
+    }
+    static constexpr int kCellItems = H * W;
+
+    // ---- x pass --------------------------------------------------------
+    // faces: row j (lr = j + 3), minus cell lc = fi; index j * W + fi
+    static constexpr int kXItems = BH * W;
+    __host__ __device__ bool x_row_ok(int j) const { return by + 2 + j <= A.ny + 1; }
+
+    __host__ __device__ void stage_xface(int it) {
+        const int j = it / W, fi = it - j * W;
+        const int lr = j + 3, cm = Cof(fi);
+        float out[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (x_row_ok(j) && cm >= fx0 && cm <= fx1 && fi >= 1 && fi + 2 < W) {
+            const int m = fi, p = fi + 1;
+            const float Em = S.E[lr][m], Ep = S.E[lr][p];
+            if (!(Em == kDryFlag || Ep == kDryFlag)) {
+                const uint32_t km = S.code[lr][m], kp = S.code[lr][p];
+                const float rdx = 1.0f / A.dx;
+                const float l0 = dt * (S.W0[lr][p] - S.W0[lr][m]) * rdx;
+                const float l1 = dt * (S.W1[lr][p] - S.W1[lr][m]) * rdx;
+                const float l2 = dt * (S.W2[lr][p] - S.W2[lr][m]) * rdx;
+                const float th = A.theta;
+                const float Sum = slope_val(km & 3u, S.U[lr][m - 1], S.U[lr][m], S.U[lr][m + 1], th);
+                const float Svm = slope_val((km >> 2) & 3u, S.V[lr][m - 1], S.V[lr][m], S.V[lr][m + 1], th);
+                const float Sup = slope_val(kp & 3u, S.U[lr][p - 1], S.U[lr][p], S.U[lr][p + 1], th);
+                const float Svp = slope_val((kp >> 2) & 3u, S.V[lr][p - 1], S.V[lr][p], S.V[lr][p + 1], th);
+                const float Rpx = S.U[lr][p] - 0.5f * Sup, Rpy = S.V[lr][p] - 0.5f * Svp;
+                const float Rmx = S.U[lr][m] + 0.5f * Sum, Rmy = S.V[lr][m] + 0.5f * Svm;
+                const float Hf = 0.5f * (S.Hi[lr][p] + S.Hi[lr + 1][p]);
+                const float hp_raw = (Ep + Hf) - S.T[lr][p];
+                const float hm_raw = (Em + Hf) + S.T[lr][m];
+                const float hp = fmaxf(0.0f, hp_raw), hm = fmaxf(0.0f, hm_raw);
+                float ghm, gnm, gtm, ghp, gnp, gtp;
+                flux_vjp(hm, Rmx, Rmy, hp, Rpx, Rpy, (km >> kCodeFx) & 31u, A.g, A.cutoff, l0, l1, l2,
+                         ghm, gnm, gtm, ghp, gnp, gtp);
+                const float ghm_r = hm_raw > 0.0f ? ghm : 0.0f;
+                const float ghp_r = hp_raw > 0.0f ? ghp : 0.0f;
+                out[0] = ghm_r; out[1] = gnm; out[2] = gtm;          // direct: E, u, v of minus cell
+                out[3] = ghp_r; out[4] = gnp; out[5] = gtp;          // and of plus cell
+                out[6] = ghm_r; out[7] = 0.5f * gnm; out[8] = 0.5f * gtm;     // T, u slope, v slope (minus)
+                out[9] = -ghp_r; out[10] = -0.5f * gnp; out[11] = -0.5f * gtp; // (plus)
+            }
+        }
+        const int k = j * W + fi;
+        for (int q = 0; q < 6; ++q) { S.FD[q][k] = out[q]; S.FS[q][k] = out[6 + q]; }
+    }
+
+    // slope centre c (lc = fi, row lr): gather from its two faces, apply the
+    // u / v slope VJPs and the depth-term VJP (K with minmod / adjustment)
+    __host__ __device__ void stage_xcentre(int it) {
+        const int j = it / W, fi = it - j * W;
+        const int lr = j + 3, c = Cof(fi);
+        float out[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (x_row_ok(j) && c >= fx0 && c <= fx1 + 1 && fi >= 1 && fi + 1 < W) {
+            const int kl = j * W + fi - 1, kr = j * W + fi;   // face to the left (c is plus), right (c is minus)
+            const float gT = S.FS[3][kl] + S.FS[0][kr];
+            const float gSu = S.FS[4][kl] + S.FS[1][kr];
+            const float gSv = S.FS[5][kl] + S.FS[2][kr];
+            const uint32_t code = S.code[lr][fi];
+            const float th = A.theta;
+            float cl, cc, cr;
+            slope_coef(code & 3u, th, cl, cc, cr);
+            out[1] = cl * gSu; out[4] = cc * gSu; out[7] = cr * gSu;
+            slope_coef((code >> 2) & 3u, th, cl, cc, cr);
+            out[2] = cl * gSv; out[5] = cc * gSv; out[8] = cr * gSv;
+            // T = (K + dx f s_c v_c) / 2g: x direction, sigma = -1, t = v
+            const int C = c;
+            const float sa = (A.bc_w == kWall && C < 2) ? -1.0f : 1.0f;
+            float gE[3] = {0.0f, 0.0f, 0.0f}, gt[3] = {0.0f, 0.0f, 0.0f};
+            depth_term_vjp((code >> 4) & 3u, (code >> 6) & 3u, -1.0f, th, A.g, A.f, A.dx * A.f,
+                           A.dx / (2.0f * A.g), col_flip(C - 1), col_flip(C), col_flip(C + 1), sa, gT, gE, gt);
+            out[0] += gE[0]; out[3] += gE[1]; out[6] += gE[2];
+            out[2] += gt[0]; out[5] += gt[1]; out[8] += gt[2];
+        }
+        const int k = j * W + fi;
+        for (int q = 0; q < 9; ++q) S.CE[q][k] = out[q];
+    }
+
+    __host__ __device__ void stage_xgather(int it) {
+        const int j = it / W, fi = it - j * W;
+        const int lr = j + 3;
+        if (!x_row_ok(j) || fi < 1 || fi + 1 >= W) return;
+        if (!is_output(Rof(lr), Cof(fi))) return;
+        const int k = j * W + fi;
+        // faces: left (cell is plus), right (cell is minus); centres -1, 0, +1
+        S.A0[lr][fi] += ((S.FD[3][k - 1] + S.FD[0][k]) + (S.CE[6][k - 1] + S.CE[3][k])) + S.CE[0][k + 1];
+        S.A1[lr][fi] += ((S.FD[4][k - 1] + S.FD[1][k]) + (S.CE[7][k - 1] + S.CE[4][k])) + S.CE[1][k + 1];
+        S.A2[lr][fi] += ((S.FD[5][k - 1] + S.FD[2][k]) + (S.CE[8][k - 1] + S.CE[5][k])) + S.CE[2][k + 1];
+    }
+
+    // ---- y pass --------------------------------------------------------
+    // faces: column i (lc = i + 3), minus cell lr = fj; index fj * BW + i
+    static constexpr int kYItems = H * BW;
+    __host__ __device__ bool y_col_ok(int i) const { return bx + 2 + i <= A.nx + 1; }
+
+    __host__ __device__ void stage_yface(int it) {
+        const int fj = it / BW, i = it - fj * BW;
+        const int lc = i + 3, rm = Rof(fj);
+        float out[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (y_col_ok(i) && rm >= fy0 && rm <= fy1 && fj >= 1 && fj + 2 < H) {
+            const int m = fj, p = fj + 1;
+            const float Em = S.E[m][lc], Ep = S.E[p][lc];
+            if (!(Em == kDryFlag || Ep == kDryFlag)) {
+                const uint32_t km = S.code[m][lc], kp = S.code[p][lc];
+                const float rdy = 1.0f / A.dy;
+                // flux order (mass, normal = v momentum, tangential = u momentum)
+                const float l0 = dt * (S.W0[p][lc] - S.W0[m][lc]) * rdy;
+                const float l1 = dt * (S.W2[p][lc] - S.W2[m][lc]) * rdy;
+                const float l2 = dt * (S.W1[p][lc] - S.W1[m][lc]) * rdy;
+                const float th = A.theta;
+                const float Sum = slope_val((km >> 8) & 3u, S.U[m - 1][lc], S.U[m][lc], S.U[m + 1][lc], th);
+                const float Svm = slope_val((km >> 10) & 3u, S.V[m - 1][lc], S.V[m][lc], S.V[m + 1][lc], th);
+                const float Sup = slope_val((kp >> 8) & 3u, S.U[p - 1][lc], S.U[p][lc], S.U[p + 1][lc], th);
+                const float Svp = slope_val((kp >> 10) & 3u, S.V[p - 1][lc], S.V[p][lc], S.V[p + 1][lc], th);
+                const float Rpx = S.U[p][lc] - 0.5f * Sup, Rpy = S.V[p][lc] - 0.5f * Svp;
+                const float Rmx = S.U[m][lc] + 0.5f * Sum, Rmy = S.V[m][lc] + 0.5f * Svm;
+                const float Hf = 0.5f * (S.Hi[p][lc] + S.Hi[p][lc + 1]);
+                const float hp_raw = (Ep + Hf) - S.S[p][lc];
+                const float hm_raw = (Em + Hf) + S.S[m][lc];
+                const float hp = fmaxf(0.0f, hp_raw), hm = fmaxf(0.0f, hm_raw);
+                float ghm, gnm, gtm, ghp, gnp, gtp;
+                flux_vjp(hm, Rmy, Rmx, hp, Rpy, Rpx, (km >> kCodeFy) & 31u, A.g, A.cutoff, l0, l1, l2,
+                         ghm, gnm, gtm, ghp, gnp, gtp);
+                const float ghm_r = hm_raw > 0.0f ? ghm : 0.0f;
+                const float ghp_r = hp_raw > 0.0f ? ghp : 0.0f;
+                // back to (E, u, v): normal = v, tangential = u
+                out[0] = ghm_r; out[1] = gtm; out[2] = gnm;
+                out[3] = ghp_r; out[4] = gtp; out[5] = gnp;
+                out[6] = ghm_r; out[7] = 0.5f * gtm; out[8] = 0.5f * gnm;
+                out[9] = -ghp_r; out[10] = -0.5f * gtp; out[11] = -0.5f * gnp;
+            }
+        }
+        const int k = fj * BW + i;
+        for (int q = 0; q < 6; ++q) { S.FD[q][k] = out[q]; S.FS[q][k] = out[6 + q]; }
+    }
+
+    __host__ __device__ void stage_ycentre(int it) {
+        const int fj = it / BW, i = it - fj * BW;
+        const int lc = i + 3, c = Rof(fj);
+        float out[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (y_col_ok(i) && c >= fy0 && c <= fy1 + 1 && fj >= 1 && fj + 1 < H) {
+            const int kl = (fj - 1) * BW + i, kr = fj * BW + i;
+            const float gT = S.FS[3][kl] + S.FS[0][kr];
+            const float gSu = S.FS[4][kl] + S.FS[1][kr];
+            const float gSv = S.FS[5][kl] + S.FS[2][kr];
+            const uint32_t code = S.code[fj][lc] >> 8;
+            const float th = A.theta;
+            float cl, cc, cr;
+            slope_coef(code & 3u, th, cl, cc, cr);
+            out[1] = cl * gSu; out[4] = cc * gSu; out[7] = cr * gSu;
+            slope_coef((code >> 2) & 3u, th, cl, cc, cr);
+            out[2] = cl * gSv; out[5] = cc * gSv; out[8] = cr * gSv;
+            // S = (L - dy f s_c u_c) / 2g: y direction, sigma = +1, t = u
+            const int R = c;
+            const float sa = (A.bc_s == kWall && R < 2) ? -1.0f : 1.0f;
+            float gE[3] = {0.0f, 0.0f, 0.0f}, gt[3] = {0.0f, 0.0f, 0.0f};
+            depth_term_vjp((code >> 4) & 3u, (code >> 6) & 3u, 1.0f, th, A.g, A.f, A.dy * A.f,
+                           A.dy / (2.0f * A.g), row_flip(R - 1), row_flip(R), row_flip(R + 1), sa, gT, gE, gt);
+            out[0] += gE[0]; out[3] += gE[1]; out[6] += gE[2];
+            out[1] += gt[0]; out[4] += gt[1]; out[7] += gt[2];
+        }
+        const int k = fj * BW + i;
+        for (int q = 0; q < 9; ++q) S.CE[q][k] = out[q];
+    }
+
+    __host__ __device__ void stage_ygather(int it) {
+        const int fj = it / BW, i = it - fj * BW;
+        const int lc = i + 3;
+        if (!y_col_ok(i) || fj < 1 || fj + 1 >= H) return;
+        if (!is_output(Rof(fj), Cof(lc))) return;
+        const int k = fj * BW + i;
+        S.A0[fj][lc] += ((S.FD[3][k - BW] + S.FD[0][k]) + (S.CE[6][k - BW] + S.CE[3][k])) + S.CE[0][k + BW];
+        S.A1[fj][lc] += ((S.FD[4][k - BW] + S.FD[1][k]) + (S.CE[7][k - BW] + S.CE[4][k])) + S.CE[1][k + BW];
+        S.A2[fj][lc] += ((S.FD[5][k - BW] + S.FD[2][k]) + (S.CE[8][k - BW] + S.CE[5][k])) + S.CE[2][k + BW];
+    }
+
+    // ---- output: base / source VJP (interior cells), conversion VJP, store
+    __host__ __device__ void stage_out(int it) {
+        const int lr = it / W, lc = it - lr * W;
+        const int R = Rof(lr), C = Cof(lc);
+        if (lr < 1 || lc < 1 || lr + 1 >= H || lc + 1 >= W || !is_output(R, C)) return;
+        float gE = S.A0[lr][lc], gU = S.A1[lr][lc], gV = S.A2[lr][lc];
+        const size_t o = (size_t)R * A.pitch + C;
+        const bool interior = !ghost_r(R) && !ghost_c(C);
+        const float E = S.E[lr][lc], U = S.U[lr][lc], V = S.V[lr][lc];
+        float xe = 0.0f, xu = 0.0f, xv = 0.0f;
+        if (interior) {
+            const float w0 = S.W0[lr][lc], w1 = S.W1[lr][lc], w2 = S.W2[lr][lc];
+            const float a = S.Hi[lr][lc], b = S.Hi[lr + 1][lc], c = S.Hi[lr][lc + 1], d = S.Hi[lr + 1][lc + 1];
+            const float Hreg = 0.25f * (((a + b) + c) + d);
+            const float hreg = E + Hreg;
+            gE += w0;
+            float ghu = w1, ghv = w2;
+            if (hreg >= A.cutoff && E != kDryFlag) {
+                const float H_x = 0.5f * (c + d) - 0.5f * (a + b);
+                const float H_y = 0.5f * (b + d) - 0.5f * (a + c);
+                ghv += dt * w1 * A.f;
+                gE += dt * w1 * A.g * H_x / A.dx;
+                ghu += -dt * w2 * A.f;
+                gE += dt * w2 * A.g * H_y / A.dy;
+            }
+            if (hreg > A.cutoff) {
+                gU += ghu * hreg; gE += ghu * U;
+                gV += ghv * hreg; gE += ghv * V;
+            }
+            if (A.x_eta && !clamped(A.x_clamp, lr, lc, o)) { xe = A.x_eta[o]; xu = A.x_hu[o]; xv = A.x_hv[o]; }
+        }
+        // conversion (eta, hu, hv) -> (E, u, v)
+        const uint32_t br = S.code[lr][lc] >> 28;
+        const float h = S.h[lr][lc];
+        float g_eta = 0.0f, g_hu = 0.0f, g_hv = 0.0f;
+        if (br == kBrWet) {
+            const float rh = adj_rcp(h);
+            g_eta = gE - (gU * U + gV * V) * rh;
+            g_hu = gU * rh;
+            g_hv = gV * rh;
+        } else if (br == kBrDesing) {
+            const float eps = A.desing;
+            const float aq = h * h / (2.0f * eps) + 0.5f * eps;
+            const float mn = fminf(aq, eps);
+            const float den = fmaxf(mn, fabsf(h));
+            float dden = 0.0f;
+            if (fabsf(h) >= mn) dden = (h >= 0.0f) ? 1.0f : -1.0f;
+            else if (aq <= eps) dden = h / eps;
+            const float rden = adj_rcp(den);
+            g_eta = gE - (gU * U + gV * V) * dden * rden;
+            g_hu = gU * rden;
+            g_hv = gV * rden;
+        }
+        A.o_eta[o] = A.alpha * g_eta + A.gamma * xe;
+        A.o_hu[o] = A.alpha * g_hu + A.gamma * xu;
+        A.o_hv[o] = A.alpha * g_hv + A.gamma * xv;
+    }
+};
+
+#ifdef __HIP_DEVICE_COMPILE__
+#define ADJ_SYNC() __syncthreads()
+#else
+#define ADJ_SYNC()
+#endif
+
+// every stage of one tile, items spread over NT threads; the host build runs
+// the same body with one "thread" that loops over all items
+template <int BW, int BH, int NT>
+__host__ __device__ inline void adj_tile_body(const AdjTileArgs& A, AdjSmem<BW, BH>& S, int tx, int ty,
+                                             int tid, int nthreads) {
+    using Tl = AdjTile<BW, BH>;
+    Tl T(A, S, tx, ty);
+    for (int it = tid; it < Tl::kHiItems; it += nthreads) T.stage_hi(it);
+    ADJ_SYNC();
+    for (int it = tid; it < Tl::kCellItems; it += nthreads) T.stage_cells(it);
+    ADJ_SYNC();
+    for (int it = tid; it < Tl::kXItems; it += nthreads) T.stage_xface(it);
+    ADJ_SYNC();
+    for (int it = tid; it < Tl::kXItems; it += nthreads) T.stage_xcentre(it);
+    ADJ_SYNC();
+    for (int it = tid; it < Tl::kXItems; it += nthreads) T.stage_xgather(it);
+    ADJ_SYNC();
+    for (int it = tid; it < Tl::kYItems; it += nthreads) T.stage_yface(it);
+    ADJ_SYNC();
+    for (int it = tid; it < Tl::kYItems; it += nthreads) T.stage_ycentre(it);
+    ADJ_SYNC();
+    for (int it = tid; it < Tl::kYItems; it += nthreads) T.stage_ygather(it);
+    ADJ_SYNC();
+    for (int it = tid; it < Tl::kCellItems; it += nthreads) T.stage_out(it);
+}
+
+constexpr int kAdjBW = 32, kAdjBH = 8, kAdjNT = 256;
+
+__global__ __launch_bounds__(kAdjNT) void k_adj_tile(AdjTileArgs A) {
+    __shared__ AdjSmem<kAdjBW, kAdjBH> S;
+    const int t = xcd_band(blockIdx.x, gridDim.x);
+    const int ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
+    adj_tile_body<kAdjBW, kAdjBH, kAdjNT>(A, S, tx, ty, threadIdx.x, kAdjNT);
+}
+
+// Ghost cells back to the interior cells they are images of.  One thread per
+// interior cell within two cells of the boundary; ghost entries are read by
+// exactly one thread and cleared by it; the 16 corner entries are cleared by
+// the last block row.
+struct FoldArgs {
+    float* e; float* u; float* v;
+    int nx, ny, pitch;
     int bc_n, bc_e, bc_s, bc_w;
 };
 
-struct Cell {
-    float E, U, V;
-};
-
-// state at padded (R, C) read the way the forward tile reads it (wall mirror
-// with the normal momentum negated, periodic wrap), converted to (E, u, v)
-__host__ __device__ inline Cell adj_cell(const AdjArgs& A, int R, int C, float* h_out = nullptr,
-                                         int* br = nullptr, float* Hm_out = nullptr) {
-    const int nx = A.nx, ny = A.ny;
-    int sr = R, sc = C;
-    bool nu = false, nv = false;
-    const bool wN = A.bc_n == kWall, wS = A.bc_s == kWall, wE = A.bc_e == kWall, wW = A.bc_w == kWall;
-    const bool pNS = A.bc_n == kPeriodic, pEW = A.bc_e == kPeriodic;
-    if (R < 2) { if (wS) { sr = 3 - R; nv = true; } else if (pNS) sr = R + ny; }
-    else if (R > ny + 1) { if (wN) { sr = 2 * ny + 3 - R; nv = true; } else if (pNS) sr = R - ny; }
-    if (C < 2) { if (wW) { sc = 3 - C; nu = true; } else if (pEW) sc = C + nx; }
-    else if (C > nx + 1) { if (wE) { sc = 2 * nx + 3 - C; nu = true; } else if (pEW) sc = C - nx; }
-    const size_t o = (size_t)sr * A.pitch + sc;
-    const float eta = A.eta[o];
-    float hu = A.hu[o], hv = A.hv[o];
-    if (nu) hu = -hu;
-    if (nv) hv = -hv;
-    auto flag = [&](float v) { return (fabsf(v - A.land) < kDryEps) ? kDryFlag : v; };
-    const float a = flag(A.Hi[(size_t)R * A.hi_pitch + C]);
-    const float b = flag(A.Hi[(size_t)(R + 1) * A.hi_pitch + C]);
-    const float c = flag(A.Hi[(size_t)R * A.hi_pitch + C + 1]);
-    const float d = flag(A.Hi[(size_t)(R + 1) * A.hi_pitch + C + 1]);
-    const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag) - (float)(c == kDryFlag) - (float)(d == kDryFlag);
-    const float sum = ((a + b) + c) + d;
-    const float Hm = (wet == 0.0f) ? A.land : sum / wet;
-    const float h = eta + Hm;
-    Cell r;
-    int branch;
-    if (fabsf(Hm - A.land) <= kDryEps) { r.E = kDryFlag; r.U = 0.0f; r.V = 0.0f; branch = 0; }
-    else if (h < A.desing) {
-        if (h <= A.cutoff) { r.E = -Hm + A.cutoff; r.U = 0.0f; r.V = 0.0f; branch = 1; }
-        else { r.E = eta; r.U = desingularize(h, hu, A.desing); r.V = desingularize(h, hv, A.desing); branch = 2; }
-    } else { r.E = eta; r.U = hu / h; r.V = hv / h; branch = 3; }
-    if (h_out) *h_out = h;
-    if (br) *br = branch;
-    if (Hm_out) *Hm_out = Hm;
-    return r;
-}
-
-__host__ __device__ inline float hi_flag(const AdjArgs& A, int r, int c) {
-    const float v = A.Hi[(size_t)r * A.hi_pitch + c];
-    return (fabsf(v - A.land) < kDryEps) ? kDryFlag : v;
-}
-
-// true where the forward update clamped cell (R, C) to the dry state
-// (eta = -Hm + cutoff, hu = hv = 0): that output is constant, so the adjoint
-// arriving at it is dropped (CDKLM16_kernel.cu:986-1001)
-__host__ __device__ inline bool dry_clamped(const AdjArgs& A, const float* out_eta, int R, int C) {
-    if (!out_eta) return false;
-    const float Hreg = 0.25f * (((hi_flag(A, R, C) + hi_flag(A, R + 1, C)) + hi_flag(A, R, C + 1))
-                                + hi_flag(A, R + 1, C + 1));
-    return out_eta[(size_t)R * A.pitch + C] <= -Hreg + A.cutoff;
-}
-
-// index of the minmod candidate that is the result (0,1,2) or -1 for zero
-__host__ __device__ inline int minmod_sel(float b, float c, float f) {
-    const uint32_t ub = __builtin_bit_cast(uint32_t, b), uc = __builtin_bit_cast(uint32_t, c);
-    const uint32_t uf = __builtin_bit_cast(uint32_t, f);
-    if ((((ub ^ uc) | (uc ^ uf)) & 0x80000000u) != 0u) return -1;
-    const float ab = fabsf(b), ac = fabsf(c), af = fabsf(f);
-    if (ab <= ac && ab <= af) return 0;
-    if (ac <= af) return 1;
-    return 2;
-}
-
-// VJP of the central-upwind flux (CDKLM16_flux) w.r.t. (hm,um,vm,hp,up,vp)
-__host__ __device__ inline void flux_vjp(float hm, float um, float vm, float hp, float up, float vp,
-                         float g, float cutoff, float feps, float gx, float gy, float gz,
-                         float& ghm, float& gum, float& gvm, float& ghp, float& gup, float& gvp) {
-    ghm = gum = gvm = ghp = gup = gvp = 0.0f;
-    const bool wp = hp > cutoff, wm = hm > cutoff;
-    float Fpx = 0, Fpy = 0, u_p = 0, c_p = 0;
-    if (wp) { Fpx = hp * up; Fpy = hp * up * up + 0.5f * g * hp * hp; u_p = up; c_p = sqrtf(g * hp); }
-    float Fmx = 0, Fmy = 0, u_m = 0, c_m = 0;
-    if (wm) { Fmx = hm * um; Fmy = hm * um * um + 0.5f * g * hm * hm; u_m = um; c_m = sqrtf(g * hm); }
-    const float a1 = u_m - c_m, a2 = u_p - c_p;
-    const float b1 = u_m + c_m, b2 = u_p + c_p;
-    const float am = fminf(fminf(a1, a2), 0.0f);
-    const float ap = fmaxf(fmaxf(b1, b2), 0.0f);
-    const float den = ap - am;
-    if (fabsf(den) < feps) return;
-    const float apam = ap * am;
-    const float Fx = ((ap * Fmx - am * Fpx) + apam * (hp - hm)) / den;
-    const float Fy = ((ap * Fmy - am * Fpy) + apam * (Fpx - Fmx)) / den;
-    const float gNx = gx / den, gNy = gy / den;
-    const float gden = -(gx * Fx + gy * Fy) / den;
-    const float gap = gNx * (Fmx + am * (hp - hm)) + gNy * (Fmy + am * (Fpx - Fmx)) + gden;
-    const float gam = gNx * (-Fpx + ap * (hp - hm)) + gNy * (-Fpy + ap * (Fpx - Fmx)) - gden;
-    float gFmx = gNx * ap - gNy * apam;
-    float gFpx = -gNx * am + gNy * apam;
-    const float gFmy = gNy * ap;
-    const float gFpy = -gNy * am;
-    ghp += gNx * apam;
-    ghm -= gNx * apam;
-    float gFmz = 0.0f, gFpz = 0.0f;
-    if (um + up > 0.0f) gFmz = gz; else gFpz = gz;
-    float gu_m = 0, gc_m = 0, gu_p = 0, gc_p = 0;
-    if (ap > 0.0f) {                       // ap = max(b1, b2, 0)
-        if (b1 >= b2) { gu_m += gap; gc_m += gap; } else { gu_p += gap; gc_p += gap; }
-    }
-    if (am < 0.0f) {                       // am = min(a1, a2, 0)
-        if (a1 <= a2) { gu_m += gam; gc_m -= gam; } else { gu_p += gam; gc_p -= gam; }
-    }
-    if (wp) {
-        ghp += gFpx * up + gFpy * (up * up + g * hp) + gFpz * up * vp + gc_p * g / (2.0f * c_p);
-        gup += gFpx * hp + gFpy * 2.0f * hp * up + gFpz * hp * vp + gu_p;
-        gvp += gFpz * hp * up;
-    }
-    if (wm) {
-        ghm += gFmx * um + gFmy * (um * um + g * hm) + gFmz * um * vm + gc_m * g / (2.0f * c_m);
-        gum += gFmx * hm + gFmy * 2.0f * hm * um + gFmz * hm * vm + gu_m;
-        gvm += gFmz * hm * um;
-    }
-}
-
-// the minmod slope of a scalar and its VJP onto (l, c, r)
-__host__ __device__ inline void slope_vjp(float l, float c, float r, float theta, float gs,
-                                          float& gl, float& gc, float& gr) {
-    const int s = minmod_sel((c - l) * theta, (r - l) * 0.5f, (r - c) * theta);
-    if (s == 0) { gc += theta * gs; gl -= theta * gs; }
-    else if (s == 1) { gr += 0.5f * gs; gl -= 0.5f * gs; }
-    else if (s == 2) { gr += theta * gs; gc -= theta * gs; }
-}
-
-// One slope centre k (0..3 among the four cells) along a line.  dir 0: x
-// (tangential v, K term -Vd*f*v), dir 1: y (tangential u, L term +Ud*f*u).
-// Computes the primal slopes (Sn = normal-velocity slope, St = tangential,
-// Tt = face depth term T or S) and, when given adjoints, accumulates the VJP.
-struct LineCells {
-    float E[4], U[4], V[4];
-    int line_idx[4];        // global column (x) or row (y) of each cell
-};
-
-template <int DIR>
-__host__ __device__ inline void slope_term(const AdjArgs& A, const LineCells& L, int k, int other,
-                           float& Su, float& Sv, float& Tt, bool adjoint,
-                           float gT, float gSu, float gSv, float* gE, float* gU, float* gV) {
-    // other = the fixed row (x) or column (y) index of the line
-    const float g = A.g, theta = A.theta, f = A.f;
-    const float two_g = 2.0f * g;
-    const int i0 = k - 1, i1 = k, i2 = k + 1;
-    const float El = L.E[i0], Ec = L.E[i1], Er = L.E[i2];
-    const float Ul = L.U[i0], Uc = L.U[i1], Ur = L.U[i2];
-    const float Vl = L.V[i0], Vc = L.V[i1], Vr = L.V[i2];
-    const int Cc = L.line_idx[i1];
-    auto flip = [&](int idx) -> float {
-        if (DIR == 0) return ((A.bc_w == kWall && idx < 2) || (A.bc_e == kWall && idx > A.nx + 1)) ? -1.0f : 1.0f;
-        return ((A.bc_s == kWall && idx < 2) || (A.bc_n == kWall && idx > A.ny + 1)) ? -1.0f : 1.0f;
+__host__ __device__ inline void fold_cell(const FoldArgs& F, int R, int C) {
+    const int nx = F.nx, ny = F.ny;
+    const size_t o = (size_t)R * F.pitch + C;
+    float se = F.e[o], su = F.u[o], sv = F.v[o];
+    auto take = [&](int gr, int gc, float fu, float fv) {
+        const size_t q = (size_t)gr * F.pitch + gc;
+        se += F.e[q]; su += fu * F.u[q]; sv += fv * F.v[q];
+        F.e[q] = 0.0f; F.u[q] = 0.0f; F.v[q] = 0.0f;
     };
-    const float sl = flip(L.line_idx[i0]), sc = flip(Cc), sr = flip(L.line_idx[i2]);
-    Su = minmod_slope(Ul, Uc, Ur, theta);
-    Sv = minmod_slope(Vl, Vc, Vr, theta);
-    // tangential velocity in the geostrophic term
-    const float tl = (DIR == 0 ? Vl : Ul) * sl, tc = (DIR == 0 ? Vc : Uc) * sc, tr = (DIR == 0 ? Vr : Ur) * sr;
-    const float fl = tl * f, fc = tc * f, fr = tr * f;
-    const float D = (DIR == 0 ? A.dx : A.dy) / two_g;
-    const float sg = (DIR == 0) ? -1.0f : 1.0f;     // x: minus, y: plus
-    const float cb = theta * g * ((Ec - El) + sg * D * (fc + fl));
-    const float cc = 0.5f * g * ((Er - El) + sg * D * ((fr + 2.0f * fc) + fl));
-    const float cf = theta * g * ((Er - Ec) + sg * D * (fc + fr));
-    float K = minmod_raw(cb, cc, cf);
-    const int sel = minmod_sel(cb, cc, cf);
-    // positivity adjustment (x: west/east faces, y: south/north faces)
-    float Hlo, Hhi;
-    if (DIR == 0) {
-        const int R = other;
-        Hlo = 0.5f * (hi_flag(A, R, Cc) + hi_flag(A, R + 1, Cc));
-        Hhi = 0.5f * (hi_flag(A, R, Cc + 1) + hi_flag(A, R + 1, Cc + 1));
-    } else {
-        const int C = other;
-        Hlo = 0.5f * (hi_flag(A, Cc, C) + hi_flag(A, Cc, C + 1));
-        Hhi = 0.5f * (hi_flag(A, Cc + 1, C) + hi_flag(A, Cc + 1, C + 1));
+    if (R <= 3) {          // south side ghosts whose image is R
+        if (F.bc_s == kWall) take(3 - R, C, 1.0f, -1.0f);
+        else take(R + ny, C, 1.0f, 1.0f);
     }
-    const bool flip_a = (DIR == 0) ? (A.bc_w == kWall && Cc < 2) : (A.bc_s == kWall && Cc < 2);
-    const float traw = (DIR == 0 ? Vc : Uc);
-    const float t_adj = flip_a ? -traw : traw;
-    const float Dd = (DIR == 0 ? A.dx : A.dy) * f;
-    // x: t = (K + dx f v)/2g; y: t = (L - dy f u)/2g
-    const float tq = (K - sg * Dd * t_adj) / two_g;
-    const float hlo = (Ec + Hlo) - tq, hhi = (Ec + Hhi) + tq;
-    int adj = 0;
-    if (!(hlo > 0.0f)) { K = sg * Dd * t_adj + two_g * (Ec + Hlo); adj = 1; }
-    if (!(hhi > 0.0f)) { K = sg * Dd * t_adj - two_g * (Ec + Hhi); adj = 2; }
-    Tt = (K - sg * Dd * tc) / two_g;
-    if (!adjoint) return;
-    // ---- VJP ----
-    const float gK = gT / two_g;
-    // Tt's tangential term
-    const float gtc_T = gT * (-sg * Dd) / two_g;
-    float gfl = 0, gfc = 0, gfr = 0, gtadj = 0;
-    if (adj == 2) { gE[i1] += -two_g * gK; gtadj += sg * Dd * gK; }
-    else if (adj == 1) { gE[i1] += two_g * gK; gtadj += sg * Dd * gK; }
-    else if (sel == 0) {
-        const float q = theta * g * gK;
-        gE[i1] += q; gE[i0] -= q; gfc += q * sg * D; gfl += q * sg * D;
-    } else if (sel == 1) {
-        const float q = 0.5f * g * gK;
-        gE[i2] += q; gE[i0] -= q; gfr += q * sg * D; gfc += 2.0f * q * sg * D; gfl += q * sg * D;
-    } else if (sel == 2) {
-        const float q = theta * g * gK;
-        gE[i2] += q; gE[i1] -= q; gfc += q * sg * D; gfr += q * sg * D;
+    if (R >= ny) {         // north side
+        if (F.bc_n == kWall) take(2 * ny + 3 - R, C, 1.0f, -1.0f);
+        else take(R - ny, C, 1.0f, 1.0f);
     }
-    // f*t terms back to the tangential velocity of each cell
-    float* gT_ = (DIR == 0) ? gV : gU;
-    gT_[i0] += f * sl * gfl;
-    gT_[i1] += f * sc * gfc + sc * gtc_T + (flip_a ? -gtadj : gtadj);
-    gT_[i2] += f * sr * gfr;
-    // velocity slopes
-    slope_vjp(Ul, Uc, Ur, theta, gSu, gU[i0], gU[i1], gU[i2]);
-    slope_vjp(Vl, Vc, Vr, theta, gSv, gV[i0], gV[i1], gV[i2]);
+    if (C <= 3) {
+        if (F.bc_w == kWall) take(R, 3 - C, -1.0f, 1.0f);
+        else take(R, C + nx, 1.0f, 1.0f);
+    }
+    if (C >= nx) {
+        if (F.bc_e == kWall) take(R, 2 * nx + 3 - C, -1.0f, 1.0f);
+        else take(R, C - nx, 1.0f, 1.0f);
+    }
+    F.e[o] = se; F.u[o] = su; F.v[o] = sv;
 }
 
-template <int DIR>
-__host__ __device__ inline void face_adjoint(const AdjArgs& A, int line, int m) {
-    // DIR 0: x-face in row `line` between columns m and m+1
-    // DIR 1: y-face in column `line` between rows m and m+1
-    const float dt = A.dt_dev ? *A.dt_dev : A.dt;
-    LineCells L;
-    for (int k = 0; k < 4; ++k) {
-        const int idx = m - 1 + k;
-        L.line_idx[k] = idx;
-        const Cell c = (DIR == 0) ? adj_cell(A, line, idx) : adj_cell(A, idx, line);
-        L.E[k] = c.E; L.U[k] = c.U; L.V[k] = c.V;
-    }
-    float gE[4] = {0, 0, 0, 0}, gU[4] = {0, 0, 0, 0}, gV[4] = {0, 0, 0, 0};
-    const bool dry = (L.E[1] == kDryFlag || L.E[2] == kDryFlag);
-    if (!dry) {
-        // output weights of the two adjacent cells (0 for ghost cells)
-        const int n_line = (DIR == 0) ? A.nx : A.ny;
-        float wm[3] = {0, 0, 0}, wp[3] = {0, 0, 0};
-        auto wload = [&](int idx, float* w) {
-            if (idx < 2 || idx > n_line + 1) return;
-            const int R = (DIR == 0) ? line : idx, C = (DIR == 0) ? idx : line;
-            if (dry_clamped(A, A.w_clamp, R, C)) return;
-            const size_t o = (size_t)R * A.pitch + C;
-            w[0] = A.w_eta[o]; w[1] = A.w_hu[o]; w[2] = A.w_hv[o];
-        };
-        wload(m, wm);
-        wload(m + 1, wp);
-        const float D = (DIR == 0) ? A.dx : A.dy;
-        const float gF0 = dt * (wp[0] - wm[0]) / D;
-        const float gF1 = dt * (wp[1] - wm[1]) / D;
-        const float gF2 = dt * (wp[2] - wm[2]) / D;
-        // primal slopes and depth terms at m (k=1) and p (k=2)
-        float Sum, Svm, Tm, Sup, Svp, Tp;
-        slope_term<DIR>(A, L, 1, line, Sum, Svm, Tm, false, 0, 0, 0, gE, gU, gV);
-        slope_term<DIR>(A, L, 2, line, Sup, Svp, Tp, false, 0, 0, 0, gE, gU, gV);
-        float Hf;
-        if (DIR == 0) Hf = 0.5f * (hi_flag(A, line, m + 1) + hi_flag(A, line + 1, m + 1));
-        else Hf = 0.5f * (hi_flag(A, m + 1, line) + hi_flag(A, m + 1, line + 1));
-        const float Rpx = L.U[2] - 0.5f * Sup, Rpy = L.V[2] - 0.5f * Svp;
-        const float Rmx = L.U[1] + 0.5f * Sum, Rmy = L.V[1] + 0.5f * Svm;
-        const float hp_raw = (L.E[2] + Hf) - Tp;
-        const float hm_raw = (L.E[1] + Hf) + Tm;
-        const float hp = fmaxf(0.0f, hp_raw), hm = fmaxf(0.0f, hm_raw);
-        float ghm, gn_m, gt_m, ghp, gn_p, gt_p;
-        if (DIR == 0) {
-            flux_vjp(hm, Rmx, Rmy, hp, Rpx, Rpy, A.g, A.cutoff, A.flux_eps, gF0, gF1, gF2,
-                     ghm, gn_m, gt_m, ghp, gn_p, gt_p);
-        } else {
-            // flux(hm, Rmy, Rmx, hp, Rpy, Rpx) = (a, b, c); G = (a, c, b)
-            flux_vjp(hm, Rmy, Rmx, hp, Rpy, Rpx, A.g, A.cutoff, A.flux_eps, gF0, gF2, gF1,
-                     ghm, gn_m, gt_m, ghp, gn_p, gt_p);
-        }
-        // map back to (Rx, Ry) adjoints
-        float gRmx, gRmy, gRpx, gRpy;
-        if (DIR == 0) { gRmx = gn_m; gRmy = gt_m; gRpx = gn_p; gRpy = gt_p; }
-        else { gRmy = gn_m; gRmx = gt_m; gRpy = gn_p; gRpx = gt_p; }
-        float gTp = 0, gTm = 0;
-        if (hp_raw > 0.0f) { gE[2] += ghp; gTp = -ghp; }
-        if (hm_raw > 0.0f) { gE[1] += ghm; gTm = ghm; }
-        gU[2] += gRpx; gV[2] += gRpy;
-        gU[1] += gRmx; gV[1] += gRmy;
-        const float gSup = -0.5f * gRpx, gSvp = -0.5f * gRpy;
-        const float gSum = 0.5f * gRmx, gSvm = 0.5f * gRmy;
-        float d0, d1, d2;
-        slope_term<DIR>(A, L, 1, line, d0, d1, d2, true, gTm, gSum, gSvm, gE, gU, gV);
-        slope_term<DIR>(A, L, 2, line, d0, d1, d2, true, gTp, gSup, gSvp, gE, gU, gV);
-    }
-    float* out = (DIR == 0) ? A.xc : A.yc;
-    const size_t o = (DIR == 0) ? (size_t)line * A.pitch + m : (size_t)m * A.pitch + line;
-    for (int k = 0; k < 4; ++k) {
-        out[(size_t)(3 * k + 0) * A.plane + o] = gE[k];
-        out[(size_t)(3 * k + 1) * A.plane + o] = gU[k];
-        out[(size_t)(3 * k + 2) * A.plane + o] = gV[k];
+// blockIdx.y 0..3: rows 2, 3, ny, ny+1 (all interior columns); 4..7: columns
+// 2, 3, nx, nx+1 over rows 4..ny-1; 8: corners
+__host__ __device__ inline void fold_item(const FoldArgs& F, int y, int x) {
+    const int nx = F.nx, ny = F.ny;
+    if (y < 4) {
+        const int R = y < 2 ? 2 + y : ny + (y - 2);
+        if ((y >= 2 && R <= 3) || R > ny + 1) return;
+        const int C = 2 + x;
+        if (C > nx + 1) return;
+        fold_cell(F, R, C);
+    } else if (y < 8) {
+        const int k = y - 4;
+        const int C = k < 2 ? 2 + k : nx + (k - 2);
+        if ((k >= 2 && C <= 3) || C > nx + 1) return;
+        const int R = 4 + x;
+        if (R > ny - 1) return;
+        fold_cell(F, R, C);
+    } else if (x < 16) {
+        const int R = (x & 3) < 2 ? (x & 3) : ny + (x & 3);
+        const int C = (x >> 2) < 2 ? (x >> 2) : nx + (x >> 2);
+        const size_t q = (size_t)R * F.pitch + C;
+        F.e[q] = 0.0f; F.u[q] = 0.0f; F.v[q] = 0.0f;
     }
 }
 
-__global__ __launch_bounds__(256) void k_adj_xface(AdjArgs A) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x + 1;     // 1..nx+1
-    const int r = blockIdx.y + 2;                                  // 2..ny+1
-    if (m > A.nx + 1) return;
-    face_adjoint<0>(A, r, m);
+__global__ __launch_bounds__(256) void k_adj_fold(FoldArgs F) {
+    fold_item(F, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-__global__ __launch_bounds__(256) void k_adj_yface(AdjArgs A) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x + 2;     // 2..nx+1
-    const int m = blockIdx.y + 1;                                  // 1..ny+1
-    if (c > A.nx + 1) return;
-    face_adjoint<1>(A, c, m);
-}
-
-__host__ __device__ inline void cell_adjoint(const AdjArgs& A, int R, int C) {
-    const int nx = A.nx, ny = A.ny;
-    const size_t o = (size_t)R * A.pitch + C;
-    const bool rint = R >= 2 && R <= ny + 1, cint = C >= 2 && C <= nx + 1;
-    float gE = 0, gU = 0, gV = 0;
-    if (rint) {   // x faces in this row with left column C-2..C+1
-        for (int k = 0; k < 4; ++k) {
-            const int cm = C + 1 - k;          // this cell is cell k of the face's window
-            if (cm < 1 || cm > nx + 1) continue;
-            const size_t q = (size_t)R * A.pitch + cm;
-            gE += A.xc[(size_t)(3 * k + 0) * A.plane + q];
-            gU += A.xc[(size_t)(3 * k + 1) * A.plane + q];
-            gV += A.xc[(size_t)(3 * k + 2) * A.plane + q];
-        }
-    }
-    if (cint) {   // y faces in this column with lower row R-2..R+1
-        for (int k = 0; k < 4; ++k) {
-            const int rm = R + 1 - k;
-            if (rm < 1 || rm > ny + 1) continue;
-            const size_t q = (size_t)rm * A.pitch + C;
-            gE += A.yc[(size_t)(3 * k + 0) * A.plane + q];
-            gU += A.yc[(size_t)(3 * k + 1) * A.plane + q];
-            gV += A.yc[(size_t)(3 * k + 2) * A.plane + q];
-        }
-    }
-    if (!(rint || cint)) {
-        A.o_eta[o] = 0.0f; A.o_hu[o] = 0.0f; A.o_hv[o] = 0.0f;
-        return;
-    }
-    float h, Hm;
-    int br;
-    const Cell c = adj_cell(A, R, C, &h, &br, &Hm);
-    float xe = 0, xu = 0, xv = 0;
-    if (rint && cint) {
-        // base (E, u*h_reg, v*h_reg) and source terms of this output cell
-        const float dt = A.dt_dev ? *A.dt_dev : A.dt;
-        float w0 = A.w_eta[o], w1 = A.w_hu[o], w2 = A.w_hv[o];
-        if (dry_clamped(A, A.w_clamp, R, C)) w0 = w1 = w2 = 0.0f;
-        const float a = hi_flag(A, R, C), b = hi_flag(A, R + 1, C);
-        const float cc = hi_flag(A, R, C + 1), d = hi_flag(A, R + 1, C + 1);
-        const float Hreg = 0.25f * (((a + b) + cc) + d);
-        const float hreg = c.E + Hreg;
-        gE += w0;
-        float ghuc = w1, ghvc = w2;
-        if (hreg >= A.cutoff && c.E != kDryFlag) {
-            const float H_x = 0.5f * (cc + d) - 0.5f * (a + b);
-            const float H_y = 0.5f * (b + d) - 0.5f * (a + cc);
-            ghvc += dt * w1 * A.f;
-            gE += dt * w1 * A.g * H_x / A.dx;
-            ghuc += -dt * w2 * A.f;
-            gE += dt * w2 * A.g * H_y / A.dy;
-        }
-        if (hreg > A.cutoff) {
-            gU += ghuc * hreg; gE += ghuc * c.U;
-            gV += ghvc * hreg; gE += ghvc * c.V;
-        }
-        if (A.x_eta && !dry_clamped(A, A.x_clamp, R, C)) { xe = A.x_eta[o]; xu = A.x_hu[o]; xv = A.x_hv[o]; }
-    }
-    // conversion VJP
-    float g_eta = 0, g_hu = 0, g_hv = 0;
-    if (br == 3) {
-        g_eta = gE - (gU * c.U + gV * c.V) / h;
-        g_hu = gU / h;
-        g_hv = gV / h;
-    } else if (br == 2) {
-        const float eps = A.desing;
-        const float aq = h * h / (2.0f * eps) + 0.5f * eps;
-        const float mn = fminf(aq, eps);
-        const float den = fmaxf(mn, fabsf(h));
-        float dden = 0.0f;
-        if (fabsf(h) >= mn) dden = (h >= 0.0f) ? 1.0f : -1.0f;
-        else if (aq <= eps) dden = h / eps;
-        g_eta = gE - (gU * c.U + gV * c.V) * dden / den;
-        g_hu = gU / den;
-        g_hv = gV / den;
-    }
-    A.o_eta[o] = A.alpha * g_eta + A.gamma * xe;
-    A.o_hu[o] = A.alpha * g_hu + A.gamma * xu;
-    A.o_hv[o] = A.alpha * g_hv + A.gamma * xv;
-}
-
-__global__ __launch_bounds__(256) void k_adj_cell(AdjArgs A) {
-    const int C = blockIdx.x * blockDim.x + threadIdx.x;
-    const int R = blockIdx.y;
-    if (C > A.nx + 3 || R > A.ny + 3) return;
-    cell_adjoint(A, R, C);
-}
-
-// ghost adjoints back to the interior cells they are images of
-__host__ __device__ inline void fold_ns(const AdjArgs& A, int C) {
-    const int ny = A.ny;
-    auto add = [&](int Rg, int Ri, float sv) {
-        const size_t g = (size_t)Rg * A.pitch + C, i = (size_t)Ri * A.pitch + C;
-        A.o_eta[i] += A.o_eta[g]; A.o_hu[i] += A.o_hu[g]; A.o_hv[i] += sv * A.o_hv[g];
-        A.o_eta[g] = 0.0f; A.o_hu[g] = 0.0f; A.o_hv[g] = 0.0f;
-    };
-    if (A.bc_s == kWall) { add(1, 2, -1.0f); add(0, 3, -1.0f); }
-    else if (A.bc_s == kPeriodic) { add(0, ny, 1.0f); add(1, ny + 1, 1.0f); }
-    if (A.bc_n == kWall) { add(ny + 2, ny + 1, -1.0f); add(ny + 3, ny, -1.0f); }
-    else if (A.bc_n == kPeriodic) { add(ny + 2, 2, 1.0f); add(ny + 3, 3, 1.0f); }
-}
-
-__host__ __device__ inline void fold_ew(const AdjArgs& A, int R) {
-    const int nx = A.nx;
-    auto add = [&](int Cg, int Ci, float su) {
-        const size_t g = (size_t)R * A.pitch + Cg, i = (size_t)R * A.pitch + Ci;
-        A.o_eta[i] += A.o_eta[g]; A.o_hu[i] += su * A.o_hu[g]; A.o_hv[i] += A.o_hv[g];
-        A.o_eta[g] = 0.0f; A.o_hu[g] = 0.0f; A.o_hv[g] = 0.0f;
-    };
-    if (A.bc_w == kWall) { add(1, 2, -1.0f); add(0, 3, -1.0f); }
-    else if (A.bc_w == kPeriodic) { add(0, nx, 1.0f); add(1, nx + 1, 1.0f); }
-    if (A.bc_e == kWall) { add(nx + 2, nx + 1, -1.0f); add(nx + 3, nx, -1.0f); }
-    else if (A.bc_e == kPeriodic) { add(nx + 2, 2, 1.0f); add(nx + 3, 3, 1.0f); }
-}
-
-__global__ void k_adj_fold_ns(AdjArgs A) {
-    const int C = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    if (C <= A.nx + 1) fold_ns(A, C);
-}
-
-__global__ void k_adj_fold_ew(AdjArgs A) {
-    const int R = blockIdx.x * blockDim.x + threadIdx.x + 2;
-    if (R <= A.ny + 1) fold_ew(A, R);
-}
-
-static int adjoint_substep(const AdjArgs& A, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_xface, dim3((A.nx + 1 + 255) / 256, A.ny), dim3(256), 0, s, A);
-    hipLaunchKernelGGL(k_adj_yface, dim3((A.nx + 255) / 256, A.ny + 1), dim3(256), 0, s, A);
-    hipLaunchKernelGGL(k_adj_cell, dim3((A.nx + 4 + 255) / 256, A.ny + 4), dim3(256), 0, s, A);
-    hipLaunchKernelGGL(k_adj_fold_ns, dim3((A.nx + 255) / 256), dim3(256), 0, s, A);
-    hipLaunchKernelGGL(k_adj_fold_ew, dim3((A.ny + 255) / 256), dim3(256), 0, s, A);
-    return check_launch("adjoint substep launch");
-}
-
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
 static int adj_validate(const gocean_domain_t* d) {
     if (int e = validate_domain(d)) return e;
     if (!(d->north_x == 0.0f && d->north_y == 1.0f))
@@ -502,14 +649,74 @@ static int adj_validate(const gocean_domain_t* d) {
     return 0;
 }
 
-static void adj_fill(const gocean_domain_t* d, AdjArgs& A) {
+static void adj_fill(const gocean_domain_t* d, AdjTileArgs& A) {
     std::memset(&A, 0, sizeof(A));
     A.nx = d->nx; A.ny = d->ny; A.pitch = d->pitch; A.hi_pitch = d->hi_pitch;
-    A.plane = (long long)(d->ny + 4) * d->pitch;
     A.dx = d->dx; A.dy = d->dy; A.g = d->g; A.theta = d->theta; A.desing = d->desing_eps;
     A.flux_eps = d->flux_slope_eps; A.cutoff = d->depth_cutoff; A.land = d->land_value; A.f = d->f;
     A.bc_n = d->bc_north; A.bc_e = d->bc_east; A.bc_s = d->bc_south; A.bc_w = d->bc_west;
+    A.tiles_x = (d->nx + kAdjBW - 1) / kAdjBW;
+    A.tiles_y = (d->ny + kAdjBH - 1) / kAdjBH;
 }
+
+struct Tape {
+    float* u; float* v; float* t; float* s; uint32_t* code;
+};
+
+static Tape tape_at(float* base, long long plane) {
+    Tape T;
+    T.u = base; T.v = base + plane; T.t = base + 2 * plane; T.s = base + 3 * plane;
+    T.code = reinterpret_cast<uint32_t*>(base + 4 * plane);
+    return T;
+}
+
+// forward substep that also writes the branch record
+static int fwd_tape(const gocean_domain_t* d, float dt, const float* dt_dev, int rk_step,
+                    const float* e_in, const float* u_in, const float* v_in,
+                    const float* e_qn, const float* u_qn, const float* v_qn,
+                    float* e_out, float* u_out, float* v_out, const float* Hi, const Tape& T,
+                    hipStream_t s) {
+    SubstepArgs A;
+    fill_args(d, A);
+    A.dt = dt; A.dt_dev = dt_dev; A.rk_step = rk_step;
+    A.eta_in = e_in; A.hu_in = u_in; A.hv_in = v_in;
+    A.eta_qn = e_qn ? e_qn : e_out; A.hu_qn = u_qn ? u_qn : u_out; A.hv_qn = v_qn ? v_qn : v_out;
+    A.eta_out = e_out; A.hu_out = u_out; A.hv_out = v_out;
+    A.Hi = Hi;
+    A.tape_u = T.u; A.tape_v = T.v; A.tape_t = T.t; A.tape_s = T.s; A.tape_code = T.code;
+    return launch_substep_tape(A, s);
+}
+
+static int adj_launch(const AdjTileArgs& A, hipStream_t s) {
+    const int nb = A.tiles_x * A.tiles_y;
+    hipLaunchKernelGGL(k_adj_tile, dim3(nb), dim3(kAdjNT), 0, s, A);
+    FoldArgs F;
+    F.e = A.o_eta; F.u = A.o_hu; F.v = A.o_hv;
+    F.nx = A.nx; F.ny = A.ny; F.pitch = A.pitch;
+    F.bc_n = A.bc_n; F.bc_e = A.bc_e; F.bc_s = A.bc_s; F.bc_w = A.bc_w;
+    const int n = (A.nx > A.ny ? A.nx : A.ny) + 16;
+    hipLaunchKernelGGL(k_adj_fold, dim3((n + 255) / 256, 9), dim3(256), 0, s, F);
+    return check_launch("adjoint substep launch");
+}
+
+// adjoint of one substep at the state whose eta is e_in, with its record T
+static int adj_substep(const gocean_domain_t* d, float dt, const float* dt_dev, const float* e_in,
+                       const Tape& T, const float* w_e, const float* w_u, const float* w_v,
+                       const float* w_clamp, const float* x_e, const float* x_u, const float* x_v,
+                       const float* x_clamp, float alpha, float gamma, float* o_e, float* o_u, float* o_v,
+                       const float* Hi, hipStream_t s) {
+    AdjTileArgs A;
+    adj_fill(d, A);
+    A.eta = e_in;
+    A.tU = T.u; A.tV = T.v; A.tT = T.t; A.tS = T.s; A.tC = T.code;
+    A.w_eta = w_e; A.w_hu = w_u; A.w_hv = w_v; A.w_clamp = w_clamp;
+    A.x_eta = x_e; A.x_hu = x_u; A.x_hv = x_v; A.x_clamp = x_clamp;
+    A.o_eta = o_e; A.o_hu = o_u; A.o_hv = o_v;
+    A.Hi = Hi; A.dt = dt; A.dt_dev = dt_dev; A.alpha = alpha; A.gamma = gamma;
+    return adj_launch(A, s);
+}
+
+static long long plane_of(const gocean_domain_t* d) { return (long long)(d->ny + 4) * d->pitch; }
 
 }  // namespace gocean
 
@@ -519,38 +726,38 @@ extern "C" {
 
 long long gocean_adjoint_scratch_floats(const gocean_domain_t* d) {
     if (!d) return -1;
-    return 27LL * (long long)(d->ny + 4) * d->pitch;
+    return 19LL * plane_of(d);
 }
 
 int gocean_adjoint_substep(const gocean_domain_t* d, float dt, const float* dt_dev,
                            const float* eta, const float* hu, const float* hv,
-                           const float* w_eta, const float* w_hu, const float* w_hv, const float* w_clamp,
-                           const float* x_eta, const float* x_hu, const float* x_hv, const float* x_clamp,
-                           float alpha, float gamma, float* o_eta, float* o_hu, float* o_hv,
-                           const float* Hi, float* scratch, void* stream) {
+                           const float* w_eta, const float* w_hu, const float* w_hv,
+                           const float* w_clamp, const float* x_eta, const float* x_hu,
+                           const float* x_hv, const float* x_clamp, float alpha, float gamma,
+                           float* o_eta, float* o_hu, float* o_hv, const float* Hi,
+                           float* scratch, void* stream) {
     clear_error();
     if (int e = adj_validate(d)) return e;
     if (!eta || !hu || !hv || !w_eta || !w_hu || !w_hv || !o_eta || !o_hu || !o_hv || !Hi || !scratch)
         return set_error(GOCEAN_EINVAL, "NULL argument");
     if ((x_eta || x_hu || x_hv) && !(x_eta && x_hu && x_hv))
         return set_error(GOCEAN_EINVAL, "extra term needs all three fields");
-    AdjArgs A;
-    adj_fill(d, A);
-    A.eta = eta; A.hu = hu; A.hv = hv;
-    A.w_eta = w_eta; A.w_hu = w_hu; A.w_hv = w_hv;
-    A.x_eta = x_eta; A.x_hu = x_hu; A.x_hv = x_hv;
-    A.w_clamp = w_clamp; A.x_clamp = x_clamp;
-    A.o_eta = o_eta; A.o_hu = o_hu; A.o_hv = o_hv;
-    A.Hi = Hi; A.xc = scratch; A.yc = scratch + 12 * A.plane;
-    A.dt = dt; A.dt_dev = dt_dev; A.alpha = alpha; A.gamma = gamma;
-    return adjoint_substep(A, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    const long long pl = plane_of(d);
+    float* fo = scratch;                           // forward output P(Q): 3 planes
+    const Tape T = tape_at(scratch + 3 * pl, pl);   // record: 5 planes
+    if (int e = fwd_tape(d, dt, dt_dev, 0, eta, hu, hv, nullptr, nullptr, nullptr, fo, fo + pl,
+                         fo + 2 * pl, Hi, T, s))
+        return e;
+    return adj_substep(d, dt, dt_dev, eta, T, w_eta, w_hu, w_hv, w_clamp ? w_clamp : fo, x_eta, x_hu,
+                       x_hv, x_clamp, alpha, gamma, o_eta, o_hu, o_hv, Hi, s);
 }
 
 int gocean_adjoint_rk2_step(const gocean_domain_t* d, float dt, const float* dt_dev,
                             const float* eta_n, const float* hu_n, const float* hv_n,
                             const float* eta_s, const float* hu_s, const float* hv_s,
-                            const float* eta_next, const float* l_eta, const float* l_hu, const float* l_hv,
-                            float* o_eta, float* o_hu, float* o_hv,
+                            const float* eta_next, const float* l_eta, const float* l_hu,
+                            const float* l_hv, float* o_eta, float* o_hu, float* o_hv,
                             const float* Hi, float* scratch, void* stream) {
     clear_error();
     if (int e = adj_validate(d)) return e;
@@ -559,17 +766,95 @@ int gocean_adjoint_rk2_step(const gocean_domain_t* d, float dt, const float* dt_
         return set_error(GOCEAN_EINVAL, "NULL argument");
     if (o_eta == l_eta || o_hu == l_hu || o_hv == l_hv)
         return set_error(GOCEAN_EINVAL, "output adjoint must not alias the input adjoint");
-    const long long plane = (long long)(d->ny + 4) * d->pitch;
-    float* ls = scratch + 24 * plane;          // lambda* (3 planes)
+    hipStream_t s = (hipStream_t)stream;
+    const long long pl = plane_of(d);
+    float* qs = scratch;                     // Q* recomputed (clamp mask of lambda*)
+    float* qn1 = scratch + 3 * pl;           // Q^{n+1} (clamp mask of lambda^{n+1})
+    float* ls = scratch + 6 * pl;            // lambda*
+    const Tape T0 = tape_at(scratch + 9 * pl, pl);
+    const Tape T1 = tape_at(scratch + 14 * pl, pl);
+    if (int e = fwd_tape(d, dt, dt_dev, 0, eta_n, hu_n, hv_n, nullptr, nullptr, nullptr, qs, qs + pl,
+                         qs + 2 * pl, Hi, T0, s))
+        return e;
+    if (int e = fwd_tape(d, dt, dt_dev, 1, eta_s, hu_s, hv_s, eta_n, hu_n, hv_n, qn1, qn1 + pl,
+                         qn1 + 2 * pl, Hi, T1, s))
+        return e;
+    const float* clamp_next = eta_next ? eta_next : qn1;
     // lambda* = 0.5 * J_P(Q*)^T lambda^{n+1}
-    int e = gocean_adjoint_substep(d, dt, dt_dev, eta_s, hu_s, hv_s, l_eta, l_hu, l_hv, eta_next,
-                                   nullptr, nullptr, nullptr, nullptr, 0.5f, 0.0f,
-                                   ls, ls + plane, ls + 2 * plane, Hi, scratch, stream);
-    if (e) return e;
+    if (int e = adj_substep(d, dt, dt_dev, eta_s, T1, l_eta, l_hu, l_hv, clamp_next, nullptr, nullptr,
+                            nullptr, nullptr, 0.5f, 0.0f, ls, ls + pl, ls + 2 * pl, Hi, s))
+        return e;
     // lambda^n = J_P(Q^n)^T lambda* + 0.5 * lambda^{n+1}
-    return gocean_adjoint_substep(d, dt, dt_dev, eta_n, hu_n, hv_n, ls, ls + plane, ls + 2 * plane, eta_s,
-                                  l_eta, l_hu, l_hv, eta_next, 1.0f, 0.5f, o_eta, o_hu, o_hv, Hi, scratch,
-                                  stream);
+    return adj_substep(d, dt, dt_dev, eta_n, T0, ls, ls + pl, ls + 2 * pl, qs, l_eta, l_hu, l_hv,
+                       clamp_next, 1.0f, 0.5f, o_eta, o_hu, o_hv, Hi, s);
+}
+
+long long gocean_adjoint_segment_work_floats(const gocean_domain_t* d, int n_steps) {
+    if (!d || n_steps < 1) return -1;
+    return (16LL * n_steps + 9LL) * plane_of(d);
+}
+
+int gocean_adjoint_segment(const gocean_domain_t* d, int n_steps, float dt, const float* dt_dev,
+                           const float* eta0, const float* hu0, const float* hv0,
+                           const float* l_eta, const float* l_hu, const float* l_hv,
+                           float* o_eta, float* o_hu, float* o_hv, const float* Hi,
+                           float* work, long long work_floats, void* stream) {
+    clear_error();
+    if (int e = adj_validate(d)) return e;
+    if (n_steps < 1) return set_error(GOCEAN_EINVAL, "n_steps must be >= 1");
+    if (!eta0 || !hu0 || !hv0 || !l_eta || !l_hu || !l_hv || !o_eta || !o_hu || !o_hv || !Hi || !work)
+        return set_error(GOCEAN_EINVAL, "NULL argument");
+    if (work_floats < gocean_adjoint_segment_work_floats(d, n_steps))
+        return set_error(GOCEAN_EINVAL, "work buffer too small: %lld floats, need %lld", work_floats,
+                         gocean_adjoint_segment_work_floats(d, n_steps));
+    hipStream_t s = (hipStream_t)stream;
+    const long long pl = plane_of(d);
+    const int k = n_steps;
+    // layout: Q_1..Q_k (3 planes each), Q*_0..Q*_{k-1} (3), records 2k (5), lambda x 3 (3)
+    float* Qb = work;
+    float* Qs = work + 3LL * k * pl;
+    float* Tb = work + 6LL * k * pl;
+    float* Lb = work + 16LL * k * pl;
+    auto Q = [&](int i) -> const float* { return i == 0 ? eta0 : Qb + 3LL * (i - 1) * pl; };
+    auto Qh = [&](int i, int f) -> const float* {
+        if (i == 0) return f == 0 ? eta0 : (f == 1 ? hu0 : hv0);
+        return Qb + (3LL * (i - 1) + f) * pl;
+    };
+    // recompute: Q*_i = P(Q_i), Q_{i+1} = (Q_i + P(Q*_i)) / 2, with records
+    for (int i = 0; i < k; ++i) {
+        float* qs = Qs + 3LL * i * pl;
+        float* qn = Qb + 3LL * i * pl;       // Q_{i+1}
+        if (int e = fwd_tape(d, dt, dt_dev, 0, Qh(i, 0), Qh(i, 1), Qh(i, 2), nullptr, nullptr, nullptr, qs,
+                             qs + pl, qs + 2 * pl, Hi, tape_at(Tb + 10LL * i * pl, pl), s))
+            return e;
+        if (int e = fwd_tape(d, dt, dt_dev, 1, qs, qs + pl, qs + 2 * pl, Qh(i, 0), Qh(i, 1), Qh(i, 2), qn,
+                             qn + pl, qn + 2 * pl, Hi, tape_at(Tb + (10LL * i + 5) * pl, pl), s))
+            return e;
+    }
+    // reverse
+    float* lam_s = Lb;
+    float* lam_a = Lb + 3 * pl;
+    float* lam_b = Lb + 6 * pl;
+    const float* le = l_eta; const float* lu = l_hu; const float* lv = l_hv;
+    for (int i = k - 1; i >= 0; --i) {
+        const float* qs = Qs + 3LL * i * pl;
+        const float* clamp_next = Q(i + 1);
+        float* oe; float* ou; float* ov;
+        if (i == 0) { oe = o_eta; ou = o_hu; ov = o_hv; }
+        else {
+            float* dst = (le == lam_a) ? lam_b : lam_a;
+            oe = dst; ou = dst + pl; ov = dst + 2 * pl;
+        }
+        if (int e = adj_substep(d, dt, dt_dev, qs, tape_at(Tb + (10LL * i + 5) * pl, pl), le, lu, lv,
+                                clamp_next, nullptr, nullptr, nullptr, nullptr, 0.5f, 0.0f, lam_s,
+                                lam_s + pl, lam_s + 2 * pl, Hi, s))
+            return e;
+        if (int e = adj_substep(d, dt, dt_dev, Q(i), tape_at(Tb + 10LL * i * pl, pl), lam_s, lam_s + pl,
+                                lam_s + 2 * pl, qs, le, lu, lv, clamp_next, 1.0f, 0.5f, oe, ou, ov, Hi, s))
+            return e;
+        le = oe; lu = ou; lv = ov;
+    }
+    return 0;
 }
 
 }  // extern "C"

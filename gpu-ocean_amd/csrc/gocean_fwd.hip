@@ -107,6 +107,23 @@ static void launch_substep_t(SubstepArgs A, hipStream_t s) {
     hipLaunchKernelGGL((k_substep<BW, BH, NT, MODE>), dim3(nb), dim3(NT), 0, s, A);
 }
 
+// recording variant (zero rotation only): also writes the branch record
+template <int BW, int BH, int NT>
+__global__ __launch_bounds__(NT) void k_substep_tape(SubstepArgs A) {
+    __shared__ TileSmem<BW, BH, kModeZero, true> S;
+    const int t = xcd_band(blockIdx.x, gridDim.x);
+    const int ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
+    cdklm_tile<BW, BH, NT, kModeZero, true>(A, tx, ty, S);
+}
+
+int launch_substep_tape(SubstepArgs A, hipStream_t s) {
+    constexpr int BW = 32, BH = 16, NT = 256;
+    A.tiles_x = (A.nx + BW - 1) / BW;
+    A.tiles_y = (A.ny + BH - 1) / BH;
+    hipLaunchKernelGGL((k_substep_tape<BW, BH, NT>), dim3(A.tiles_x * A.tiles_y), dim3(NT), 0, s, A);
+    return check_launch("recording substep launch");
+}
+
 int substep_mode(const SubstepArgs& A, bool field) {
     if (field) return kModeField;
     return (A.north_x == 0.0f && A.north_y == 1.0f) ? kModeZero : kModeConst;

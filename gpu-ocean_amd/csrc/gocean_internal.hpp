@@ -26,6 +26,8 @@ int check_launch(const char* what);
 int validate_domain(const gocean_domain_t* d);
 void fill_args(const gocean_domain_t* d, SubstepArgs& A);
 int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s);
+// zero-rotation substep that also writes the branch record (A.tape_*)
+int launch_substep_tape(SubstepArgs A, hipStream_t s);
 int periodic_bc(const gocean_domain_t* d, float* e, float* u, float* v, hipStream_t s);
 int max_dt(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
            const float* Hm, float courant, float* out, hipStream_t s);

@@ -202,18 +202,23 @@ int gocean_ensemble_replay(gocean_ensemble_t* ens, void* stream);
  * GOCEAN_EINVAL.  All state / adjoint arrays are padded (ny+4) x pitch
  * planes; ghost entries of adjoint inputs are ignored and those of adjoint
  * outputs are zero (their sensitivity is folded into the interior cells the
- * ghosts mirror or wrap).  Branches (minmod candidate, wave-speed extrema,
+ * ghosts mirror or wrap).  Branches (minmod candidate, wave-speed min/max,
  * upwinding, positivity adjustment, wet / desingularised / dry cells) follow
- * the forward evaluation.
- *
- * Scratch: gocean_adjoint_scratch_floats(d) floats of device memory, owned
- * by the caller. */
+ * the forward evaluation: each entry point first re-runs the forward
+ * substeps it differentiates in a recording variant that stores every such
+ * choice, then applies the hand-written Jacobian-transpose kernels.
+ * Scratch / work buffers are caller-owned device memory. */
+
+/* Scratch of gocean_adjoint_substep / gocean_adjoint_rk2_step, in floats. */
 long long gocean_adjoint_scratch_floats(const gocean_domain_t* d);
 
 /* o = alpha * J^T w + gamma * x, with J the Jacobian of the substep map
  * Q -> base(Q) + dt * L(Q) (the rk_step 0 update, CDKLM16_kernel.cu:905-1001)
- * at Q = (eta, hu, hv).  w_clamp / x_clamp (nullable): forward eta outputs;
- * cells they show clamped to the dry state pass no w / x.  x may be NULL. */
+ * at Q = (eta, hu, hv).  Cells that the forward update clamps to the dry state
+ * (eta = -Hm + cutoff, CDKLM16_kernel.cu:986-1001) pass no w: the mask is
+ * taken from w_clamp (a forward eta output) or, when w_clamp is NULL, from
+ * this substep's own output.  x_clamp (nullable) masks x the same way; x may
+ * be NULL. */
 int gocean_adjoint_substep(const gocean_domain_t* d, float dt, const float* dt_dev,
                            const float* eta, const float* hu, const float* hv,
                            const float* w_eta, const float* w_hu, const float* w_hv,
@@ -224,14 +229,28 @@ int gocean_adjoint_substep(const gocean_domain_t* d, float dt, const float* dt_d
 
 /* lambda^n = (dQ^{n+1}/dQ^n)^T lambda^{n+1} for one SSP-RK2 step
  * Q* = P(Q^n), Q^{n+1} = (Q^n + P(Q*)) / 2 (CDKLM16.py:533-544).  Needs the
- * step's Q^n and Q* (and, nullable, eta of Q^{n+1} for the dry-clamp mask);
- * o must not alias l. */
+ * step's Q^n and Q* (and, nullable, eta of Q^{n+1} for the dry-clamp mask;
+ * NULL recomputes it); o must not alias l. */
 int gocean_adjoint_rk2_step(const gocean_domain_t* d, float dt, const float* dt_dev,
                             const float* eta_n, const float* hu_n, const float* hv_n,
                             const float* eta_s, const float* hu_s, const float* hv_s,
                             const float* eta_next, const float* l_eta, const float* l_hu,
                             const float* l_hv, float* o_eta, float* o_hu, float* o_hv,
                             const float* Hi, float* scratch, void* stream);
+
+/* Work buffer of gocean_adjoint_segment for n_steps steps, in floats
+ * ((16 n + 9) planes). */
+long long gocean_adjoint_segment_work_floats(const gocean_domain_t* d, int n_steps);
+
+/* Reverse mode of n_steps fixed-dt RK2 steps started from the checkpoint
+ * (eta0, hu0, hv0): recomputes the segment (with branch records) into `work`,
+ * then maps lambda at the segment's end (l_*) to lambda at its start (o_*).
+ * One call per checkpoint segment; everything runs on `stream`. */
+int gocean_adjoint_segment(const gocean_domain_t* d, int n_steps, float dt, const float* dt_dev,
+                           const float* eta0, const float* hu0, const float* hv0,
+                           const float* l_eta, const float* l_hu, const float* l_hv,
+                           float* o_eta, float* o_hu, float* o_hv, const float* Hi,
+                           float* work, long long work_floats, void* stream);
 
 #ifdef __cplusplus
 }
