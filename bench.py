@@ -201,6 +201,117 @@ def bench_adjoint(ctx, n=512, steps=500, K=50, n_obs=64, seed=44):
             "J": float(J), "grad_norms_eta_hu_hv": [round(v, 6) for v in g]}
 
 
+def bench_c5(ctx, steps=200, K=25, n_obs=16, seed=55, plain_members=4):
+    """C5: 32 domains with sides from {256, 384, 512, 768}, walls, own random
+    depth; per-member fixed dt (CFL 0.7); forward + adjoint of 200 RK2 steps
+    with a checkpoint every 25, in fused launches over all members; a
+    16-point eta misfit per domain (observations: twin run from a perturbed
+    eta plus N(0, 0.01 m) noise) and its per-domain gradient.  Also times the
+    single-domain reverse mode member by member on the first
+    ``plain_members`` members, for the batched-vs-plain ratio."""
+    import torch
+    from SWESimulators import BathymetryAndICs as B, CDKLM16Ensemble, adjoint as A
+    rng = np.random.default_rng(seed)
+    inps = B.config5()
+    cells = sum(i.nx * i.ny for i in inps)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, inps, courant_number=0.7)
+    ea = A.EnsembleAdjoint(ens, checkpoint_every=K)
+    state0 = ens.state[0].clone()
+    obs = []
+    for k, i in enumerate(inps):
+        ox = rng.uniform(2.0, i.nx - 2.0, n_obs) * i.dx
+        oy = rng.uniform(2.0, i.ny - 2.0, n_obs) * i.dy
+        obs.append((ox, oy))
+    # twin run
+    twin0 = state0.clone()
+    for k, i in enumerate(inps):
+        o = int(ens.state_off[k])
+        pert = 0.05 * B.gaussian_random_field((i.ny + 4, i.nx + 4), 16.0, rng).astype(np.float32)
+        twin0[o:o + ens.planes[k]] += torch.from_numpy(pert.ravel()).to(twin0.device)
+    with torch.no_grad():
+        tw = ea.forward(twin0, steps)
+        ys = []
+        for k, i in enumerate(inps):
+            o = int(ens.state_off[k])
+            e = tw[o:o + ens.planes[k]].view(i.ny + 4, ens.pitch[k])
+            ys.append(A.bilinear_eta(e, obs[k][0], obs[k][1], i.dx, i.dy)
+                      + torch.from_numpy(rng.normal(0.0, 0.01, n_obs).astype(np.float32)).to(e.device))
+
+    def misfit(out):
+        J = 0.0
+        for k, i in enumerate(inps):
+            o = int(ens.state_off[k])
+            e = out[o:o + ens.planes[k]].view(i.ny + 4, ens.pitch[k])
+            J = J + A.point_misfit(e, obs[k][0], obs[k][1], ys[k], i.dx, i.dy)
+        return J
+
+    # forward alone: batched eager steps
+    ens.state[0].copy_(state0)
+    ens.step_n(2)
+    torch.cuda.synchronize()
+    ens.state[0].copy_(state0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step_n(steps)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+
+    def fwd_adj():
+        out = ea.forward(state0, steps).requires_grad_(True)
+        J = misfit(out)
+        lam = torch.autograd.grad(J, out)[0]
+        return J, ea.backward(lam)
+
+    fwd_adj()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    J, grad = fwd_adj()
+    torch.cuda.synchronize()
+    t_tot = time.perf_counter() - t0
+    gnorm = [round(float(grad[int(ens.state_off[k]):int(ens.state_off[k]) + ens.planes[k]].norm()), 6)
+             for k in range(ens.n)]
+    # single-domain reverse mode, member by member (first plain_members members)
+    from SWESimulators import Common, _gocean as G
+    dts = ens.dt_host()
+    plain_cells = 0
+    plain_s = 0.0
+    for k in range(plain_members):
+        i = inps[k]
+        d = Common.make_domain(i.nx, i.ny, i.dx, i.dy, i.g, 0.0, 1.3, 2, i.bc, pitch=ens.pitch[k],
+                               hi_pitch=i.nx + 5, f=i.f)
+        Hi_k = ens.Hi[int(ens.hi_off[k]):int(ens.hi_off[k + 1])].view(i.ny + 4 + 1, i.nx + 5)
+        adj = A.RK2Adjoint(d, Hi_k, checkpoint_every=K)
+        o, pl = int(ens.state_off[k]), ens.planes[k]
+        q0 = tuple(state0[o + f * pl:o + (f + 1) * pl].view(i.ny + 4, ens.pitch[k]) for f in range(3))
+
+        def one():
+            qs = [t.clone().requires_grad_(True) for t in q0]
+            out = A.CDKLM16RK2.apply(*qs, adj, steps, float(dts[k]))
+            A.point_misfit(out[0], obs[k][0], obs[k][1], ys[k], i.dx, i.dy).backward()
+        one()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        one()
+        torch.cuda.synchronize()
+        plain_s += time.perf_counter() - t0
+        plain_cells += i.nx * i.ny
+    batched_rate = cells * steps / t_tot / 1e6
+    plain_rate = plain_cells * steps / plain_s / 1e6
+    ens.cleanUp()
+    return {"workload": f"32 domains, sides {{256,384,512,768}} (seeded), walls, own random depth, "
+                        f"per-member dt (CFL 0.7), {steps} RK2 steps, checkpoint every {K}, "
+                        f"{n_obs} point observations per domain",
+            "cells": cells, "sides": [i.nx for i in inps],
+            "fwd_batched_s": round(t_fwd, 4), "fwd_adj_batched_s": round(t_tot, 4),
+            "fwd_adj_over_fwd": round(t_tot / t_fwd, 2),
+            "fwd_batched_mcell_steps_s": round(cells * steps / t_fwd / 1e6, 1),
+            "fwd_adj_batched_mcell_steps_s": round(batched_rate, 1),
+            "fwd_adj_plain_per_member_mcell_steps_s": round(plain_rate, 1),
+            "fwd_adj_batched_vs_plain": round(batched_rate / plain_rate, 2),
+            "plain_members_timed": plain_members,
+            "J": float(J), "grad_norm_per_domain": gnorm}
+
+
 def measured_traffic(cells):
     """HBM bytes per substep launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy), when
@@ -267,11 +378,19 @@ def main():
         log("[bench] batched ensemble C3")
         extra["ensemble"] = bench_ensemble(ctx)
         log(f"[bench] ensemble: {extra['ensemble']}")
+        # many small members: one-member-per-launch is launch / host bound
+        # there, which is where batching changes the picture
+        log("[bench] batched ensemble, 1024 members of 64x64")
+        extra["ensemble_small_members"] = bench_ensemble(ctx, members=1024, n=64, steps=50, per_member_steps=2)
+        log(f"[bench] ensemble small: {extra['ensemble_small_members']}")
 
     if not a.no_adjoint:
         log("[bench] adjoint C4")
         extra["adjoint"] = bench_adjoint(ctx)
         log(f"[bench] adjoint: {extra['adjoint']}")
+        log("[bench] batched uneven ensemble with adjoint C5")
+        extra["ensemble_adjoint"] = bench_c5(ctx)
+        log(f"[bench] ensemble adjoint: {extra['ensemble_adjoint']}")
 
     cpu = None
     if not a.no_cpu:

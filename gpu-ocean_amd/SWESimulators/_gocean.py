@@ -99,16 +99,20 @@ PROTOTYPES.extend([
     ("gocean_ensemble_replay", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("gocean_adjoint_scratch_floats", ctypes.c_longlong, [ctypes.POINTER(Domain)]),
     ("gocean_adjoint_substep", c_int,
-     [ctypes.POINTER(Domain), c_float, c_void_p, c_void_p, c_void_p, c_void_p,
+     [ctypes.POINTER(Domain), ctypes.POINTER(Wind), c_float, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
       c_float, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("gocean_adjoint_rk2_step", c_int,
-     [ctypes.POINTER(Domain), c_float, c_void_p, c_void_p, c_void_p, c_void_p,
+     [ctypes.POINTER(Domain), ctypes.POINTER(Wind), c_float, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gocean_ensemble_adjoint_setup", ctypes.c_longlong, [c_void_p, c_int]),
+    ("gocean_ensemble_adjoint_segment", c_int,
+     [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      ctypes.c_longlong, c_void_p]),
     ("gocean_adjoint_segment_work_floats", ctypes.c_longlong, [ctypes.POINTER(Domain), c_int]),
     ("gocean_adjoint_segment", c_int,
-     [ctypes.POINTER(Domain), c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p,
+     [ctypes.POINTER(Domain), ctypes.POINTER(Wind), c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, ctypes.c_longlong, c_void_p]),
 ])

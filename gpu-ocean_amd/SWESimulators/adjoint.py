@@ -159,7 +159,7 @@ class RK2Adjoint(object):
             for s in range(n_seg - 1, -1, -1):
                 k = min(self.K, self.n_steps - s * self.K)
                 G.check(self.lib.gocean_adjoint_segment(
-                    self.d, k, self.dt, None, *(G.ptr(a) for a in self.checkpoints[s]),
+                    self.d, self.wind, k, self.dt, None, *(G.ptr(a) for a in self.checkpoints[s]),
                     *(G.ptr(a) for a in cur), *(G.ptr(a) for a in nxt), G.ptr(self.Hi),
                     G.ptr(self.work), self.work.numel(), G.stream_handle(self.stream)),
                     "gocean_adjoint_segment")
@@ -213,6 +213,104 @@ def simulator_adjoint(sim, checkpoint_every=50):
     return RK2Adjoint(sim.domain, sim.bathymetry.Bi.tensor, stream=sim.gpu_stream,
                       wind=sim._wind(), checkpoint_every=checkpoint_every,
                       tile_config=sim.tile_config)
+
+
+def _ghost_index(nx, ny, pitch, bc_north, bc_east):
+    """Flat indices within one padded plane: output ghost cells the forward
+    rewrites (periodic images) with the interior cell each copies, and the
+    ghost cells it leaves untouched (see the module docstring)."""
+    pNS = bc_north == G_BC_PERIODIC
+    pEW = bc_east == G_BC_PERIODIC
+    R = np.arange(ny + 4, dtype=np.int64)[:, None]
+    C = np.arange(nx + 4, dtype=np.int64)[None, :]
+    gr = (R < 2) | (R > ny + 1)
+    gc = (C < 2) | (C > nx + 1)
+    ghost = gr | gc
+    written = ghost & (~gr | pNS) & (~gc | pEW)
+    Ri = np.where(R < 2, R + ny, np.where(R > ny + 1, R - ny, R))
+    Ci = np.where(C < 2, C + nx, np.where(C > nx + 1, C - nx, C))
+    o = R * pitch + C
+    return o[written], (Ri * pitch + Ci)[written], o[ghost & ~written]
+
+
+class EnsembleAdjoint(object):
+    """Reverse mode of a ``CDKLM16Ensemble`` (rk_order 2, constant f, no
+    rotation, r == 0): the forward keeps a copy of the flat ensemble state
+    every ``checkpoint_every`` steps; the backward issues one
+    ``gocean_ensemble_adjoint_segment`` per checkpoint segment, each a few
+    launches that cover every member.  Adjoints are flat tensors laid out
+    like the ensemble state, so member k's gradient is ``ens.views``-shaped
+    at ``ens.state_off[k]``."""
+
+    def __init__(self, ens, checkpoint_every=25):
+        torch = _torch()
+        self.ens = ens
+        self.lib = G.load()
+        self.K = max(1, int(checkpoint_every))
+        nw = int(self.lib.gocean_ensemble_adjoint_setup(ens.handle, self.K))
+        if nw < 0:
+            G.check(int(nw), "gocean_ensemble_adjoint_setup")
+        self.stream = ens.gpu_stream
+        dev = ens.state[0].device
+        with torch.cuda.stream(self.stream):
+            self.work = torch.empty(nw, dtype=torch.float32, device=dev)
+        src, dst, keep = [], [], []
+        for k in range(ens.n):
+            a, b, c = _ghost_index(ens.nx[k], ens.ny[k], ens.pitch[k], ens.bcs[k][0], ens.bcs[k][1])
+            for f in range(3):
+                off = int(ens.state_off[k]) + f * ens.planes[k]
+                src.append(a + off); dst.append(b + off); keep.append(c + off)
+        mk = lambda parts: torch.as_tensor(np.concatenate(parts), device=dev)
+        self.wrap_src, self.wrap_dst, self.keep = mk(src), mk(dst), mk(keep)
+        self.checkpoints = None
+
+    def forward(self, state0, n_steps):
+        """Run n_steps RK2 steps of every member (per-member device dt) from
+        the flat state ``state0``; returns the final flat state."""
+        torch = _torch()
+        ens = self.ens
+        self.n_steps = int(n_steps)
+        cur = torch.cuda.current_stream(state0.device)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            ens.state[0].copy_(state0.detach())
+            self.checkpoints = []
+            done = 0
+            while done < self.n_steps:
+                self.checkpoints.append(ens.state[0].clone())
+                k = min(self.K, self.n_steps - done)
+                ens.step_n(k)
+                done += k
+            out = ens.state[0].clone()
+        cur.wait_stream(self.stream)
+        out.record_stream(cur)
+        return out
+
+    def backward(self, lam):
+        """Flat d(loss)/d(final state) -> flat d(loss)/d(state0)."""
+        torch = _torch()
+        if self.checkpoints is None:
+            raise RuntimeError("backward() needs a preceding forward()")
+        ens = self.ens
+        cur_stream = torch.cuda.current_stream(lam.device)
+        self.stream.wait_stream(cur_stream)
+        with torch.cuda.stream(self.stream):
+            a = lam.detach().to(torch.float32).contiguous().clone()
+            kept = a[self.keep].clone()
+            a.index_add_(0, self.wrap_dst, a[self.wrap_src])
+            b = torch.empty_like(a)
+            n_seg = len(self.checkpoints)
+            for s in range(n_seg - 1, -1, -1):
+                k = min(self.K, self.n_steps - s * self.K)
+                G.check(self.lib.gocean_ensemble_adjoint_segment(
+                    ens.handle, k, G.ptr(ens.dt), G.ptr(self.checkpoints[s]), G.ptr(a), G.ptr(b),
+                    G.ptr(ens.Hi), G.ptr(self.work), self.work.numel(), G.stream_handle(self.stream)),
+                    "gocean_ensemble_adjoint_segment")
+                a, b = b, a
+            a[self.keep] += kept
+        cur_stream.wait_stream(self.stream)
+        a.record_stream(cur_stream)
+        return a
 
 
 def bilinear_eta(eta, obs_x, obs_y, dx, dy):

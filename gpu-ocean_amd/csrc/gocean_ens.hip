@@ -9,10 +9,12 @@
 // optional captured HIP graph of n fixed-dt steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
 #include "../../include/gocean.h"
+#include "adjoint_tile.hpp"
 #include "cdklm16_tile.hpp"
 #include "gocean_internal.hpp"
 
@@ -27,6 +29,11 @@ struct EnsMember {
     float f, north_x, north_y;
     float wind_t, wx0, wx1, wy0, wy1;
     int bc_n, bc_e, bc_s, bc_w;
+    // reverse mode (set by gocean_ensemble_adjoint_setup): the member's block
+    // of the work buffer and its tiles in the recording-forward (32x16) and
+    // adjoint (kAdjBW x kAdjBH) launches
+    long long work_off;
+    int ftiles_x, ftile_begin, atiles_x, atile_begin;
 };
 
 struct EnsLaunch {
@@ -185,6 +192,122 @@ __global__ void k_ens_periodic(const EnsMember* members, float* state) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// reverse mode of the batched ensemble: the same per-tile device code as the
+// single-domain adjoint, with each workgroup's member from a tile map.
+// A state (3 planes), record (5 planes) or adjoint lives either in a flat
+// buffer laid out like the ensemble state (member m at state_off) or in the
+// member's block of the work buffer (plane index idx from work_off).
+// ---------------------------------------------------------------------------
+struct EnsRef {
+    const float* base;
+    int in_work;
+    int idx;
+};
+
+__device__ __forceinline__ long long ens_off(const EnsMember& M, const EnsRef& r) {
+    return r.in_work ? M.work_off + (long long)r.idx * M.plane : M.state_off;
+}
+
+constexpr int kEnsTapeBW = 32, kEnsTapeBH = 16, kEnsTapeNT = 256;
+
+struct EnsTapeLaunch {
+    const EnsMember* members;
+    const int* tile_member;
+    EnsRef in, qn, out, tape;
+    const float* Hi;
+    const float* dt;
+    int rk_step;
+};
+
+__global__ __launch_bounds__(kEnsTapeNT) void k_ens_substep_tape(EnsTapeLaunch L) {
+    __shared__ TileSmem<kEnsTapeBW, kEnsTapeBH, kModeZero, true> S;
+    const int t = xcd_band(blockIdx.x, gridDim.x);
+    const int m = L.tile_member[t];
+    const EnsMember& M = L.members[m];
+    SubstepArgs A;
+    const long long pl = M.plane;
+    const float* in = L.in.base + ens_off(M, L.in);
+    const float* qn = L.qn.base + ens_off(M, L.qn);
+    float* out = const_cast<float*>(L.out.base) + ens_off(M, L.out);
+    float* tp = const_cast<float*>(L.tape.base) + ens_off(M, L.tape);
+    A.eta_in = in; A.hu_in = in + pl; A.hv_in = in + 2 * pl;
+    A.eta_qn = qn; A.hu_qn = qn + pl; A.hv_qn = qn + 2 * pl;
+    A.eta_out = out; A.hu_out = out + pl; A.hv_out = out + 2 * pl;
+    A.tape_u = tp; A.tape_v = tp + pl; A.tape_t = tp + 2 * pl; A.tape_s = tp + 3 * pl;
+    A.tape_code = reinterpret_cast<uint32_t*>(tp + 4 * pl);
+    A.Hi = L.Hi + M.hi_off;
+    A.f_field = nullptr; A.nx_field = nullptr; A.ny_field = nullptr;
+    A.dt_dev = L.dt + m;
+    A.nx = M.nx; A.ny = M.ny; A.pitch = M.pitch; A.hi_pitch = M.hi_pitch;
+    A.dx = M.dx; A.dy = M.dy; A.g = M.g; A.r = M.r; A.theta = M.theta;
+    A.desing_eps = M.desing; A.flux_eps = M.flux_eps; A.cutoff = M.cutoff; A.land = M.land;
+    A.f = M.f; A.north_x = M.north_x; A.north_y = M.north_y;
+    A.wind_t = M.wind_t; A.wx0 = M.wx0; A.wx1 = M.wx1; A.wy0 = M.wy0; A.wy1 = M.wy1;
+    A.dt = 0.0f;
+    A.bc_n = M.bc_n; A.bc_e = M.bc_e; A.bc_s = M.bc_s; A.bc_w = M.bc_w;
+    A.rk_order = 2; A.rk_step = L.rk_step;
+    A.tiles_x = M.ftiles_x; A.tiles_y = 0;
+    const int lt = t - M.ftile_begin;
+    const int ty = lt / M.ftiles_x, tx = lt - ty * M.ftiles_x;
+    cdklm_tile<kEnsTapeBW, kEnsTapeBH, kEnsTapeNT, kModeZero, true>(A, tx, ty, S);
+}
+
+struct EnsAdjLaunch {
+    const EnsMember* members;
+    const int* tile_member;
+    EnsRef eta, tape, w, w_clamp, x, x_clamp, o;   // x.base NULL: no x term
+    const float* Hi;
+    const float* dt;
+    float alpha, gamma;
+};
+
+__global__ __launch_bounds__(kAdjNT) void k_ens_adj_tile(EnsAdjLaunch L) {
+    __shared__ AdjSmem<kAdjBW, kAdjBH> S;
+    const int t = xcd_band(blockIdx.x, gridDim.x);
+    const int m = L.tile_member[t];
+    const EnsMember& M = L.members[m];
+    const long long pl = M.plane;
+    AdjTileArgs A;
+    A.eta = L.eta.base + ens_off(M, L.eta);
+    const float* tp = L.tape.base + ens_off(M, L.tape);
+    A.tU = tp; A.tV = tp + pl; A.tT = tp + 2 * pl; A.tS = tp + 3 * pl;
+    A.tC = reinterpret_cast<const uint32_t*>(tp + 4 * pl);
+    const float* w = L.w.base + ens_off(M, L.w);
+    A.w_eta = w; A.w_hu = w + pl; A.w_hv = w + 2 * pl;
+    A.w_clamp = L.w_clamp.base + ens_off(M, L.w_clamp);
+    if (L.x.base) {
+        const float* x = L.x.base + ens_off(M, L.x);
+        A.x_eta = x; A.x_hu = x + pl; A.x_hv = x + 2 * pl;
+        A.x_clamp = L.x_clamp.base + ens_off(M, L.x_clamp);
+    } else {
+        A.x_eta = nullptr; A.x_hu = nullptr; A.x_hv = nullptr; A.x_clamp = nullptr;
+    }
+    float* o = const_cast<float*>(L.o.base) + ens_off(M, L.o);
+    A.o_eta = o; A.o_hu = o + pl; A.o_hv = o + 2 * pl;
+    A.Hi = L.Hi + M.hi_off;
+    A.dt_dev = L.dt + m; A.dt = 0.0f;
+    A.alpha = L.alpha; A.gamma = L.gamma;
+    A.nx = M.nx; A.ny = M.ny; A.pitch = M.pitch; A.hi_pitch = M.hi_pitch;
+    A.dx = M.dx; A.dy = M.dy; A.g = M.g; A.theta = M.theta; A.desing = M.desing;
+    A.flux_eps = M.flux_eps; A.cutoff = M.cutoff; A.land = M.land; A.f = M.f;
+    A.bc_n = M.bc_n; A.bc_e = M.bc_e; A.bc_s = M.bc_s; A.bc_w = M.bc_w;
+    A.tiles_x = M.atiles_x; A.tiles_y = 0;
+    const int lt = t - M.atile_begin;
+    const int ty = lt / M.atiles_x, tx = lt - ty * M.atiles_x;
+    adj_tile_body<kAdjBW, kAdjBH, kAdjNT>(A, S, tx, ty, threadIdx.x, kAdjNT);
+}
+
+__global__ __launch_bounds__(256) void k_ens_adj_fold(const EnsMember* members, EnsRef o) {
+    const EnsMember& M = members[blockIdx.z];
+    FoldArgs F;
+    float* b = const_cast<float*>(o.base) + ens_off(M, o);
+    F.e = b; F.u = b + M.plane; F.v = b + 2 * M.plane;
+    F.nx = M.nx; F.ny = M.ny; F.pitch = M.pitch;
+    F.bc_n = M.bc_n; F.bc_e = M.bc_e; F.bc_s = M.bc_s; F.bc_w = M.bc_w;
+    fold_item(F, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 }  // namespace gocean
 
 using namespace gocean;
@@ -203,6 +326,11 @@ struct gocean_ensemble {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     hipStream_t capture_stream = nullptr;
+    // reverse mode
+    int adj_kmax = 0;
+    int n_ftiles = 0, n_atiles = 0, max_side = 0;
+    int* d_ftile_member = nullptr;
+    int* d_atile_member = nullptr;
 };
 
 namespace gocean {
@@ -356,6 +484,8 @@ int gocean_ensemble_destroy(gocean_ensemble_t* E) {
     if (E->capture_stream) hipStreamDestroy(E->capture_stream);
     hipFree(E->d_members);
     hipFree(E->d_tile_member);
+    hipFree(E->d_ftile_member);
+    hipFree(E->d_atile_member);
     delete E;
     return 0;
 }
@@ -454,6 +584,109 @@ int gocean_ensemble_replay(gocean_ensemble_t* E, void* stream) {
     if (!E->graph_exec) return set_error(GOCEAN_ESTATE, "no captured graph: call gocean_ensemble_capture first");
     const hipError_t e = hipGraphLaunch(E->graph_exec, (hipStream_t)stream);
     if (e != hipSuccess) return set_error((int)e, "graph launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// reverse mode
+// ---------------------------------------------------------------------------
+long long gocean_ensemble_adjoint_setup(gocean_ensemble_t* E, int max_steps) {
+    clear_error();
+    if (!E) return set_error(GOCEAN_EINVAL, "ensemble is NULL");
+    if (max_steps < 1) return set_error(GOCEAN_EINVAL, "max_steps must be >= 1");
+    if (E->mode != kModeZero)
+        return set_error(GOCEAN_EINVAL, "the adjoint supports constant f and zero grid rotation only");
+    if (E->rk_order != 2) return set_error(GOCEAN_EINVAL, "the adjoint supports rk_order 2 only");
+    std::vector<int> ftiles, atiles;
+    long long off = 0;
+    int max_side = 0;
+    for (int k = 0; k < E->n; ++k) {
+        EnsMember& M = E->members[k];
+        if (M.r != 0.0f) return set_error(GOCEAN_EINVAL, "member %d: the adjoint supports r == 0 only", k);
+        M.work_off = off;
+        off += (16LL * max_steps + 9LL) * M.plane;
+        M.ftiles_x = (M.nx + kEnsTapeBW - 1) / kEnsTapeBW;
+        M.ftile_begin = (int)ftiles.size();
+        for (int t = 0; t < M.ftiles_x * ((M.ny + kEnsTapeBH - 1) / kEnsTapeBH); ++t) ftiles.push_back(k);
+        M.atiles_x = (M.nx + kAdjBW - 1) / kAdjBW;
+        M.atile_begin = (int)atiles.size();
+        for (int t = 0; t < M.atiles_x * ((M.ny + kAdjBH - 1) / kAdjBH); ++t) atiles.push_back(k);
+        max_side = std::max(max_side, std::max(M.nx, M.ny));
+    }
+    hipFree(E->d_ftile_member);
+    hipFree(E->d_atile_member);
+    E->d_ftile_member = nullptr;
+    E->d_atile_member = nullptr;
+    if (hipMalloc(&E->d_ftile_member, sizeof(int) * ftiles.size()) != hipSuccess ||
+        hipMalloc(&E->d_atile_member, sizeof(int) * atiles.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(GOCEAN_ENOMEM, "device allocation of the adjoint tile maps failed");
+    }
+    hipMemcpy(E->d_ftile_member, ftiles.data(), sizeof(int) * ftiles.size(), hipMemcpyHostToDevice);
+    hipMemcpy(E->d_atile_member, atiles.data(), sizeof(int) * atiles.size(), hipMemcpyHostToDevice);
+    hipMemcpy(E->d_members, E->members.data(), sizeof(EnsMember) * E->n, hipMemcpyHostToDevice);
+    if (int e = check_launch("ensemble adjoint setup")) return e;
+    E->adj_kmax = max_steps;
+    E->n_ftiles = (int)ftiles.size();
+    E->n_atiles = (int)atiles.size();
+    E->max_side = max_side;
+    return off;
+}
+
+int gocean_ensemble_adjoint_segment(gocean_ensemble_t* E, int n_steps, const float* dt,
+                                    const float* state0, const float* lam_in, float* lam_out,
+                                    const float* Hi, float* work, long long work_floats, void* stream) {
+    clear_error();
+    if (!E || !dt || !state0 || !lam_in || !lam_out || !Hi || !work)
+        return set_error(GOCEAN_EINVAL, "NULL argument");
+    if (E->adj_kmax < 1) return set_error(GOCEAN_ESTATE, "call gocean_ensemble_adjoint_setup first");
+    if (n_steps < 1 || n_steps > E->adj_kmax)
+        return set_error(GOCEAN_EINVAL, "n_steps %d not in [1, %d]", n_steps, E->adj_kmax);
+    const EnsMember& last = E->members[E->n - 1];
+    if (work_floats < last.work_off + (16LL * E->adj_kmax + 9LL) * last.plane)
+        return set_error(GOCEAN_EINVAL, "work buffer too small");
+    hipStream_t s = (hipStream_t)stream;
+    const int K = E->adj_kmax, k = n_steps;
+    auto W = [&](int idx) { EnsRef r; r.base = work; r.in_work = 1; r.idx = idx; return r; };
+    auto F = [&](const float* b) { EnsRef r; r.base = b; r.in_work = 0; r.idx = 0; return r; };
+    auto Q = [&](int i) { return i == 0 ? F(state0) : W(3 * (i - 1)); };   // Q_i
+    auto Qs = [&](int i) { return W(3 * K + 3 * i); };                       // Q*_i
+    auto Tp = [&](int i, int sub) { return W(6 * K + 10 * i + 5 * sub); };  // branch record
+    const int gx = (E->max_side + 16 + 255) / 256;
+    auto fwd = [&](int rk_step, EnsRef in, EnsRef qn, EnsRef out, EnsRef tape) {
+        EnsTapeLaunch L;
+        L.members = E->d_members; L.tile_member = E->d_ftile_member;
+        L.in = in; L.qn = qn; L.out = out; L.tape = tape; L.Hi = Hi; L.dt = dt; L.rk_step = rk_step;
+        hipLaunchKernelGGL(k_ens_substep_tape, dim3(E->n_ftiles), dim3(kEnsTapeNT), 0, s, L);
+        return check_launch("ensemble recording substep launch");
+    };
+    auto adj = [&](EnsRef eta, EnsRef tape, EnsRef w, EnsRef wc, EnsRef x, EnsRef xc, float alpha,
+                   float gamma, EnsRef o) {
+        EnsAdjLaunch L;
+        L.members = E->d_members; L.tile_member = E->d_atile_member;
+        L.eta = eta; L.tape = tape; L.w = w; L.w_clamp = wc; L.x = x; L.x_clamp = xc; L.o = o;
+        L.Hi = Hi; L.dt = dt; L.alpha = alpha; L.gamma = gamma;
+        hipLaunchKernelGGL(k_ens_adj_tile, dim3(E->n_atiles), dim3(kAdjNT), 0, s, L);
+        hipLaunchKernelGGL(k_ens_adj_fold, dim3(gx, 9, E->n), dim3(256), 0, s, E->d_members, o);
+        return check_launch("ensemble adjoint launch");
+    };
+    for (int i = 0; i < k; ++i) {
+        if (int e = fwd(0, Q(i), Qs(i), Qs(i), Tp(i, 0))) return e;
+        if (int e = fwd(1, Qs(i), Q(i), W(3 * i), Tp(i, 1))) return e;
+    }
+    EnsRef none; none.base = nullptr; none.in_work = 0; none.idx = 0;
+    const EnsRef lam_s = W(16 * K), lam_a = W(16 * K + 3), lam_b = W(16 * K + 6);
+    EnsRef le = F(lam_in);
+    for (int i = k - 1; i >= 0; --i) {
+        const EnsRef next = W(3 * i);          // Q_{i+1}
+        EnsRef o;
+        if (i == 0) o = F(lam_out);
+        else o = (le.in_work && le.idx == lam_a.idx) ? lam_b : lam_a;
+        if (int e = adj(Qs(i), Tp(i, 1), le, next, none, none, 0.5f, 0.0f, lam_s)) return e;
+        if (int e = adj(Q(i), Tp(i, 0), lam_s, Qs(i), le, next, 1.0f, 0.5f, o)) return e;
+        le = o;
+    }
     return 0;
 }
 

@@ -194,6 +194,24 @@ int gocean_ensemble_capture(gocean_ensemble_t* ens, int n_steps, float* state0, 
                             float* state2, const float* Hi, const float* fields, const float* dt);
 int gocean_ensemble_replay(gocean_ensemble_t* ens, void* stream);
 
+/* Reverse mode of the batched ensemble (no counterpart in the reference; the
+ * batched form of gocean_adjoint_segment for data assimilation over an
+ * ensemble of domains).  Members must use constant f, north = (0, 1), r == 0
+ * and rk_order 2.  Setup builds the reverse-mode tile maps for segments of
+ * up to max_steps steps and returns the floats of work buffer the segments
+ * need (member m's block holds its (16 max_steps + 9) planes), or a negative
+ * error code. */
+long long gocean_ensemble_adjoint_setup(gocean_ensemble_t* ens, int max_steps);
+/* n_steps (<= max_steps) RK2 steps of every member from the checkpoint
+ * state0 with the per-member device dt: recompute with branch records, then
+ * map the members' adjoints at the segment end (lam_in, laid out like the
+ * state) to the segment start (lam_out).  Fused launches: each recording
+ * substep, adjoint tile pass and ghost fold covers all members at once. */
+int gocean_ensemble_adjoint_segment(gocean_ensemble_t* ens, int n_steps, const float* dt,
+                                    const float* state0, const float* lam_in, float* lam_out,
+                                    const float* Hi, float* work, long long work_floats,
+                                    void* stream);
+
 /* ---- adjoint (reverse mode) of the fixed-dt RK2 step ----
  * No counterpart in the reference, which has no gradients: this serves
  * variational data assimilation / parameter estimation callers that need
@@ -206,7 +224,9 @@ int gocean_ensemble_replay(gocean_ensemble_t* ens, void* stream);
  * upwinding, positivity adjustment, wet / desingularised / dry cells) follow
  * the forward evaluation: each entry point first re-runs the forward
  * substeps it differentiates in a recording variant that stores every such
- * choice, then applies the hand-written Jacobian-transpose kernels.
+ * choice, then applies the hand-written Jacobian-transpose kernels.  `wind`
+ * (nullable = none) is the uniform wind of the forward run being
+ * differentiated; the recomputed trajectory must match it.
  * Scratch / work buffers are caller-owned device memory. */
 
 /* Scratch of gocean_adjoint_substep / gocean_adjoint_rk2_step, in floats. */
@@ -219,8 +239,8 @@ long long gocean_adjoint_scratch_floats(const gocean_domain_t* d);
  * taken from w_clamp (a forward eta output) or, when w_clamp is NULL, from
  * this substep's own output.  x_clamp (nullable) masks x the same way; x may
  * be NULL. */
-int gocean_adjoint_substep(const gocean_domain_t* d, float dt, const float* dt_dev,
-                           const float* eta, const float* hu, const float* hv,
+int gocean_adjoint_substep(const gocean_domain_t* d, const gocean_wind_t* wind, float dt,
+                           const float* dt_dev, const float* eta, const float* hu, const float* hv,
                            const float* w_eta, const float* w_hu, const float* w_hv,
                            const float* w_clamp, const float* x_eta, const float* x_hu,
                            const float* x_hv, const float* x_clamp, float alpha, float gamma,
@@ -231,8 +251,9 @@ int gocean_adjoint_substep(const gocean_domain_t* d, float dt, const float* dt_d
  * Q* = P(Q^n), Q^{n+1} = (Q^n + P(Q*)) / 2 (CDKLM16.py:533-544).  Needs the
  * step's Q^n and Q* (and, nullable, eta of Q^{n+1} for the dry-clamp mask;
  * NULL recomputes it); o must not alias l. */
-int gocean_adjoint_rk2_step(const gocean_domain_t* d, float dt, const float* dt_dev,
-                            const float* eta_n, const float* hu_n, const float* hv_n,
+int gocean_adjoint_rk2_step(const gocean_domain_t* d, const gocean_wind_t* wind, float dt,
+                            const float* dt_dev, const float* eta_n, const float* hu_n,
+                            const float* hv_n,
                             const float* eta_s, const float* hu_s, const float* hv_s,
                             const float* eta_next, const float* l_eta, const float* l_hu,
                             const float* l_hv, float* o_eta, float* o_hu, float* o_hv,
@@ -246,8 +267,9 @@ long long gocean_adjoint_segment_work_floats(const gocean_domain_t* d, int n_ste
  * (eta0, hu0, hv0): recomputes the segment (with branch records) into `work`,
  * then maps lambda at the segment's end (l_*) to lambda at its start (o_*).
  * One call per checkpoint segment; everything runs on `stream`. */
-int gocean_adjoint_segment(const gocean_domain_t* d, int n_steps, float dt, const float* dt_dev,
-                           const float* eta0, const float* hu0, const float* hv0,
+int gocean_adjoint_segment(const gocean_domain_t* d, const gocean_wind_t* wind, int n_steps,
+                           float dt, const float* dt_dev, const float* eta0, const float* hu0,
+                           const float* hv0,
                            const float* l_eta, const float* l_hu, const float* l_hv,
                            float* o_eta, float* o_hu, float* o_hv, const float* Hi,
                            float* work, long long work_floats, void* stream);

@@ -112,10 +112,10 @@ def test_no_gpu_means_zero_devices_or_more():
 def test_adjoint_scope_rejected(kw, msg):
     lib = G.load()
     d = _dom(**kw)
-    rc = lib.gocean_adjoint_rk2_step(d, 1.0, None, *([16] * 6), None, 16, 16, 16, 32, 32, 32,
+    rc = lib.gocean_adjoint_rk2_step(d, None, 1.0, None, *([16] * 6), None, 16, 16, 16, 32, 32, 32,
                                      64, 128, None)
     assert rc == -1 and msg in lib.gocean_last_error().decode()
-    rc = lib.gocean_adjoint_substep(d, 1.0, None, *([16] * 6), None, None, None, None, None,
+    rc = lib.gocean_adjoint_substep(d, None, 1.0, None, *([16] * 6), None, None, None, None, None,
                                     1.0, 0.0, 32, 32, 32, 64, 128, None)
     assert rc == -1 and msg in lib.gocean_last_error().decode()
 
@@ -124,13 +124,13 @@ def test_adjoint_arguments_rejected():
     lib = G.load()
     d = _dom()
     assert lib.gocean_adjoint_scratch_floats(d) == 19 * (d.ny + 4) * d.pitch
-    rc = lib.gocean_adjoint_rk2_step(d, 1.0, None, *([16] * 6), None, 16, 16, 16, 16, 32, 32,
+    rc = lib.gocean_adjoint_rk2_step(d, None, 1.0, None, *([16] * 6), None, 16, 16, 16, 16, 32, 32,
                                      64, 128, None)
     assert rc == -1 and "alias" in lib.gocean_last_error().decode()
-    rc = lib.gocean_adjoint_substep(d, 1.0, None, *([16] * 6), None, 16, None, None, None,
+    rc = lib.gocean_adjoint_substep(d, None, 1.0, None, *([16] * 6), None, 16, None, None, None,
                                     1.0, 0.0, 32, 32, 32, 64, 128, None)
     assert rc == -1 and "extra" in lib.gocean_last_error().decode()
-    rc = lib.gocean_adjoint_substep(d, 1.0, None, *([16] * 6), None, None, None, None, None,
+    rc = lib.gocean_adjoint_substep(d, None, 1.0, None, *([16] * 6), None, None, None, None, None,
                                     1.0, 0.0, 32, 32, 32, 64, None, None)
     assert rc == -1 and "NULL" in lib.gocean_last_error().decode()
 
@@ -141,12 +141,12 @@ def test_adjoint_segment_arguments_rejected():
     pl = (d.ny + 4) * d.pitch
     assert lib.gocean_adjoint_segment_work_floats(d, 5) == (16 * 5 + 9) * pl
     assert lib.gocean_adjoint_segment_work_floats(d, 0) == -1
-    rc = lib.gocean_adjoint_segment(d, 5, 1.0, None, 16, 16, 16, 32, 32, 32, 48, 48, 48, 64, 128,
+    rc = lib.gocean_adjoint_segment(d, None, 5, 1.0, None, 16, 16, 16, 32, 32, 32, 48, 48, 48, 64, 128,
                                     (16 * 5 + 9) * pl - 1, None)
     assert rc == -1 and "too small" in lib.gocean_last_error().decode()
-    rc = lib.gocean_adjoint_segment(d, 0, 1.0, None, 16, 16, 16, 32, 32, 32, 48, 48, 48, 64, 128,
+    rc = lib.gocean_adjoint_segment(d, None, 0, 1.0, None, 16, 16, 16, 32, 32, 32, 48, 48, 48, 64, 128,
                                     1 << 40, None)
     assert rc == -1 and "n_steps" in lib.gocean_last_error().decode()
-    rc = lib.gocean_adjoint_segment(_dom(rk_order=3), 2, 1.0, None, 16, 16, 16, 32, 32, 32, 48, 48,
+    rc = lib.gocean_adjoint_segment(_dom(rk_order=3), None, 2, 1.0, None, 16, 16, 16, 32, 32, 32, 48, 48,
                                     48, 64, 128, 1 << 40, None)
     assert rc == -1 and "rk_order 2" in lib.gocean_last_error().decode()

@@ -119,7 +119,7 @@ def test_adjoint_substep_vs_torch_fp64(bcname, bc, kind, f):
     G.check(lib.gocean_cdklm_substep(d, dt, None, 0, *(G.ptr(a) for a in q), None, None, None,
                                      *(G.ptr(a) for a in fwd), G.ptr(Hi_d), G.Fields(), G.Wind(), -1,
                                      None), "forward substep")
-    G.check(lib.gocean_adjoint_substep(d, dt, None, *(G.ptr(a) for a in q), *(G.ptr(a) for a in w),
+    G.check(lib.gocean_adjoint_substep(d, None, dt, None, *(G.ptr(a) for a in q), *(G.ptr(a) for a in w),
                                        G.ptr(fwd[0]), None, None, None, None, 1.0, 0.0,
                                        *(G.ptr(a) for a in o), G.ptr(Hi_d), G.ptr(scratch), None),
             "adjoint substep")
@@ -407,3 +407,51 @@ def test_adjoint_rejects_out_of_scope_domains():
     d.rk_order = 3
     with pytest.raises(ValueError):
         A.RK2Adjoint(d, _dev(Hi))
+
+
+def test_ensemble_adjoint_matches_single_domain():
+    """The batched ensemble reverse mode (fused launches over members of
+    different sizes, boundaries, f, dx and dt) gives each member exactly the
+    gradient of the single-domain reverse mode."""
+    _say("ensemble adjoint vs single-domain adjoint")
+    from SWESimulators import Common, CDKLM16Ensemble
+    from test_gpu_ensemble import members
+    Common_, G, A = _mods()
+    ms = members(seed=3)[:3]
+    ctx = Common.HIPContext(0)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, rk_order=2, courant_number=0.6)
+    n, K = 7, 3
+    ea = A.EnsembleAdjoint(ens, checkpoint_every=K)
+    state0 = ens.state[0].clone()
+    out = ea.forward(state0, n)
+    rng = np.random.default_rng(4)
+    lam = torch.zeros_like(state0)
+    for k in range(ens.n):
+        o, pl = int(ens.state_off[k]), ens.planes[k]
+        for f in range(3):
+            v = lam[o + f * pl:o + (f + 1) * pl].view(ens.ny[k] + 4, ens.pitch[k])
+            v[2:-2, 2:-2] = torch.from_numpy(rng.standard_normal((ens.ny[k], ens.nx[k])).astype(np.float32))
+    grad = ea.backward(lam)
+    torch.cuda.synchronize()
+    dts = ens.dt_host()
+    for k, m in enumerate(ms):
+        nx, ny = ens.nx[k], ens.ny[k]
+        d = Common.make_domain(nx, ny, m["dx"], m["dy"], m["g"], 0.0, 1.3, 2, m["bc"], pitch=ens.pitch[k],
+                               hi_pitch=nx + 5, f=m["f"])
+        Hi_k = ens.Hi[int(ens.hi_off[k]):int(ens.hi_off[k + 1])].view(ny + 5, nx + 5)
+        w = G.Wind()
+        w.x0 = w.x1 = float(np.float32(m["wind_x"]))
+        w.y0 = w.y1 = float(np.float32(m["wind_y"]))
+        adj = A.RK2Adjoint(d, Hi_k, wind=w, checkpoint_every=K)
+        o, pl = int(ens.state_off[k]), ens.planes[k]
+        view = lambda t: tuple(t[o + f * pl:o + (f + 1) * pl].view(ny + 4, ens.pitch[k]) for f in range(3))
+        fo = adj.forward(view(state0), n, float(dts[k]))
+        gk = adj.backward(view(lam))
+        torch.cuda.synchronize()
+        for a, b in zip(fo, view(out)):
+            assert torch.equal(a, b), ("forward", k)
+        for name, a, b in zip(("eta", "hu", "hv"), gk, view(grad)):
+            same = torch.equal(a, b)
+            _say("   member", k, name, "bit-identical" if same else
+                 f"max |d| {float((a - b).abs().max()):.3e} of {float(a.abs().max()):.3e}")
+            assert same, (k, name)
