@@ -336,6 +336,7 @@ def main():
     ap.add_argument("--n", type=int, default=2048, help="domain side (headline: 2048)")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-c5", action="store_true", help="skip the batched uneven ensemble adjoint")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -388,9 +389,10 @@ def main():
         log("[bench] adjoint C4")
         extra["adjoint"] = bench_adjoint(ctx)
         log(f"[bench] adjoint: {extra['adjoint']}")
-        log("[bench] batched uneven ensemble with adjoint C5")
-        extra["ensemble_adjoint"] = bench_c5(ctx)
-        log(f"[bench] ensemble adjoint: {extra['ensemble_adjoint']}")
+        if not a.no_c5:
+            log("[bench] batched uneven ensemble with adjoint C5")
+            extra["ensemble_adjoint"] = bench_c5(ctx)
+            log(f"[bench] ensemble adjoint: {extra['ensemble_adjoint']}")
 
     cpu = None
     if not a.no_cpu:

@@ -43,7 +43,23 @@ struct AdjTileArgs {
     float dx, dy, g, theta, desing, flux_eps, cutoff, land, f;
     int bc_n, bc_e, bc_s, bc_w;
     int tiles_x, tiles_y;
+    // derived on the host (derivative values only, never branch decisions):
+    // 1/dx, 1/dy, 1/(2g), dx/(2g), dy/(2g), dx*f, dy*f, g/dx, g/dy
+    float r_dx, r_dy, r_two_g, Dx, Dy, Ddx, Ddy, g_dx, g_dy;
 };
+
+// fill the derived constants of AdjTileArgs from its base fields
+__host__ __device__ inline void adj_derive(AdjTileArgs& A) {
+    A.r_dx = 1.0f / A.dx;
+    A.r_dy = 1.0f / A.dy;
+    A.r_two_g = 1.0f / (2.0f * A.g);
+    A.Dx = A.dx / (2.0f * A.g);
+    A.Dy = A.dy / (2.0f * A.g);
+    A.Ddx = A.dx * A.f;
+    A.Ddy = A.dy * A.f;
+    A.g_dx = A.g / A.dx;
+    A.g_dy = A.g / A.dy;
+}
 
 __host__ __device__ inline float adj_rcp(float x) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -87,10 +103,10 @@ __host__ __device__ inline void slope_coef(uint32_t sel, float theta, float& kl,
 // or the positivity adjustment sig Dd s_a t_c +- 2g (E_c + H).  Adds
 // d/d(E, t) of cells l, c, r times gT.
 __host__ __device__ inline void depth_term_vjp(uint32_t ks, uint32_t adj, float sig, float theta, float g,
-                                               float f, float Dd, float D, float sl, float sc, float sr,
-                                               float sa, float gT, float* gE, float* gt) {
+                                               float f, float Dd, float D, float r_two_g, float sl, float sc,
+                                               float sr, float sa, float gT, float* gE, float* gt) {
     const float two_g = 2.0f * g;
-    const float gK = gT / two_g;
+    const float gK = gT * r_two_g;
     float el, ec, er, tl, tc, tr;
     if (adj != 0u) {
         el = 0.0f; er = 0.0f; ec = adj == 1u ? two_g : -two_g;
@@ -303,10 +319,10 @@ struct AdjTile {
             const float Em = S.E[lr][m], Ep = S.E[lr][p];
             if (!(Em == kDryFlag || Ep == kDryFlag)) {
                 const uint32_t km = S.code[lr][m], kp = S.code[lr][p];
-                const float rdx = 1.0f / A.dx;
-                const float l0 = dt * (S.W0[lr][p] - S.W0[lr][m]) * rdx;
-                const float l1 = dt * (S.W1[lr][p] - S.W1[lr][m]) * rdx;
-                const float l2 = dt * (S.W2[lr][p] - S.W2[lr][m]) * rdx;
+                const float rdx = dt * A.r_dx;
+                const float l0 = (S.W0[lr][p] - S.W0[lr][m]) * rdx;
+                const float l1 = (S.W1[lr][p] - S.W1[lr][m]) * rdx;
+                const float l2 = (S.W2[lr][p] - S.W2[lr][m]) * rdx;
                 const float th = A.theta;
                 const float Sum = slope_val(km & 3u, S.U[lr][m - 1], S.U[lr][m], S.U[lr][m + 1], th);
                 const float Svm = slope_val((km >> 2) & 3u, S.V[lr][m - 1], S.V[lr][m], S.V[lr][m + 1], th);
@@ -355,8 +371,8 @@ struct AdjTile {
             const int C = c;
             const float sa = (A.bc_w == kWall && C < 2) ? -1.0f : 1.0f;
             float gE[3] = {0.0f, 0.0f, 0.0f}, gt[3] = {0.0f, 0.0f, 0.0f};
-            depth_term_vjp((code >> 4) & 3u, (code >> 6) & 3u, -1.0f, th, A.g, A.f, A.dx * A.f,
-                           A.dx / (2.0f * A.g), col_flip(C - 1), col_flip(C), col_flip(C + 1), sa, gT, gE, gt);
+            depth_term_vjp((code >> 4) & 3u, (code >> 6) & 3u, -1.0f, th, A.g, A.f, A.Ddx, A.Dx, A.r_two_g,
+                           col_flip(C - 1), col_flip(C), col_flip(C + 1), sa, gT, gE, gt);
             out[0] += gE[0]; out[3] += gE[1]; out[6] += gE[2];
             out[2] += gt[0]; out[5] += gt[1]; out[8] += gt[2];
         }
@@ -390,11 +406,11 @@ struct AdjTile {
             const float Em = S.E[m][lc], Ep = S.E[p][lc];
             if (!(Em == kDryFlag || Ep == kDryFlag)) {
                 const uint32_t km = S.code[m][lc], kp = S.code[p][lc];
-                const float rdy = 1.0f / A.dy;
+                const float rdy = dt * A.r_dy;
                 // flux order (mass, normal = v momentum, tangential = u momentum)
-                const float l0 = dt * (S.W0[p][lc] - S.W0[m][lc]) * rdy;
-                const float l1 = dt * (S.W2[p][lc] - S.W2[m][lc]) * rdy;
-                const float l2 = dt * (S.W1[p][lc] - S.W1[m][lc]) * rdy;
+                const float l0 = (S.W0[p][lc] - S.W0[m][lc]) * rdy;
+                const float l1 = (S.W2[p][lc] - S.W2[m][lc]) * rdy;
+                const float l2 = (S.W1[p][lc] - S.W1[m][lc]) * rdy;
                 const float th = A.theta;
                 const float Sum = slope_val((km >> 8) & 3u, S.U[m - 1][lc], S.U[m][lc], S.U[m + 1][lc], th);
                 const float Svm = slope_val((km >> 10) & 3u, S.V[m - 1][lc], S.V[m][lc], S.V[m + 1][lc], th);
@@ -442,8 +458,8 @@ struct AdjTile {
             const int R = c;
             const float sa = (A.bc_s == kWall && R < 2) ? -1.0f : 1.0f;
             float gE[3] = {0.0f, 0.0f, 0.0f}, gt[3] = {0.0f, 0.0f, 0.0f};
-            depth_term_vjp((code >> 4) & 3u, (code >> 6) & 3u, 1.0f, th, A.g, A.f, A.dy * A.f,
-                           A.dy / (2.0f * A.g), row_flip(R - 1), row_flip(R), row_flip(R + 1), sa, gT, gE, gt);
+            depth_term_vjp((code >> 4) & 3u, (code >> 6) & 3u, 1.0f, th, A.g, A.f, A.Ddy, A.Dy, A.r_two_g,
+                           row_flip(R - 1), row_flip(R), row_flip(R + 1), sa, gT, gE, gt);
             out[0] += gE[0]; out[3] += gE[1]; out[6] += gE[2];
             out[1] += gt[0]; out[4] += gt[1]; out[7] += gt[2];
         }
@@ -483,9 +499,9 @@ struct AdjTile {
                 const float H_x = 0.5f * (c + d) - 0.5f * (a + b);
                 const float H_y = 0.5f * (b + d) - 0.5f * (a + c);
                 ghv += dt * w1 * A.f;
-                gE += dt * w1 * A.g * H_x / A.dx;
+                gE += dt * w1 * A.g_dx * H_x;
                 ghu += -dt * w2 * A.f;
-                gE += dt * w2 * A.g * H_y / A.dy;
+                gE += dt * w2 * A.g_dy * H_y;
             }
             if (hreg > A.cutoff) {
                 gU += ghu * hreg; gE += ghu * U;
@@ -504,12 +520,13 @@ struct AdjTile {
             g_hv = gV * rh;
         } else if (br == kBrDesing) {
             const float eps = A.desing;
-            const float aq = h * h / (2.0f * eps) + 0.5f * eps;
+            const float r_eps = adj_rcp(eps);
+            const float aq = h * h * (0.5f * r_eps) + 0.5f * eps;
             const float mn = fminf(aq, eps);
             const float den = fmaxf(mn, fabsf(h));
             float dden = 0.0f;
             if (fabsf(h) >= mn) dden = (h >= 0.0f) ? 1.0f : -1.0f;
-            else if (aq <= eps) dden = h / eps;
+            else if (aq <= eps) dden = h * r_eps;
             const float rden = adj_rcp(den);
             g_eta = gE - (gU * U + gV * V) * dden * rden;
             g_hu = gU * rden;
@@ -553,7 +570,7 @@ __host__ __device__ inline void adj_tile_body(const AdjTileArgs& A, AdjSmem<BW, 
     for (int it = tid; it < Tl::kCellItems; it += nthreads) T.stage_out(it);
 }
 
-constexpr int kAdjBW = 32, kAdjBH = 8, kAdjNT = 256;
+constexpr int kAdjBW = 32, kAdjBH = 16, kAdjNT = 512;
 
 // Ghost cells back to the interior cells they are images of.  One thread per
 // interior cell within two cells of the boundary; ghost entries are read by

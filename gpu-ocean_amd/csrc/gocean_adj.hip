@@ -47,6 +47,7 @@ static void adj_fill(const gocean_domain_t* d, AdjTileArgs& A) {
     A.bc_n = d->bc_north; A.bc_e = d->bc_east; A.bc_s = d->bc_south; A.bc_w = d->bc_west;
     A.tiles_x = (d->nx + kAdjBW - 1) / kAdjBW;
     A.tiles_y = (d->ny + kAdjBH - 1) / kAdjBH;
+    adj_derive(A);
 }
 
 struct Tape {

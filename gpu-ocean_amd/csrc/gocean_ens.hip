@@ -34,6 +34,7 @@ struct EnsMember {
     // adjoint (kAdjBW x kAdjBH) launches
     long long work_off;
     int ftiles_x, ftile_begin, atiles_x, atile_begin;
+    float r_dx, r_dy, r_two_g, Dx, Dy, Ddx, Ddy, g_dx, g_dy;   // as AdjTileArgs
 };
 
 struct EnsLaunch {
@@ -293,6 +294,8 @@ __global__ __launch_bounds__(kAdjNT) void k_ens_adj_tile(EnsAdjLaunch L) {
     A.flux_eps = M.flux_eps; A.cutoff = M.cutoff; A.land = M.land; A.f = M.f;
     A.bc_n = M.bc_n; A.bc_e = M.bc_e; A.bc_s = M.bc_s; A.bc_w = M.bc_w;
     A.tiles_x = M.atiles_x; A.tiles_y = 0;
+    A.r_dx = M.r_dx; A.r_dy = M.r_dy; A.r_two_g = M.r_two_g; A.Dx = M.Dx; A.Dy = M.Dy;
+    A.Ddx = M.Ddx; A.Ddy = M.Ddy; A.g_dx = M.g_dx; A.g_dy = M.g_dy;
     const int lt = t - M.atile_begin;
     const int ty = lt / M.atiles_x, tx = lt - ty * M.atiles_x;
     adj_tile_body<kAdjBW, kAdjBH, kAdjNT>(A, S, tx, ty, threadIdx.x, kAdjNT);
@@ -605,6 +608,12 @@ long long gocean_ensemble_adjoint_setup(gocean_ensemble_t* E, int max_steps) {
         EnsMember& M = E->members[k];
         if (M.r != 0.0f) return set_error(GOCEAN_EINVAL, "member %d: the adjoint supports r == 0 only", k);
         M.work_off = off;
+        AdjTileArgs D;
+        std::memset(&D, 0, sizeof(D));
+        D.dx = M.dx; D.dy = M.dy; D.g = M.g; D.f = M.f;
+        adj_derive(D);
+        M.r_dx = D.r_dx; M.r_dy = D.r_dy; M.r_two_g = D.r_two_g; M.Dx = D.Dx; M.Dy = D.Dy;
+        M.Ddx = D.Ddx; M.Ddy = D.Ddy; M.g_dx = D.g_dx; M.g_dy = D.g_dy;
         off += (16LL * max_steps + 9LL) * M.plane;
         M.ftiles_x = (M.nx + kEnsTapeBW - 1) / kEnsTapeBW;
         M.ftile_begin = (int)ftiles.size();
