@@ -14,9 +14,8 @@ Differences from the reference, by design:
   * runs of equal time steps are issued with one library call
     (``gocean_rk_steps``); ``step_n`` and ``capture_graph`` expose fixed-dt
     stepping without any host synchronisation;
-  * open boundaries (3, 4), gridded wind, stochastic model error
-    (OceanStateNoise) and NetCDF output are not part of this build and raise
-    ``NotImplementedError``.
+  * open boundaries (3, 4), gridded wind and NetCDF output are not part of
+    this build and raise ``NotImplementedError``.
 """
 
 from __future__ import annotations
@@ -26,7 +25,7 @@ import logging
 
 import numpy as np
 
-from . import Common, Simulator, WindStress
+from . import Common, OceanStateNoise, Simulator, WindStress
 from . import _gocean as G
 
 
@@ -68,8 +67,6 @@ class CDKLM16(Simulator.Simulator):
         if any(b not in (Common.WALL, Common.PERIODIC) for b in bc_codes):
             raise NotImplementedError("open boundary conditions (3: flow relaxation, "
                                       "4: linear interpolation) are not part of this build")
-        if small_scale_perturbation:
-            raise NotImplementedError("stochastic model error (OceanStateNoise) is not part of this build")
         if write_netcdf:
             raise NotImplementedError("NetCDF output is not part of this build")
         if max_wind_direction_perturbation > 0:
@@ -154,9 +151,16 @@ class CDKLM16(Simulator.Simulator):
         self.north_const = (float(np.float32(np.sin(a32))), float(np.float32(np.cos(a32))))
         self._make_device_descriptors()
 
-        self.small_scale_perturbation = False
+        # model error (CDKLM16.py:352-362)
+        self.small_scale_perturbation = small_scale_perturbation
         self.small_scale_model_error = None
         self.small_scale_perturbation_interpolation_factor = small_scale_perturbation_interpolation_factor
+        if small_scale_perturbation:
+            self.small_scale_model_error = OceanStateNoise.OceanStateNoise.fromsim(
+                self, soar_q0=small_scale_perturbation_amplitude,
+                interpolation_factor=small_scale_perturbation_interpolation_factor,
+                use_lcg=use_lcg, xorwow_seed=xorwow_seed,
+                block_width=block_width_model_error, block_height=block_height_model_error)
         self.model_time_step = model_time_step
         self.total_time_steps = 0
         if model_time_step is None:
@@ -260,7 +264,9 @@ class CDKLM16(Simulator.Simulator):
         if self.t == 0:
             self._periodic_bc(self.gpu_data.h0, self.gpu_data.hu0, self.gpu_data.hv0)
             self._periodic_bc(self.gpu_data.h1, self.gpu_data.hu1, self.gpu_data.hv1)
-        groupable = (not update_dt) and (not self.hasDrifters) and len(self.wind_stress.t) == 1
+        perturb = self.small_scale_perturbation and apply_stochastic_term
+        groupable = (not update_dt) and (not self.hasDrifters) and len(self.wind_stress.t) == 1 \
+            and not perturb
         t_now = 0.0
         if groupable:
             # the loop's dt sequence does not depend on the state: issue each
@@ -287,6 +293,10 @@ class CDKLM16(Simulator.Simulator):
                 self.updateDt()
             local_dt = np.float32(min(self.dt, np.float32(t_end - t_now)))
             self._rk_steps(1, local_dt, self._wind())
+            if perturb:
+                # model error, then the ghost cells (CDKLM16.py:580-586)
+                self.small_scale_model_error.perturbSim(self)
+                self._periodic_bc(self.gpu_data.h0, self.gpu_data.hu0, self.gpu_data.hv0)
             self.drifterStep(local_dt)
             self.t += np.float64(local_dt)
             t_now += np.float64(local_dt)
@@ -329,6 +339,7 @@ class CDKLM16(Simulator.Simulator):
     def perturbState(self, q0_scale=1):
         if self.small_scale_perturbation:
             self.small_scale_model_error.perturbSim(self, q0_scale=q0_scale)
+            self._periodic_bc(self.gpu_data.h0, self.gpu_data.hu0, self.gpu_data.hv0)
 
     def applyBoundaryConditions(self):
         self._periodic_bc(self.gpu_data.h0, self.gpu_data.hu0, self.gpu_data.hv0)

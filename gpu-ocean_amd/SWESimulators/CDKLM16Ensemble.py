@@ -112,6 +112,10 @@ class CDKLM16Ensemble(object):
                     "ensemble periodic bc")
         self.t = np.zeros(self.n)
         self.num_iterations = 0
+        self.noise = None
+        self.f = [float(np.float32(_get(m, "f", 0.0))) for m in members]
+        self.g = [float(np.float32(_get(m, "g", 9.81))) for m in members]
+        self.dxy = [(float(np.float32(_get(m, "dx"))), float(np.float32(_get(m, "dy")))) for m in members]
         if dt is None:
             self.update_dt()
         else:
@@ -160,25 +164,131 @@ class CDKLM16Ensemble(object):
     def _ptr2(self):
         return G.ptr(self.state2) if self.state2 is not None else None
 
-    def step_n(self, n_steps):
-        """``n_steps`` RK steps of every member with its own device dt."""
+    def _rk(self, n_steps):
         G.check(self.lib.gocean_ensemble_rk_steps(
             self.handle, int(n_steps), self.state[0].data_ptr(), self.state[1].data_ptr(), self._ptr2(),
             self.Hi.data_ptr(), None, self.dt.data_ptr(), G.stream_handle(self.gpu_stream)),
             "gocean_ensemble_rk_steps")
+
+    def step_n(self, n_steps, model_error=False):
+        """``n_steps`` RK steps of every member with its own device dt; with
+        ``model_error`` each step is followed by the batched perturbation and
+        the ghost-cell exchange (CDKLM16.step with a small-scale
+        perturbation, CDKLM16.py:580-586)."""
+        if not model_error:
+            self._rk(n_steps)
+        else:
+            for _ in range(int(n_steps)):
+                self._rk(1)
+                self.perturb()
         self.num_iterations += int(n_steps)
 
-    def capture(self, n_steps):
-        """Capture ``n_steps`` RK steps of all members in a HIP graph."""
+    # ------------------------------------------------------------------
+    # model error (OceanStateNoise for every member in batched launches)
+    # ------------------------------------------------------------------
+    def enable_model_error(self, soar_q0=None, soar_L=None, interpolation_factor=1, use_lcg=False,
+                           seed=None, beta=0.0):
+        """Set up the batched model error of every member: the member-wise
+        OceanStateNoise (OceanStateNoise.py:49-170) with one device table.
+        ``seed`` seeds the LCG states (use_lcg) or keys the Philox generator."""
+        torch = Common._torch()
+        arr = (G.NoiseMember * self.n)()
+        for k in range(self.n):
+            nx, ny = self.nx[k], self.ny[k]
+            dx, dy = self.dxy[k]
+            f = int(interpolation_factor)
+            d = arr[k].noise
+            d.nx, d.ny, d.pitch, d.hi_pitch = nx, ny, self.pitch[k], nx + 5
+            d.dx, d.dy = dx, dy
+            d.interpolation_factor = f
+            d.periodic_ns = int(self.bcs[k][0] == Common.PERIODIC)
+            d.periodic_ew = int(self.bcs[k][1] == Common.PERIODIC)
+            d.soar_q0 = float(np.float32(dx / 100000 if soar_q0 is None else soar_q0))
+            d.soar_L = float(np.float32(0.75 * np.float32(dx * f) if soar_L is None else soar_L))
+            d.g, d.f, d.beta = self.g[k], self.f[k], float(np.float32(beta))
+            d.north_x, d.north_y = 0.0, 1.0
+            d.land_value = float(np.float32(1.0e20))
+            arr[k].state_offset = int(self.state_off[k])
+            arr[k].hi_offset = int(self.hi_off[k])
+            arr[k].field_offset = -1
+        kind = G.RNG_LCG_NORMAL if use_lcg else G.RNG_PHILOX_NORMAL
+        h = ctypes.c_void_p()
+        G.check(self.lib.gocean_noise_batch_create(arr, self.n, kind, ctypes.byref(h)),
+                "gocean_noise_batch_create")
+        self.noise = h
+        self.noise_members = arr
+        self.noise_use_lcg = use_lcg
+        offs = (ctypes.c_int64 * (3 * self.n))()
+        tot = (ctypes.c_int64 * 3)()
+        G.check(self.lib.gocean_noise_batch_layout(h, offs, tot), "gocean_noise_batch_layout")
+        self.noise_offsets = np.array(list(offs), dtype=np.int64).reshape(self.n, 3)
+        rs = np.random.RandomState(seed)
+        dev = self.gpu_ctx.torch_device
+        with torch.cuda.stream(self.gpu_stream):
+            self.noise_seed = None
+            if use_lcg:
+                host = (rs.rand(int(tot[0])) * 2147483648.0).astype(np.uint64).astype(np.int64)
+                self.noise_seed = torch.from_numpy(host).to(dev)
+            self.noise_random = torch.zeros(max(1, int(tot[1])), dtype=torch.float32, device=dev)
+            self.noise_coarse = torch.zeros(max(1, int(tot[2])), dtype=torch.float32, device=dev)
+            self.noise_counter = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.noise_key = int(rs.randint(0, 2 ** 31)) | (int(rs.randint(0, 2 ** 31)) << 31)
+
+    def perturb(self, q0_scale=1.0, perturbation_scale=1.0, update_random_field=True):
+        """perturbSim for every member (OceanStateNoise.py:376-496) followed
+        by the periodic ghost exchange, all on the device."""
+        if self.noise is None:
+            raise RuntimeError("enable_model_error() first")
+        st = G.stream_handle(self.gpu_stream)
+        G.check(self.lib.gocean_noise_batch_perturb(
+            self.noise, self.state[0].data_ptr(), self.Hi.data_ptr(), None, float(np.float32(q0_scale)),
+            float(np.float32(perturbation_scale)), int(bool(update_random_field)),
+            G.ptr(self.noise_seed), self.noise_random.data_ptr(), self.noise_coarse.data_ptr(),
+            self.noise_key, self.noise_counter.data_ptr(), st), "gocean_noise_batch_perturb")
+        G.check(self.lib.gocean_ensemble_periodic_bc(self.handle, self.state[0].data_ptr(), st),
+                "ensemble periodic bc")
+
+    def noise_views(self, k):
+        """(random field, coarse buffer) device views of member k."""
+        d = self.noise_members[k].noise
+        f = d.interpolation_factor
+        cnx, cny = d.nx // f, d.ny // f
+        gx = 0 if d.periodic_ew else 4
+        gy = 0 if d.periodic_ns else 4
+        rnx, rny = cnx + 2 * gx, cny + 2 * gy
+        so, ro, co = (int(v) for v in self.noise_offsets[k])
+        return (self.noise_random[ro:ro + rnx * rny].view(rny, rnx),
+                self.noise_coarse[co:co + (cnx + 4) * (cny + 4)].view(cny + 4, cnx + 4))
+
+    def capture(self, n_steps, model_error=False):
+        """Capture ``n_steps`` RK steps of all members in a HIP graph (with
+        ``model_error``, each step followed by the batched perturbation; the
+        generator state lives on the device, so every replay draws new
+        numbers)."""
         self.gpu_stream.synchronize()
-        G.check(self.lib.gocean_ensemble_capture(
-            self.handle, int(n_steps), self.state[0].data_ptr(), self.state[1].data_ptr(), self._ptr2(),
-            self.Hi.data_ptr(), None, self.dt.data_ptr()), "gocean_ensemble_capture")
         self.graph_steps = int(n_steps)
+        if not model_error:
+            G.check(self.lib.gocean_ensemble_capture(
+                self.handle, int(n_steps), self.state[0].data_ptr(), self.state[1].data_ptr(), self._ptr2(),
+                self.Hi.data_ptr(), None, self.dt.data_ptr()), "gocean_ensemble_capture")
+            self.torch_graph = None
+            return
+        torch = Common._torch()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.gpu_stream):
+            for _ in range(int(n_steps)):
+                self._rk(1)
+                self.perturb()
+        self.torch_graph = g
 
     def replay(self):
-        G.check(self.lib.gocean_ensemble_replay(self.handle, G.stream_handle(self.gpu_stream)),
-                "gocean_ensemble_replay")
+        if getattr(self, "torch_graph", None) is not None:
+            torch = Common._torch()
+            with torch.cuda.stream(self.gpu_stream):
+                self.torch_graph.replay()
+        else:
+            G.check(self.lib.gocean_ensemble_replay(self.handle, G.stream_handle(self.gpu_stream)),
+                    "gocean_ensemble_replay")
         self.num_iterations += self.graph_steps
 
     def cells(self):
@@ -188,6 +298,9 @@ class CDKLM16Ensemble(object):
         if getattr(self, "handle", None) is not None:
             self.lib.gocean_ensemble_destroy(self.handle)
             self.handle = None
+        if getattr(self, "noise", None) is not None:
+            self.lib.gocean_noise_batch_destroy(self.noise)
+            self.noise = None
 
     def __del__(self):
         try:

@@ -118,6 +118,54 @@ PROTOTYPES.extend([
 ])
 
 
+class Noise(ctypes.Structure):
+    """gocean_noise_t"""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+        ("pitch", ctypes.c_int32), ("hi_pitch", ctypes.c_int32),
+        ("dx", c_float), ("dy", c_float),
+        ("interpolation_factor", ctypes.c_int32),
+        ("periodic_ns", ctypes.c_int32), ("periodic_ew", ctypes.c_int32),
+        ("soar_q0", c_float), ("soar_L", c_float),
+        ("g", c_float), ("f", c_float), ("beta", c_float),
+        ("north_x", c_float), ("north_y", c_float),
+        ("land_value", c_float),
+    ]
+
+
+class NoiseMember(ctypes.Structure):
+    """gocean_noise_member_t"""
+    _fields_ = [("noise", Noise), ("state_offset", ctypes.c_int64),
+                ("hi_offset", ctypes.c_int64), ("field_offset", ctypes.c_int64)]
+
+
+RNG_LCG_NORMAL, RNG_LCG_UNIFORM, RNG_PHILOX_NORMAL, RNG_PHILOX_UNIFORM = 0, 1, 2, 3
+
+PROTOTYPES.extend([
+    ("gocean_noise_sizes", c_int, [ctypes.POINTER(Noise), ctypes.POINTER(ctypes.c_int32)]),
+    ("gocean_noise_generate", c_int,
+     [ctypes.POINTER(Noise), c_int, c_void_p, c_void_p, ctypes.c_uint64, c_void_p, c_void_p]),
+    ("gocean_noise_soar", c_int,
+     [ctypes.POINTER(Noise), c_float, c_float, c_void_p, c_void_p, c_int, c_void_p]),
+    ("gocean_noise_balance", c_int,
+     [ctypes.POINTER(Noise), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      ctypes.POINTER(Fields), c_int, c_int, c_void_p]),
+    ("gocean_noise_norms", c_int, [ctypes.POINTER(Noise), c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gocean_noise_make_perpendicular", c_int,
+     [ctypes.POINTER(Noise), c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gocean_noise_batch_create", c_int,
+     [ctypes.POINTER(NoiseMember), c_int, c_int, ctypes.POINTER(c_void_p)]),
+    ("gocean_noise_batch_destroy", c_int, [c_void_p]),
+    ("gocean_noise_batch_layout", c_int,
+     [c_void_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    ("gocean_noise_batch_generate", c_int,
+     [c_void_p, c_void_p, c_void_p, ctypes.c_uint64, c_void_p, c_void_p]),
+    ("gocean_noise_batch_perturb", c_int,
+     [c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_int,
+      c_void_p, c_void_p, c_void_p, ctypes.c_uint64, c_void_p, c_void_p]),
+])
+
+
 
 _lib = None
 _lock = threading.Lock()

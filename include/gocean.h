@@ -274,6 +274,111 @@ int gocean_adjoint_segment(const gocean_domain_t* d, const gocean_wind_t* wind, 
                            float* o_eta, float* o_hu, float* o_hv, const float* Hi,
                            float* work, long long work_floats, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Model error: SOAR-correlated eta perturbation with geostrophically
+ * balanced (hu, hv), added to the state (OceanStateNoise,
+ * gpu_ocean/SWESimulators/OceanStateNoise.py:399-496; kernels
+ * gpu_kernels/ocean_noise.cu and gpu_kernels/reductions.cu).
+ *
+ * Buffers (device, row-major, no padding):
+ *   seed    seed_ny x seed_nx uint64 (LCG generator state)
+ *   random  rand_ny x rand_nx float  (xi; nu for the perpendicular field)
+ *   coarse  (coarse_ny+4) x (coarse_nx+4) float
+ * eta/hu/hv are (ny+4) x pitch, Hi (ny+5) x hi_pitch, as everywhere else;
+ * only interior cells of eta/hu/hv are written (the reference's kernels
+ * leave ghost cells to the boundary-condition launch that follows).
+ * ------------------------------------------------------------------------ */
+typedef struct gocean_noise {
+    int32_t nx, ny;                 /* interior cells of the simulator grid */
+    int32_t pitch, hi_pitch;        /* floats per row of eta/hu/hv (and fields), of Hi */
+    float dx, dy;
+    int32_t interpolation_factor;   /* positive odd, divides nx and ny */
+    int32_t periodic_ns, periodic_ew;
+    float soar_q0, soar_L;          /* SOAR amplitude and length scale */
+    float g, f, beta;               /* coriolis = f + beta ((i+1/2)dx n_x + (j+1/2)dy n_y),
+                                       ghosted (i, j); f == 0 reads fields->f */
+    float north_x, north_y;         /* north vector when fields->north_* are NULL */
+    float land_value;               /* Hi value marking land (1e20) */
+} gocean_noise_t;
+
+#define GOCEAN_RNG_LCG_NORMAL 0     /* normalDistribution, ocean_noise.cu:163 */
+#define GOCEAN_RNG_LCG_UNIFORM 1    /* uniformDistribution, ocean_noise.cu:121 */
+#define GOCEAN_RNG_PHILOX_NORMAL 2  /* in place of XORWOW fill_normal, OceanStateNoise.py:324 */
+#define GOCEAN_RNG_PHILOX_UNIFORM 3 /* in place of XORWOW fill_uniform, OceanStateNoise.py:346 */
+
+/* sizes = coarse_nx, coarse_ny, rand_nx, rand_ny, seed_nx, seed_ny,
+ * rand_ghost_x, rand_ghost_y (OceanStateNoise.py:97-135). */
+int gocean_noise_sizes(const gocean_noise_t* nz, int32_t sizes[8]);
+
+/* Fill `random` with N(0,1) or U(0,1) numbers (generateNormalDistribution /
+ * generateUniformDistribution, OceanStateNoise.py:320-352).  LCG kinds
+ * advance `seed` twice per pair, bit for bit as ocean_noise.cu:80-195
+ * (including its column rule: with odd rand_nx the last column is not
+ * written).  Philox kinds (Philox4x32-10, counter = (element/4, *counter,
+ * member 0), key) read the device counter and advance it by one. */
+int gocean_noise_generate(const gocean_noise_t* nz, int kind, uint64_t* seed, float* random,
+                          uint64_t key, uint64_t* counter, void* stream);
+
+/* coarse = [coarse +] scale * q0 * SOAR(random): the SOAR kernel,
+ * ocean_noise.cu:264-370 (additive != 0 adds, as for the perpendicular
+ * field, OceanStateNoise.py:449-458).  q0 = soar_q0 * q0_scale. */
+int gocean_noise_soar(const gocean_noise_t* nz, float q0, float scale, const float* random,
+                      float* coarse, int additive, void* stream);
+
+/* Add d_eta and the balanced (d_hu, d_hv) to the interior of eta, hu, hv:
+ * geostrophicBalance (ocean_noise.cu:379-505) for interpolation_factor 1,
+ * bicubicInterpolation (ocean_noise.cu:522-772, alignment offsets from
+ * OceanStateNoise._obtain_coarse_grid_offset) for odd factors > 1.
+ * fields may be NULL (constant f and north). */
+int gocean_noise_balance(const gocean_noise_t* nz, const float* coarse, float* eta, float* hu,
+                         float* hv, const float* Hi, const gocean_fields_t* fields, int offset_i,
+                         int offset_j, void* stream);
+
+/* out = [sum xi^2, sum nu^2, sum xi nu] over the random field (squareSum /
+ * squareSumDouble, reductions.cu:27-185); nu == NULL writes out[0] only. */
+int gocean_noise_norms(const gocean_noise_t* nz, const float* xi, const float* nu, float* out,
+                       void* stream);
+
+/* nu <- sqrt(|nu|^2/|nu_perp|^2) (nu - (xi.nu/|xi|^2) xi) with `norms` from
+ * gocean_noise_norms (makePerpendicular, ocean_noise.cu:206-243). */
+int gocean_noise_make_perpendicular(const gocean_noise_t* nz, const float* xi, float* nu,
+                                    const float* norms, void* stream);
+
+/* Batched model error: every member of an ensemble perturbed by one launch
+ * per kernel (the reference calls perturbSim member by member,
+ * CDKLM16.py:581-582 inside OceanModelEnsemble.modelStep).  Member m's
+ * state lives in one flat buffer as for gocean_ensemble_*: eta at
+ * state_offset, hu at +plane, hv at +2*plane, plane = (ny+4)*pitch; Hi at
+ * hi_offset; fields (f, north_x, north_y planes) at field_offset or -1.
+ * The batch lays out flat seed / random / coarse buffers (offsets from
+ * gocean_noise_batch_layout, in elements).  Calls on one batch must not
+ * overlap. */
+typedef struct gocean_noise_member {
+    gocean_noise_t noise;
+    int64_t state_offset;
+    int64_t hi_offset;
+    int64_t field_offset;
+} gocean_noise_member_t;
+
+typedef struct gocean_noise_batch gocean_noise_batch_t;
+
+int gocean_noise_batch_create(const gocean_noise_member_t* members, int n, int kind,
+                              gocean_noise_batch_t** out);
+int gocean_noise_batch_destroy(gocean_noise_batch_t* batch);
+/* offsets[3m+0..2] = member m's seed, random and coarse offsets; totals the
+ * buffer lengths (either may be NULL). */
+int gocean_noise_batch_layout(const gocean_noise_batch_t* batch, int64_t* offsets, int64_t totals[3]);
+/* New random fields for every member (Philox: counter (element/4, *counter,
+ * member index)). */
+int gocean_noise_batch_generate(gocean_noise_batch_t* batch, uint64_t* seed, float* random,
+                                uint64_t key, uint64_t* counter, void* stream);
+/* perturbOceanState for every member: [generate,] SOAR with amplitude
+ * soar_q0 * q0_scale, balance with zero alignment offsets. */
+int gocean_noise_batch_perturb(gocean_noise_batch_t* batch, float* state, const float* Hi,
+                               const float* fields, float q0_scale, float perturbation_scale,
+                               int update_random, uint64_t* seed, float* random, float* coarse,
+                               uint64_t key, uint64_t* counter, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -150,3 +150,82 @@ def test_adjoint_segment_arguments_rejected():
     rc = lib.gocean_adjoint_segment(_dom(rk_order=3), None, 2, 1.0, None, 16, 16, 16, 32, 32, 32, 48, 48,
                                     48, 64, 128, 1 << 40, None)
     assert rc == -1 and "rk_order 2" in lib.gocean_last_error().decode()
+
+
+# ---------------------------------------------------------------------------
+# model error
+# ---------------------------------------------------------------------------
+def _noise(**kw):
+    d = G.Noise()
+    vals = dict(nx=30, ny=40, pitch=34, hi_pitch=35, dx=7.0, dy=7.0, interpolation_factor=1,
+                periodic_ns=1, periodic_ew=1, soar_q0=7e-5, soar_L=5.25, g=9.81, f=0.02, beta=0.0,
+                north_x=0.0, north_y=1.0, land_value=1e20)
+    vals.update(kw)
+    for k, v in vals.items():
+        setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize("kw,want", [
+    (dict(), (30, 40, 30, 40, 15, 40, 0, 0)),
+    (dict(periodic_ns=0, periodic_ew=0), (30, 40, 38, 48, 19, 48, 4, 4)),
+    (dict(periodic_ew=0), (30, 40, 38, 40, 19, 40, 4, 0)),
+    (dict(nx=45, ny=30, interpolation_factor=5, periodic_ns=0), (9, 6, 9, 14, 5, 14, 0, 4)),
+])
+def test_noise_sizes(kw, want):
+    """OceanStateNoise.py:97-135 (and the odd rand_nx of a 9-wide coarse grid)."""
+    lib = G.load()
+    out = (ctypes.c_int32 * 8)()
+    assert lib.gocean_noise_sizes(_noise(**kw), out) == 0
+    assert tuple(out) == want
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(interpolation_factor=2), "odd"),
+    (dict(interpolation_factor=7), "divisible"),
+    (dict(nx=0), "too small"),
+])
+def test_noise_invalid_descriptor(kw, msg):
+    lib = G.load()
+    out = (ctypes.c_int32 * 8)()
+    assert lib.gocean_noise_sizes(_noise(**kw), out) == -1
+    assert msg in lib.gocean_last_error().decode()
+
+
+def test_noise_arguments_rejected():
+    lib = G.load()
+    d = _noise()
+    assert lib.gocean_noise_generate(d, 7, 16, 16, 0, 16, None) == -1
+    assert "kind" in lib.gocean_last_error().decode()
+    assert lib.gocean_noise_generate(d, G.RNG_LCG_NORMAL, None, 16, 0, None, None) == -1
+    assert "seed" in lib.gocean_last_error().decode()
+    assert lib.gocean_noise_generate(d, G.RNG_PHILOX_NORMAL, None, 16, 0, None, None) == -1
+    assert "counter" in lib.gocean_last_error().decode()
+    assert lib.gocean_noise_soar(_noise(soar_L=0.0), 1.0, 1.0, 16, 16, 0, None) == -1
+    assert "soar_L" in lib.gocean_last_error().decode()
+    assert lib.gocean_noise_balance(_noise(f=0.0), 16, 16, 16, 16, 16, None, 0, 0, None) == -1
+    assert "Coriolis field" in lib.gocean_last_error().decode()
+    assert lib.gocean_noise_balance(_noise(pitch=20), 16, 16, 16, 16, 16, None, 0, 0, None) == -1
+    assert "pitch" in lib.gocean_last_error().decode()
+    d5 = _noise(nx=30, ny=40, interpolation_factor=5)
+    assert lib.gocean_noise_balance(d5, 16, 16, 16, 16, 16, None, 3, 0, None) == -1
+    assert "offsets" in lib.gocean_last_error().decode()
+    f = G.Fields()
+    f.north_x = 16
+    assert lib.gocean_noise_balance(d, 16, 16, 16, 16, 16, f, 0, 0, None) == -1
+    assert "together" in lib.gocean_last_error().decode()
+
+
+def test_noise_batch_rejects_bad_member():
+    lib = G.load()
+    arr = (G.NoiseMember * 2)()
+    arr[0].noise = _noise()
+    arr[1].noise = _noise(interpolation_factor=4)
+    h = ctypes.c_void_p()
+    assert lib.gocean_noise_batch_create(arr, 2, G.RNG_LCG_NORMAL, ctypes.byref(h)) == -1
+    assert "member 1" in lib.gocean_last_error().decode()
+    arr[1].noise = _noise(f=0.0)
+    arr[1].field_offset = -1
+    assert lib.gocean_noise_batch_create(arr, 2, G.RNG_LCG_NORMAL, ctypes.byref(h)) == -1
+    assert "Coriolis" in lib.gocean_last_error().decode()
+    assert lib.gocean_noise_batch_create(arr, 2, 9, ctypes.byref(h)) == -1
