@@ -312,6 +312,109 @@ def bench_c5(ctx, steps=200, K=25, n_obs=16, seed=55, plain_members=4):
             "J": float(J), "grad_norm_per_domain": gnorm}
 
 
+def _event_time(st, fn, reps):
+    """Average seconds per call of fn() on stream st (HIP events)."""
+    import torch
+    fn()
+    st.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    st.synchronize()
+    return e0.elapsed_time(e1) / 1000.0 / reps
+
+
+def bench_model_error(ctx, inp, reps=50, ens_members=100, ens_n=512, ens_steps=20):
+    """Model error (OceanStateNoise.perturbSim) on the headline domain: each
+    kernel timed with HIP events against its algorithmic bytes, the step with
+    and without per-step model error, and the batched ensemble step with and
+    without it.  Algorithmic bytes: LCG generator 12 B per number (seed read
+    and write 16 B per pair, two 4 B numbers); Philox 4 B per number; SOAR 8 B
+    per coarse cell (read xi, write d_eta); balance 32 B per cell at factor 1
+    (d_eta, Hi, eta/hu/hv read, eta/hu/hv written), 28 B + 4/factor^2 B at
+    factor > 1."""
+    import torch
+    from SWESimulators import BathymetryAndICs as B, CDKLM16Ensemble, OceanStateNoise
+    from SWESimulators import _gocean as G
+    out = {}
+    lib = G.load()
+    inp3 = B.config2(nx=inp.nx - inp.nx % 3, ny=inp.ny - inp.ny % 3)    # sides divisible by 3
+    for factor in (1, 3):
+        fin = inp if factor == 1 else inp3
+        sim = make_sim(fin, ctx)
+        st = sim.gpu_stream
+        cells = fin.nx * fin.ny
+        for use_lcg in (True, False):
+            nz = OceanStateNoise.OceanStateNoise.fromsim(sim, soar_q0=1e-4, interpolation_factor=factor,
+                                                         use_lcg=use_lcg)
+            sh = G.stream_handle(st)
+            rnum = int(nz.rand_nx) * int(nz.rand_ny)
+            ccells = (int(nz.coarse_nx) + 4) * (int(nz.coarse_ny) + 4)
+            if use_lcg:
+                gen = lambda: G.check(lib.gocean_noise_generate(
+                    nz.desc, G.RNG_LCG_NORMAL, nz.seed.data_ptr(), nz.random_numbers.data.gpudata, 0, None, sh))
+            else:
+                gen = lambda: G.check(lib.gocean_noise_generate(
+                    nz.desc, G.RNG_PHILOX_NORMAL, None, nz.random_numbers.data.gpudata, nz.rng.key,
+                    nz.rng.counter.data_ptr(), sh))
+            t_gen = _event_time(st, gen, reps)
+            nz.perturbSim(sim)        # sets the state pitches of the descriptor
+            t_soar = _event_time(st, lambda: G.check(lib.gocean_noise_soar(
+                nz.desc, float(nz.soar_q0), 1.0, nz.random_numbers.data.gpudata,
+                nz.coarse_buffer.data.gpudata, 0, sh)), reps)
+            gd = sim.gpu_data
+            t_bal = _event_time(st, lambda: G.check(lib.gocean_noise_balance(
+                nz.desc, nz.coarse_buffer.data.gpudata, gd.h0.data.gpudata, gd.hu0.data.gpudata,
+                gd.hv0.data.gpudata, sim.bathymetry.Bi.data.gpudata, nz.fields, 0, 0, sh)), reps)
+            t_all = _event_time(st, lambda: nz.perturbSim(sim), reps)
+            b_gen = (12.0 if use_lcg else 4.0) * rnum
+            b_soar = 8.0 * ccells
+            b_bal = (32.0 if factor == 1 else 28.0 + 4.0 / factor ** 2) * cells
+            key = f"factor{factor}_{'lcg' if use_lcg else 'philox'}"
+            out[key] = {
+                "generate_us": round(t_gen * 1e6, 1), "generate_GBs": round(b_gen / t_gen / 1e9, 1),
+                "soar_us": round(t_soar * 1e6, 1), "soar_GBs": round(b_soar / t_soar / 1e9, 1),
+                "balance_us": round(t_bal * 1e6, 1), "balance_GBs": round(b_bal / t_bal / 1e9, 1),
+                "perturb_us": round(t_all * 1e6, 1),
+                "perturb_GBs_algorithmic": round((b_gen + b_soar + b_bal) / t_all / 1e9, 1),
+                "domain": f"{fin.nx}x{fin.ny}"}
+            nz.cleanUp()
+        if factor == 3:
+            sim.cleanUp()
+    sim = make_sim(inp, ctx)
+    st = sim.gpu_stream
+    # RK2 step with and without per-step model error (factor 1, Philox)
+    t_step = _event_time(st, lambda: sim.step_n(1), reps)
+    nz = OceanStateNoise.OceanStateNoise.fromsim(sim, soar_q0=1e-4)
+    t_step_me = _event_time(st, lambda: (sim.step_n(1), nz.perturbSim(sim), sim.applyBoundaryConditions()), reps)
+    out["step_us"] = round(t_step * 1e6, 1)
+    out["step_with_model_error_us"] = round(t_step_me * 1e6, 1)
+    out["model_error_overhead"] = round(t_step_me / t_step - 1.0, 3)
+    sim.cleanUp()
+    # batched ensemble: step with and without model error
+    inps = B.config3(members=ens_members, n=ens_n)
+    ecells = sum(i.nx * i.ny for i in inps)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, inps, courant_number=0.7)
+    ens.enable_model_error(soar_q0=1e-4, seed=1)
+    est = ens.gpu_stream
+    t_e = _event_time(est, lambda: ens.step_n(ens_steps), 3) / ens_steps
+    t_eme = _event_time(est, lambda: ens.step_n(ens_steps, model_error=True), 3) / ens_steps
+    ens.capture(ens_steps, model_error=True)
+    t_g = _event_time(est, ens.replay, 3) / ens_steps
+    out["ensemble"] = {
+        "workload": f"{ens_members} members x {ens_n}x{ens_n}, walls, Philox, factor 1",
+        "step_mcell_steps_s": round(ecells / t_e / 1e6, 1),
+        "step_with_model_error_mcell_steps_s": round(ecells / t_eme / 1e6, 1),
+        "step_with_model_error_graph_mcell_steps_s": round(ecells / t_g / 1e6, 1),
+        "model_error_overhead": round(min(t_eme, t_g) / t_e - 1.0, 3)}
+    ens.cleanUp()
+    out["workload"] = (f"{inp.nx}x{inp.ny} headline domain, q0 1e-4, cutoff 2, "
+                       f"interpolation factor 1 and 3, LCG and Philox generators")
+    return out
+
+
 def measured_traffic(cells):
     """HBM bytes per substep launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy), when
@@ -337,6 +440,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="skip the batched uneven ensemble adjoint")
+    ap.add_argument("--no-model-error", action="store_true", help="skip the model-error leg")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -393,6 +497,11 @@ def main():
             log("[bench] batched uneven ensemble with adjoint C5")
             extra["ensemble_adjoint"] = bench_c5(ctx)
             log(f"[bench] ensemble adjoint: {extra['ensemble_adjoint']}")
+
+    if not a.no_model_error:
+        log("[bench] model error (OceanStateNoise) on the headline domain and the ensemble")
+        extra["model_error"] = bench_model_error(ctx, inp)
+        log(f"[bench] model error: {extra['model_error']}")
 
     cpu = None
     if not a.no_cpu:

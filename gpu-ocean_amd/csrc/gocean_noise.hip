@@ -34,7 +34,7 @@ namespace {
 constexpr int kCut = 2;                    // config.soar_cutoff
 constexpr int kRngNT = 256;
 constexpr int kSoarBW = 32, kSoarBH = 8;   // SOAR tile over the coarse buffer
-constexpr int kBalBW = 32, kBalBH = 8;     // balance tile over fine interior cells
+constexpr int kBalBW = 64, kBalBH = 4;     // balance tile over fine interior cells
 constexpr int kNormNT = 1024;
 
 struct NoiseMember {
@@ -67,13 +67,22 @@ struct NoiseLaunch {
     float* coarse;
 };
 
-__device__ __forceinline__ const NoiseMember& member_of(const NoiseLaunch& L, int b) {
-    return L.members ? L.members[L.map[b]] : L.one;
+// The member a block works on.  Single domain: the table of one in the
+// kernel arguments; batch: the device table through the block -> member map.
+// Kept as two instantiations so that every field read is a scalar load from
+// a known address space.
+template <bool BATCH>
+__device__ __forceinline__ const NoiseMember& member_of(const NoiseLaunch& L) {
+    if constexpr (BATCH) return L.members[L.map[blockIdx.x]];
+    else return L.one;
 }
 
+// a mod n for a within a few periods of [0, n): conditional wraps instead of
+// an integer division
 __device__ __forceinline__ int pmod(int a, int n) {
-    const int r = a % n;
-    return r < 0 ? r + n : r;
+    while (a < 0) a += n;
+    while (a >= n) a -= n;
+    return a;
 }
 
 // ---------------------------------------------------------------------------
@@ -123,10 +132,11 @@ __device__ __forceinline__ float philox_unit(unsigned x) {
 // kind: GOCEAN_RNG_*.  LCG: one thread per seed, two numbers each, the
 // column rule of ocean_noise.cu:186-192.  Philox: one thread per four
 // consecutive numbers of the member's row-major field.
+template <bool BATCH>
 __global__ __launch_bounds__(kRngNT) void k_noise_rng(NoiseLaunch L, int kind,
                                                       const unsigned long long* counter,
                                                       unsigned long long key) {
-    const NoiseMember& M = member_of(L, blockIdx.x);
+    const NoiseMember& M = member_of<BATCH>(L);
     const int t = (blockIdx.x - M.rng_begin) * kRngNT + threadIdx.x;
     float* rnd = L.random + M.rand_off;
     if (kind <= GOCEAN_RNG_LCG_UNIFORM) {
@@ -173,11 +183,15 @@ __global__ void k_counter_bump(unsigned long long* counter) { *counter += 1ull; 
 // 2+cutoff ghost cells, and (a-2+o) mod n when periodic (ocean_noise.cu:
 // 305-327); the sum runs dy-outer, dx-inner as the reference's loop.
 // ---------------------------------------------------------------------------
+template <bool BATCH>
 __global__ __launch_bounds__(kSoarBW * kSoarBH) void k_noise_soar(NoiseLaunch L, float q0_scale,
                                                                   float scale, int additive) {
-    constexpr int W = kSoarBW + 2 * kCut, H = kSoarBH + 2 * kCut;
+    constexpr int W = kSoarBW + 2 * kCut, H = kSoarBH + 2 * kCut, NW = (2 * kCut + 1) * (2 * kCut + 1);
     __shared__ float xi[H][W + 1];
-    const NoiseMember& M = member_of(L, blockIdx.x);
+    const NoiseMember& M = member_of<BATCH>(L);
+    float w[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) w[k] = M.w[k];
     const int lt = blockIdx.x - M.soar_begin;
     const int by = (lt / M.soar_tiles_x) * kSoarBH;
     const int bx = (lt % M.soar_tiles_x) * kSoarBW;
@@ -197,7 +211,7 @@ __global__ __launch_bounds__(kSoarBW * kSoarBH) void k_noise_soar(NoiseLaunch L,
     for (int oy = 0; oy <= 2 * kCut; ++oy)
 #pragma unroll
         for (int ox = 0; ox <= 2 * kCut; ++ox)
-            acc = acc + M.w[oy * (2 * kCut + 1) + ox] * xi[ty + oy][tx + ox];
+            acc = acc + w[oy * (2 * kCut + 1) + ox] * xi[ty + oy][tx + ox];
     const float v = ((M.q0 * q0_scale) * acc) * scale;
     float* c = L.coarse + M.coarse_off + (long long)ay * (M.cnx + 4) + ax;
     *c = additive ? *c + v : v;
@@ -225,36 +239,38 @@ __device__ __forceinline__ float4 hermite(float x) {
                        x3 - x2);
 }
 
-// Bicubic value from the 4x4 coarse neighbourhood c[j][i] = coarse(cj-1+j,
-// ci-1+i) with central-difference slopes and cross slopes
-// (ocean_noise.cu:652-690): sum_ij hx_i F_ij hy_j.
-__device__ __forceinline__ float bicubic(const float c[4][4], float4 hx, float4 hy) {
-    const float f00 = c[1][1], f01 = c[2][1], f10 = c[1][2], f11 = c[2][2];
-    const float fx00 = (c[1][2] - c[1][0]) / 2.0f, fx01 = (c[2][2] - c[2][0]) / 2.0f;
-    const float fx10 = (c[1][3] - c[1][1]) / 2.0f, fx11 = (c[2][3] - c[2][1]) / 2.0f;
-    const float fy00 = (c[2][1] - c[0][1]) / 2.0f, fy01 = (c[3][1] - c[1][1]) / 2.0f;
-    const float fy10 = (c[2][2] - c[0][2]) / 2.0f, fy11 = (c[3][2] - c[1][2]) / 2.0f;
-    const float fy_10 = (c[2][0] - c[0][0]) / 2.0f, fy_11 = (c[3][0] - c[1][0]) / 2.0f;
-    const float fy20 = (c[2][3] - c[0][3]) / 2.0f, fy21 = (c[3][3] - c[1][3]) / 2.0f;
-    const float fxy00 = (fy10 - fy_10) / 2.0f, fxy01 = (fy11 - fy_11) / 2.0f;
-    const float fxy10 = (fy20 - fy00) / 2.0f, fxy11 = (fy21 - fy01) / 2.0f;
-    const float r0 = f00 * hy.x + f01 * hy.y + fy00 * hy.z + fy01 * hy.w;
-    const float r1 = f10 * hy.x + f11 * hy.y + fy10 * hy.z + fy11 * hy.w;
-    const float r2 = fx00 * hy.x + fx01 * hy.y + fxy00 * hy.z + fxy01 * hy.w;
-    const float r3 = fx10 * hy.x + fx11 * hy.y + fxy10 * hy.z + fxy11 * hy.w;
-    return hx.x * r0 + hx.y * r1 + hx.z * r2 + hx.w * r3;
+// The reference's bicubic surface (ocean_noise.cu:652-700: corner values,
+// central-difference slopes and cross slopes, B F B^T) is the tensor product
+// of the 1-D Hermite cubic with central-difference slopes, so along each
+// axis the four coarse values c(-1), c(0), c(1), c(2) enter with the weights
+// (-h10/2, h00 - h11/2, h01 + h10/2, h11/2) of the Hermite basis h.
+__device__ __forceinline__ float4 cr_weights(float r) {
+    const float4 h = hermite(r);
+    return make_float4(-0.5f * h.z, h.x - 0.5f * h.w, h.y + 0.5f * h.z, 0.5f * h.w);
 }
+
+// Coarse window per block, large enough for factors >= 3 (the window spans
+// floor((BW+1)/factor) + 5 columns); larger windows fall back to reads from
+// the coarse buffer.
+constexpr int kWinW = kBalBW / 3 + 8, kWinH = kBalBH / 3 + 8;
+// threads [0, BW+2) set up the columns, [kRowT, kRowT+BH+2) the rows
+constexpr int kRowT = ((kBalBW + 2 + 63) / 64) * 64;
+static_assert(kRowT + kBalBH + 2 <= kBalBW * kBalBH, "column/row set-up threads");
 
 // One thread per fine interior cell (ghosted ti = 2..nx+1, tj = 2..ny+1).
 // Factor 1: d_eta is the coarse buffer itself, h_mid includes the current
 // eta and a cell with a land corner gets eta = hu = hv = 0
 // (ocean_noise.cu:443-504).  Factor > 1: d_eta on the tile and one ring is
-// interpolated into LDS first; h_mid = H_mid + d_eta and a land cell gets no
-// perturbation (:607-770).
+// interpolated into LDS first, from a coarse window staged in LDS; h_mid =
+// H_mid + d_eta and a land cell gets no perturbation (:607-770).
+template <bool BATCH>
 __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_balance(NoiseLaunch L, int offset_i,
                                                                    int offset_j) {
     __shared__ float de[kBalBH + 2][kBalBW + 3];
-    const NoiseMember& M = member_of(L, blockIdx.x);
+    __shared__ float win[kWinH][kWinW + 1];
+    __shared__ float4 wxs[kBalBW + 2], wys[kBalBH + 2];
+    __shared__ int cxs[kBalBW + 2], cys[kBalBH + 2];
+    const NoiseMember& M = member_of<BATCH>(L);
     const int lt = blockIdx.x - M.bal_begin;
     const int by = 2 + (lt / M.bal_tiles_x) * kBalBH;
     const int bx = 2 + (lt % M.bal_tiles_x) * kBalBW;
@@ -262,25 +278,66 @@ __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_balance(NoiseLaunch L
     const int cp = M.cnx + 4;
     const bool interp = M.factor > 1;
     if (interp) {   // uniform per block
-        for (int k = threadIdx.x; k < (kBalBH + 2) * (kBalBW + 2); k += kBalBW * kBalBH) {
+        // coarse cells of the first and last fine column / row of the tile's ring
+        int c0, c1, r0, r1;
+        float tmp;
+        coarse_cell(bx - 1, offset_i, M.dx, M.cdx, M.cnx, c0, tmp);
+        coarse_cell(min(bx + kBalBW, M.nx + 2), offset_i, M.dx, M.cdx, M.cnx, c1, tmp);
+        coarse_cell(by - 1, offset_j, M.dy, M.cdy, M.cny, r0, tmp);
+        coarse_cell(min(by + kBalBH, M.ny + 2), offset_j, M.dy, M.cdy, M.cny, r1, tmp);
+        const int wx0 = c0 - 1, wy0 = r0 - 1;
+        const int ww = c1 - c0 + 4, wh = r1 - r0 + 4;
+        const bool staged = ww <= kWinW && wh <= kWinH;
+        const int t = threadIdx.x;
+        if (t < kBalBW + 2) {
+            int ci;
+            float rx;
+            coarse_cell(bx - 1 + t, offset_i, M.dx, M.cdx, M.cnx, ci, rx);
+            cxs[t] = ci;
+            wxs[t] = cr_weights(rx);
+        } else if (t >= kRowT && t < kRowT + kBalBH + 2) {
+            int cj;
+            float ry;
+            coarse_cell(by - 1 + (t - kRowT), offset_j, M.dy, M.cdy, M.cny, cj, ry);
+            cys[t - kRowT] = cj;
+            wys[t - kRowT] = cr_weights(ry);
+        }
+        if (staged)
+            for (int k = t; k < wh * ww; k += kBalBW * kBalBH) {
+                const int j = k / ww, i = k - j * ww;
+                const int gy = min(max(wy0 + j, 0), M.cny + 3);
+                const int gx = min(max(wx0 + i, 0), M.cnx + 3);
+                win[j][i] = C[(long long)gy * cp + gx];
+            }
+        __syncthreads();
+        for (int k = t; k < (kBalBH + 2) * (kBalBW + 2); k += kBalBW * kBalBH) {
             const int j = k / (kBalBW + 2), i = k - j * (kBalBW + 2);
-            const int t_i = bx + i - 1, t_j = by + j - 1;
-            if (t_i > M.nx + 2 || t_j > M.ny + 2) continue;   // beyond the ring of this domain
-            int ci, cj;
-            float rx, ry;
-            coarse_cell(t_i, offset_i, M.dx, M.cdx, M.cnx, ci, rx);
-            coarse_cell(t_j, offset_j, M.dy, M.cdy, M.cny, cj, ry);
-            float c[4][4];
+            if (bx + i - 1 > M.nx + 2 || by + j - 1 > M.ny + 2) continue;   // beyond this domain's ring
+            const float4 wx = wxs[i], wy = wys[j];
+            const int ci = cxs[i], cj = cys[j];
+            float v = 0.0f;
+            if (staged) {
+                const int lx = ci - 1 - wx0, ly = cj - 1 - wy0;
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const int gy = min(max(cj - 1 + jj, 0), M.cny + 3);
+                for (int b = 0; b < 4; ++b) {
+                    const float* row = &win[ly + b][lx];
+                    const float rv = wx.x * row[0] + wx.y * row[1] + wx.z * row[2] + wx.w * row[3];
+                    v = v + (b == 0 ? wy.x : b == 1 ? wy.y : b == 2 ? wy.z : wy.w) * rv;
+                }
+            } else {
 #pragma unroll
-                for (int ii = 0; ii < 4; ++ii) {
-                    const int gx = min(max(ci - 1 + ii, 0), M.cnx + 3);
-                    c[jj][ii] = C[(long long)gy * cp + gx];
+                for (int b = 0; b < 4; ++b) {
+                    const long long ro = (long long)min(max(cj - 1 + b, 0), M.cny + 3) * cp;
+                    float rv = 0.0f;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const float wa = a == 0 ? wx.x : a == 1 ? wx.y : a == 2 ? wx.z : wx.w;
+                        rv = rv + wa * C[ro + min(max(ci - 1 + a, 0), M.cnx + 3)];
+                    }
+                    v = v + (b == 0 ? wy.x : b == 1 ? wy.y : b == 2 ? wy.z : wy.w) * rv;
                 }
             }
-            de[j][i] = bicubic(c, hermite(rx), hermite(ry));
+            de[j][i] = v;
         }
         __syncthreads();
     }
@@ -346,12 +403,12 @@ __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_balance(NoiseLaunch L
 // ---------------------------------------------------------------------------
 // norms and the perpendicular pair
 // ---------------------------------------------------------------------------
-// One block per member: out[3m..3m+2] = sum xi^2, sum nu^2, sum xi nu (nu may
-// be NULL: only the first).  Compensated per-thread sums, fixed-order tree.
+// out[0..2] = sum xi^2, sum nu^2, sum xi nu (nu may be NULL: only the
+// first), one block.  Compensated per-thread sums, fixed-order tree.
 __global__ __launch_bounds__(kNormNT) void k_noise_norms(NoiseLaunch L, const float* xi_base,
                                                          const float* nu_base, float* out) {
     __shared__ float red[3][kNormNT];
-    const NoiseMember& M = L.members ? L.members[blockIdx.x] : L.one;
+    const NoiseMember& M = L.one;
     const long long n = (long long)M.rand_nx * M.rand_ny;
     const float* xi = xi_base + M.rand_off;
     const float* nu = nu_base ? nu_base + M.rand_off : nullptr;
@@ -378,10 +435,10 @@ __global__ __launch_bounds__(kNormNT) void k_noise_norms(NoiseLaunch L, const fl
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[3 * blockIdx.x] = red[0][0];
+        out[0] = red[0][0];
         if (nu) {
-            out[3 * blockIdx.x + 1] = red[1][0];
-            out[3 * blockIdx.x + 2] = red[2][0];
+            out[1] = red[1][0];
+            out[2] = red[2][0];
         }
     }
 }
@@ -390,10 +447,10 @@ __global__ __launch_bounds__(kNormNT) void k_noise_norms(NoiseLaunch L, const fl
 // element of the member's field (ocean_noise.cu:206-243).
 __global__ __launch_bounds__(kRngNT) void k_noise_perp(NoiseLaunch L, const float* xi_base,
                                                        float* nu_base, const float* norms) {
-    const NoiseMember& M = member_of(L, blockIdx.x);
-    const long long k = (long long)(blockIdx.x - M.rng_begin) * kRngNT + threadIdx.x;
+    const NoiseMember& M = L.one;
+    const long long k = (long long)blockIdx.x * kRngNT + threadIdx.x;
     if (k >= (long long)M.rand_nx * M.rand_ny) return;
-    const float* r = norms + 3 * (L.members ? L.map[blockIdx.x] : 0);
+    const float* r = norms;
     const float par = r[2] / r[0];
     const float perp = r[1] - par * r[2];
     float* nu = nu_base + M.rand_off + k;
@@ -539,7 +596,7 @@ int gocean_noise_generate(const gocean_noise_t* nz, int kind, uint64_t* seed, fl
     L.seed = (unsigned long long*)seed;
     L.random = random;
     const hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_noise_rng, dim3(rng_blocks(L.one, kind)), dim3(kRngNT), 0, s, L, kind,
+    hipLaunchKernelGGL(k_noise_rng<false>, dim3(rng_blocks(L.one, kind)), dim3(kRngNT), 0, s, L, kind,
                        (const unsigned long long*)counter, (unsigned long long)key);
     if (!lcg) hipLaunchKernelGGL(k_counter_bump, dim3(1), dim3(1), 0, s, (unsigned long long*)counter);
     return check_launch("k_noise_rng");
@@ -559,7 +616,7 @@ int gocean_noise_soar(const gocean_noise_t* nz, float q0, float scale, const flo
     L.one.soar_tiles_x = tx;
     L.random = (float*)random;
     L.coarse = coarse;
-    hipLaunchKernelGGL(k_noise_soar, dim3(tx * ty), dim3(kSoarBW * kSoarBH), 0, (hipStream_t)stream, L,
+    hipLaunchKernelGGL(k_noise_soar<false>, dim3(tx * ty), dim3(kSoarBW * kSoarBH), 0, (hipStream_t)stream, L,
                        1.0f, scale, additive);
     return check_launch("k_noise_soar");
 }
@@ -599,7 +656,7 @@ int gocean_noise_balance(const gocean_noise_t* nz, const float* coarse, float* e
         L.fny = fields->north_y;
         L.one.nxf_off = L.one.nyf_off = 0;
     }
-    hipLaunchKernelGGL(k_noise_balance, dim3(tx * ty), dim3(kBalBW * kBalBH), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_noise_balance<false>, dim3(tx * ty), dim3(kBalBW * kBalBH), 0, (hipStream_t)stream,
                        L, offset_i, offset_j);
     return check_launch("k_noise_balance");
 }
@@ -763,7 +820,7 @@ int gocean_noise_batch_generate(gocean_noise_batch_t* B, uint64_t* seed, float* 
     L.seed = (unsigned long long*)seed;
     L.random = random;
     const hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_noise_rng, dim3(B->n_rng), dim3(kRngNT), 0, s, L, B->kind,
+    hipLaunchKernelGGL(k_noise_rng<true>, dim3(B->n_rng), dim3(kRngNT), 0, s, L, B->kind,
                        (const unsigned long long*)counter, (unsigned long long)key);
     if (!lcg) hipLaunchKernelGGL(k_counter_bump, dim3(1), dim3(1), 0, s, (unsigned long long*)counter);
     return check_launch("k_noise_rng (batch)");
@@ -783,12 +840,12 @@ int gocean_noise_batch_perturb(gocean_noise_batch_t* B, float* state, const floa
     L.random = random;
     L.coarse = coarse;
     L.map = B->d_soar_map;
-    hipLaunchKernelGGL(k_noise_soar, dim3(B->n_soar), dim3(kSoarBW * kSoarBH), 0, s, L, q0_scale,
+    hipLaunchKernelGGL(k_noise_soar<true>, dim3(B->n_soar), dim3(kSoarBW * kSoarBH), 0, s, L, q0_scale,
                        perturbation_scale, 0);
     L.map = B->d_bal_map;
     L.eta = L.hu = L.hv = state;
     L.Hi = Hi;
     L.ff = L.fnx = L.fny = fields;
-    hipLaunchKernelGGL(k_noise_balance, dim3(B->n_bal), dim3(kBalBW * kBalBH), 0, s, L, 0, 0);
+    hipLaunchKernelGGL(k_noise_balance<true>, dim3(B->n_bal), dim3(kBalBW * kBalBH), 0, s, L, 0, 0);
     return check_launch("model error (batch)");
 }
