@@ -113,6 +113,7 @@ class CDKLM16Ensemble(object):
         self.t = np.zeros(self.n)
         self.num_iterations = 0
         self.noise = None
+        self.drifters = None
         self.f = [float(np.float32(_get(m, "f", 0.0))) for m in members]
         self.g = [float(np.float32(_get(m, "g", 9.81))) for m in members]
         self.dxy = [(float(np.float32(_get(m, "dx"))), float(np.float32(_get(m, "dy")))) for m in members]
@@ -175,12 +176,15 @@ class CDKLM16Ensemble(object):
         ``model_error`` each step is followed by the batched perturbation and
         the ghost-cell exchange (CDKLM16.step with a small-scale
         perturbation, CDKLM16.py:580-586)."""
-        if not model_error:
+        if not model_error and self.drifters is None:
             self._rk(n_steps)
         else:
             for _ in range(int(n_steps)):
                 self._rk(1)
-                self.perturb()
+                if model_error:
+                    self.perturb()
+                if self.drifters is not None:
+                    self.drift()
         self.num_iterations += int(n_steps)
 
     # ------------------------------------------------------------------
@@ -248,6 +252,65 @@ class CDKLM16Ensemble(object):
         G.check(self.lib.gocean_ensemble_periodic_bc(self.handle, self.state[0].data_ptr(), st),
                 "ensemble periodic bc")
 
+    # ------------------------------------------------------------------
+    # drifters (GPUDrifterCollection per member in batched launches)
+    # ------------------------------------------------------------------
+    def enable_drifters(self, num_drifters, positions=None, sensitivity=1.0, wind_drift_factor=0.0):
+        """``num_drifters`` drifters (plus the observation slot) per member,
+        moved after every step by one batched launch (drifterStep,
+        CDKLM16.py:600-610, for every member).  positions: (n, num+1, 2)
+        metres; default: the centres of a near-square grid of sub-domains."""
+        torch = Common._torch()
+        num = int(num_drifters)
+        arr = (G.DriftMember * self.n)()
+        for k in range(self.n):
+            arr[k].domain = self.members_c[k].domain
+            arr[k].state_offset = int(self.state_off[k])
+            arr[k].hm_offset = int(self.hm_off[k])
+            arr[k].positions_offset = k * (num + 1) * 2
+            arr[k].observations_offset = k * num * 2
+            arr[k].num_drifters = num
+            arr[k].sensitivity = float(np.float32(sensitivity))
+            arr[k].wind_drift_factor = float(np.float32(wind_drift_factor))
+            arr[k].wind = self.members_c[k].wind
+        h = ctypes.c_void_p()
+        G.check(self.lib.gocean_drift_batch_create(arr, self.n, ctypes.byref(h)), "gocean_drift_batch_create")
+        self.drifters = h
+        self.drift_members = arr
+        self.num_drifters = num
+        if positions is None:
+            positions = np.zeros((self.n, num + 1, 2))
+            rows = max(1, int(np.round(np.sqrt(num))))
+            cols = int(np.ceil(num / rows)) if num else 1
+            kk = np.arange(num)
+            for k in range(self.n):
+                dx, dy = self.dxy[k]
+                positions[k, :num, 0] = (kk % cols + 0.5) * self.nx[k] * dx / cols
+                positions[k, :num, 1] = (kk // cols + 0.5) * self.ny[k] * dy / rows
+        dev = self.gpu_ctx.torch_device
+        with torch.cuda.stream(self.gpu_stream):
+            self.drifter_positions = torch.from_numpy(
+                np.ascontiguousarray(positions, dtype=np.float32).reshape(self.n, num + 1, 2)).to(dev)
+            self.drifter_observations = torch.zeros((self.n, max(num, 1), 2), dtype=torch.float32, device=dev)
+
+    def drift(self):
+        """One drift step of every member's drifters with its own device dt."""
+        G.check(self.lib.gocean_drift_batch_step(self.drifters, self.state[0].data_ptr(), self.Hm.data_ptr(),
+                                                 self.drifter_positions.data_ptr(), self.dt.data_ptr(),
+                                                 G.stream_handle(self.gpu_stream)), "gocean_drift_batch_step")
+
+    def observe(self, positions=None):
+        """(n, num, 2) device tensor of hu, hv observed by every member:
+        at ``positions`` ((num, 2) or (num+1, 2) device tensor, e.g. the true
+        state's drifters) when given, else at each member's own drifters
+        (observeUnderlyingFlow per particle, BaseOceanStateEnsemble.py:105-136)."""
+        shared = positions is not None
+        pos = positions if shared else self.drifter_positions
+        G.check(self.lib.gocean_observe_flow_batch(self.drifters, self.state[0].data_ptr(), pos.data_ptr(),
+                                                   int(shared), self.drifter_observations.data_ptr(),
+                                                   G.stream_handle(self.gpu_stream)), "gocean_observe_flow_batch")
+        return self.drifter_observations
+
     def noise_views(self, k):
         """(random field, coarse buffer) device views of member k."""
         d = self.noise_members[k].noise
@@ -264,10 +327,10 @@ class CDKLM16Ensemble(object):
         """Capture ``n_steps`` RK steps of all members in a HIP graph (with
         ``model_error``, each step followed by the batched perturbation; the
         generator state lives on the device, so every replay draws new
-        numbers)."""
+        numbers; with drifters enabled, each step also moves them)."""
         self.gpu_stream.synchronize()
         self.graph_steps = int(n_steps)
-        if not model_error:
+        if not model_error and self.drifters is None:
             G.check(self.lib.gocean_ensemble_capture(
                 self.handle, int(n_steps), self.state[0].data_ptr(), self.state[1].data_ptr(), self._ptr2(),
                 self.Hi.data_ptr(), None, self.dt.data_ptr()), "gocean_ensemble_capture")
@@ -278,7 +341,10 @@ class CDKLM16Ensemble(object):
         with torch.cuda.graph(g, stream=self.gpu_stream):
             for _ in range(int(n_steps)):
                 self._rk(1)
-                self.perturb()
+                if model_error:
+                    self.perturb()
+                if self.drifters is not None:
+                    self.drift()
         self.torch_graph = g
 
     def replay(self):
@@ -301,6 +367,9 @@ class CDKLM16Ensemble(object):
         if getattr(self, "noise", None) is not None:
             self.lib.gocean_noise_batch_destroy(self.noise)
             self.noise = None
+        if getattr(self, "drifters", None) is not None:
+            self.lib.gocean_drift_batch_destroy(self.drifters)
+            self.drifters = None
 
     def __del__(self):
         try:

@@ -166,6 +166,28 @@ PROTOTYPES.extend([
 ])
 
 
+class DriftMember(ctypes.Structure):
+    """gocean_drift_member_t"""
+    _fields_ = [("domain", Domain), ("state_offset", ctypes.c_int64), ("hm_offset", ctypes.c_int64),
+                ("positions_offset", ctypes.c_int64), ("observations_offset", ctypes.c_int64),
+                ("num_drifters", ctypes.c_int32), ("sensitivity", c_float),
+                ("wind_drift_factor", c_float), ("wind", Wind)]
+
+
+PROTOTYPES.extend([
+    ("gocean_drift", c_int,
+     [ctypes.POINTER(Domain), c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float,
+      ctypes.POINTER(Wind), c_float, c_int, c_int, c_void_p]),
+    ("gocean_drifters_enforce_bc", c_int, [c_float, c_float, c_int, c_int, c_int, c_void_p, c_void_p]),
+    ("gocean_observe_flow", c_int,
+     [ctypes.POINTER(Domain), c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    ("gocean_drift_batch_create", c_int, [ctypes.POINTER(DriftMember), c_int, ctypes.POINTER(c_void_p)]),
+    ("gocean_drift_batch_destroy", c_int, [c_void_p]),
+    ("gocean_drift_batch_step", c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("gocean_observe_flow_batch", c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+])
+
+
 
 _lib = None
 _lock = threading.Lock()
@@ -219,3 +241,4 @@ def stream_handle(stream) -> int:
     if isinstance(stream, int):
         return stream
     return stream.cuda_stream
+

@@ -379,6 +379,69 @@ int gocean_noise_batch_perturb(gocean_noise_batch_t* batch, float* state, const 
                                int update_random, uint64_t* seed, float* random, float* coarse,
                                uint64_t key, uint64_t* counter, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Drifters and the observation operator.
+ * positions: (num_drifters + 1) x 2 floats (x, y in metres from the
+ * south-west interior corner): the drifters, then the observation position
+ * (GPUDrifterCollection.driftersDevice, GPUDrifterCollection.py:61-64).
+ * x_zero / y_zero: the ghosted cell column / row holding x = 0 / y = 0 (2).
+ * ------------------------------------------------------------------------ */
+
+/* One drift step of every position, observation included
+ * (passiveDrifterKernel, gpu_kernels/driftKernels.cu:108-210, launched by
+ * GPUDrifterCollection.drift, GPUDrifterCollection.py:212-232): bilinear
+ * velocity hu/(Hm+eta), hv/(Hm+eta) with the reference's ceil-based cell
+ * index, plus wind_drift_factor times the uniform wind lerped by wind->t,
+ * times sensitivity * dt; periodic sides wrap.  wind may be NULL when
+ * wind_drift_factor is 0. */
+int gocean_drift(const gocean_domain_t* d, int num_drifters, float* positions, const float* eta,
+                 const float* hu, const float* hv, const float* Hm, float dt, float sensitivity,
+                 const gocean_wind_t* wind, float wind_drift_factor, int x_zero, int y_zero,
+                 void* stream);
+
+/* Periodic wrap of every position (enforceBoundaryConditions,
+ * driftKernels.cu:214-250). */
+int gocean_drifters_enforce_bc(float domain_size_x, float domain_size_y, int periodic_ns,
+                               int periodic_ew, int num_drifters, float* positions, void* stream);
+
+/* obs[k] = (hu, hv) of the cell holding position k, k < num_drifters
+ * (observeUnderlyingFlow, gpu_kernels/observationKernels.cu:29-82). */
+int gocean_observe_flow(const gocean_domain_t* d, int num_drifters, const float* positions,
+                        const float* hu, const float* hv, float* obs, int x_zero, int y_zero,
+                        void* stream);
+
+/* Batched drifters over the members of an ensemble (the reference moves and
+ * observes them particle by particle, BaseOceanStateEnsemble.py:105-136).
+ * Member m's state is in one flat buffer as for gocean_ensemble_* (eta at
+ * state_offset, hu at +plane, hv at +2*plane), Hm at hm_offset, its
+ * positions at positions_offset ((num+1) x 2 floats) and its observations at
+ * observations_offset (num x 2 floats); x_zero = y_zero = 2. */
+typedef struct gocean_drift_member {
+    gocean_domain_t domain;
+    int64_t state_offset;
+    int64_t hm_offset;
+    int64_t positions_offset;
+    int64_t observations_offset;
+    int32_t num_drifters;
+    float sensitivity;
+    float wind_drift_factor;
+    gocean_wind_t wind;
+} gocean_drift_member_t;
+
+typedef struct gocean_drift_batch gocean_drift_batch_t;
+
+int gocean_drift_batch_create(const gocean_drift_member_t* members, int n, gocean_drift_batch_t** out);
+int gocean_drift_batch_destroy(gocean_drift_batch_t* batch);
+/* One drift step of every member's positions; dt: one device float per member. */
+int gocean_drift_batch_step(gocean_drift_batch_t* batch, const float* state, const float* Hm,
+                            float* positions, const float* dt, void* stream);
+/* Every member observes hu, hv at its num_drifters positions into its
+ * observation block: shared != 0 reads one positions array for all members
+ * (the true state's drifters, as BaseOceanStateEnsemble.py:105-136), else
+ * each member's own block of the drift positions buffer. */
+int gocean_observe_flow_batch(gocean_drift_batch_t* batch, const float* state, const float* positions,
+                              int shared, float* observations, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
