@@ -17,7 +17,7 @@ from . import _gocean as G
 def _drift_bc(bc):
     """Wall/periodic codes for the drift kernel: periodic stays periodic,
     anything else is a closed side."""
-    return tuple(Common.PERIODIC if b == Common.PERIODIC else Common.WALL for b in bc.get())
+    return tuple(Common.PERIODIC if b == Common.PERIODIC else Common.WALL for b in bc.codes())
 
 
 class GPUDrifterCollection(BaseDrifterCollection.BaseDrifterCollection):
