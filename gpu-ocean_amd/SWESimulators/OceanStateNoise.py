@@ -240,14 +240,6 @@ class OceanStateNoise(object):
             self.generateNormalDistribution()
         soar_q0 = np.float32(self.soar_q0 * q0_scale)
         offset_i, offset_j = self._obtain_coarse_grid_offset(align_with_cell_i, align_with_cell_j)
-        G.check(self.lib.gocean_noise_soar(self.desc, float(soar_q0), float(np.float32(perturbation_scale)),
-                                           self.random_numbers.data.gpudata,
-                                           self.coarse_buffer.data.gpudata, 0, s), "gocean_noise_soar")
-        if perpendicular_scale > 0:
-            G.check(self.lib.gocean_noise_soar(self.desc, float(soar_q0),
-                                               float(np.float32(perpendicular_scale)),
-                                               self.perpendicular_random_numbers.data.gpudata,
-                                               self.coarse_buffer.data.gpudata, 1, s), "gocean_noise_soar")
         d = self.desc
         d.pitch = _pitch(eta)
         d.hi_pitch = _pitch(H)
@@ -256,6 +248,19 @@ class OceanStateNoise(object):
         d.beta = float(np.float32(beta))
         d.north_x, d.north_y = self.north_const
         d.land_value = float(np.float32(land_mask_value))
+        scale = float(np.float32(perturbation_scale))
+        if self.interpolation_factor == 1 and not perpendicular_scale > 0:
+            # SOAR and geostrophicBalance in one launch
+            G.check(self.lib.gocean_noise_soar_balance(
+                d, float(soar_q0), scale, self.random_numbers.data.gpudata, self.coarse_buffer.data.gpudata,
+                _addr(eta), _addr(hu), _addr(hv), _addr(H), self.fields, s), "gocean_noise_soar_balance")
+            return
+        G.check(self.lib.gocean_noise_soar(d, float(soar_q0), scale, self.random_numbers.data.gpudata,
+                                           self.coarse_buffer.data.gpudata, 0, s), "gocean_noise_soar")
+        if perpendicular_scale > 0:
+            G.check(self.lib.gocean_noise_soar(d, float(soar_q0), float(np.float32(perpendicular_scale)),
+                                               self.perpendicular_random_numbers.data.gpudata,
+                                               self.coarse_buffer.data.gpudata, 1, s), "gocean_noise_soar")
         G.check(self.lib.gocean_noise_balance(d, self.coarse_buffer.data.gpudata, _addr(eta), _addr(hu),
                                               _addr(hv), _addr(H), self.fields, int(offset_i),
                                               int(offset_j), s), "gocean_noise_balance")

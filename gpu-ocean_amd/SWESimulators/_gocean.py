@@ -150,6 +150,9 @@ PROTOTYPES.extend([
     ("gocean_noise_balance", c_int,
      [ctypes.POINTER(Noise), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
       ctypes.POINTER(Fields), c_int, c_int, c_void_p]),
+    ("gocean_noise_soar_balance", c_int,
+     [ctypes.POINTER(Noise), c_float, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      ctypes.POINTER(Fields), c_void_p]),
     ("gocean_noise_norms", c_int, [ctypes.POINTER(Noise), c_void_p, c_void_p, c_void_p, c_void_p]),
     ("gocean_noise_make_perpendicular", c_int,
      [ctypes.POINTER(Noise), c_void_p, c_void_p, c_void_p, c_void_p]),

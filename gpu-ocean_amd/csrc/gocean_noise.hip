@@ -13,6 +13,7 @@
 //                    directly (factor 1, ocean_noise.cu:379-505) or after
 //                    bicubic interpolation from the coarse grid (odd factor
 //                    > 1, ocean_noise.cu:522-772)
+//   k_noise_fused1   SOAR and balance in one launch for factor 1
 //   k_noise_norms / k_noise_perp   sum xi^2, nu^2, xi.nu (reductions.cu) and
 //                    the perpendicular pair (ocean_noise.cu:206-243)
 //
@@ -257,6 +258,57 @@ constexpr int kWinW = kBalBW / 3 + 8, kWinH = kBalBH / 3 + 8;
 constexpr int kRowT = ((kBalBW + 2 + 63) / 64) * 64;
 static_assert(kRowT + kBalBH + 2 <= kBalBW * kBalBH, "column/row set-up threads");
 
+// Balance of one interior cell given d_eta and its central differences:
+// h_mid, Coriolis, the geostrophic (d_hu, d_hv) and the land rules of
+// geostrophicBalance (interp false, ocean_noise.cu:443-504) or
+// bicubicInterpolation (interp true, :720-770).
+__device__ __forceinline__ void balance_cell(const NoiseLaunch& L, const NoiseMember& M, int ti, int tj,
+                                             float d, float ex, float ey, bool interp) {
+    const float* Hi = L.Hi + M.hi_off;
+    const long long h0 = (long long)tj * M.hi_pitch + ti;
+    const float H00 = Hi[h0], H01 = Hi[h0 + 1], H10 = Hi[h0 + M.hi_pitch], H11 = Hi[h0 + M.hi_pitch + 1];
+    const bool dry = H00 == M.land || H01 == M.land || H10 == M.land || H11 == M.land;
+    const float Hmid = 0.25f * (H00 + H01 + H10 + H11);
+
+    const long long s = (long long)tj * M.pitch + ti;
+    float nxv = M.north_x, nyv = M.north_y;
+    if (M.nxf_off >= 0) {
+        nxv = L.fnx[M.nxf_off + s];
+        nyv = L.fny[M.nyf_off + s];
+    }
+    float cor = M.f + M.beta * (((float)ti + 0.5f) * M.dx * nxv + ((float)tj + 0.5f) * M.dy * nyv);
+    if (M.f == 0.0f) cor = L.ff[M.f_off + s];
+    const float gc = M.g / cor;
+
+    float* eta = L.eta + M.eta_off + s;
+    float* hu = L.hu + M.hu_off + s;
+    float* hv = L.hv + M.hv_off + s;
+    const float e0 = *eta;
+    if (interp) {
+        const float hm = Hmid + d;
+        float dhu = -gc * hm * ey;
+        float dhv = gc * hm * ex;
+        if (dry) { d = 0.0f; dhu = 0.0f; dhv = 0.0f; }
+        *eta = e0 + d;
+        *hu = *hu + dhu;
+        *hv = *hv + dhv;
+    } else {
+        const float hm = d + Hmid + e0;
+        const float dhu = -gc * hm * ey;
+        const float dhv = gc * hm * ex;
+        if (dry) {
+            *eta = 0.0f;
+            *hu = 0.0f;
+            *hv = 0.0f;
+        } else {
+            *eta = e0 + d;
+            *hu = *hu + dhu;
+            *hv = *hv + dhv;
+        }
+    }
+}
+
+
 // One thread per fine interior cell (ghosted ti = 2..nx+1, tj = 2..ny+1).
 // Factor 1: d_eta is the coarse buffer itself, h_mid includes the current
 // eta and a cell with a land corner gets eta = hu = hv = 0
@@ -345,22 +397,6 @@ __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_balance(NoiseLaunch L
     const int ti = bx + tx, tj = by + ty;
     if (ti >= M.nx + 2 || tj >= M.ny + 2) return;
 
-    const float* Hi = L.Hi + M.hi_off;
-    const long long h0 = (long long)tj * M.hi_pitch + ti;
-    const float H00 = Hi[h0], H01 = Hi[h0 + 1], H10 = Hi[h0 + M.hi_pitch], H11 = Hi[h0 + M.hi_pitch + 1];
-    const bool dry = H00 == M.land || H01 == M.land || H10 == M.land || H11 == M.land;
-    const float Hmid = 0.25f * (H00 + H01 + H10 + H11);
-
-    const long long s = (long long)tj * M.pitch + ti;
-    float nxv = M.north_x, nyv = M.north_y;
-    if (M.nxf_off >= 0) {
-        nxv = L.fnx[M.nxf_off + s];
-        nyv = L.fny[M.nyf_off + s];
-    }
-    float cor = M.f + M.beta * (((float)ti + 0.5f) * M.dx * nxv + ((float)tj + 0.5f) * M.dy * nyv);
-    if (M.f == 0.0f) cor = L.ff[M.f_off + s];
-    const float gc = M.g / cor;
-
     float d, ex, ey;
     if (interp) {
         d = de[ty + 1][tx + 1];
@@ -372,32 +408,66 @@ __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_balance(NoiseLaunch L
         ex = (C[c0 + 1] - C[c0 - 1]) / (2.0f * M.dx);
         ey = (C[c0 + cp] - C[c0 - cp]) / (2.0f * M.dy);
     }
-    float* eta = L.eta + M.eta_off + s;
-    float* hu = L.hu + M.hu_off + s;
-    float* hv = L.hv + M.hv_off + s;
-    const float e0 = *eta;
-    if (interp) {
-        const float hm = Hmid + d;
-        float dhu = -gc * hm * ey;
-        float dhv = gc * hm * ex;
-        if (dry) { d = 0.0f; dhu = 0.0f; dhv = 0.0f; }
-        *eta = e0 + d;
-        *hu = *hu + dhu;
-        *hv = *hv + dhv;
-    } else {
-        const float hm = d + Hmid + e0;
-        const float dhu = -gc * hm * ey;
-        const float dhv = gc * hm * ex;
-        if (dry) {
-            *eta = 0.0f;
-            *hu = 0.0f;
-            *hv = 0.0f;
-        } else {
-            *eta = e0 + d;
-            *hu = *hu + dhu;
-            *hv = *hv + dhv;
-        }
+    balance_cell(L, M, ti, tj, d, ex, ey, interp);
+}
+
+// SOAR and geostrophic balance in one launch for interpolation factor 1
+// (the SOAR kernel followed by geostrophicBalance, ocean_noise.cu:264-505):
+// the block computes d_eta for its 64x4 tile and a 2-cell ring in LDS from
+// the random field (2-cell SOAR halo on top), writes the coarse-buffer cells
+// it owns (the tile's interior, plus the ghost rings next to it on the
+// domain's edges: each coarse cell once), and balances its interior cells.
+template <bool BATCH>
+__global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_fused1(NoiseLaunch L, float q0_scale,
+                                                                  float scale) {
+    constexpr int DW = kBalBW + 4, DH = kBalBH + 4;             // d_eta: tile + 2-cell ring
+    constexpr int RW = DW + 2 * kCut, RH = DH + 2 * kCut;       // random values: + SOAR halo
+    constexpr int NW = (2 * kCut + 1) * (2 * kCut + 1);
+    __shared__ float xi[RH][RW + 1];
+    __shared__ float de[DH][DW + 1];
+    const NoiseMember& M = member_of<BATCH>(L);
+    float w[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) w[k] = M.w[k];
+    const int lt = blockIdx.x - M.bal_begin;
+    const int by = 2 + (lt / M.bal_tiles_x) * kBalBH;
+    const int bx = 2 + (lt % M.bal_tiles_x) * kBalBW;
+    const int ay0 = by - 2, ax0 = bx - 2;     // coarse (= fine, ghosted) index of de[0][0]
+    const float* rnd = L.random + M.rand_off;
+    for (int k = threadIdx.x; k < RH * RW; k += kBalBW * kBalBH) {
+        const int j = k / RW, i = k - j * RW;
+        const int gy = M.periodic_ns ? pmod(ay0 + j - 2 * kCut, M.cny) : min(ay0 + j, M.rand_ny - 1);
+        const int gx = M.periodic_ew ? pmod(ax0 + i - 2 * kCut, M.cnx) : min(ax0 + i, M.rand_nx - 1);
+        xi[j][i] = rnd[(long long)gy * M.rand_nx + gx];
     }
+    __syncthreads();
+    const float q = M.q0 * q0_scale;
+    const int cp = M.cnx + 4;
+    float* C = L.coarse + M.coarse_off;
+    for (int k = threadIdx.x; k < DH * DW; k += kBalBW * kBalBH) {
+        const int j = k / DW, i = k - j * DW;
+        const int ay = ay0 + j, ax = ax0 + i;
+        if (ay > M.cny + 3 || ax > M.cnx + 3) continue;
+        float acc = 0.0f;
+#pragma unroll
+        for (int oy = 0; oy <= 2 * kCut; ++oy)
+#pragma unroll
+            for (int ox = 0; ox <= 2 * kCut; ++ox)
+                acc = acc + w[oy * (2 * kCut + 1) + ox] * xi[j + oy][i + ox];
+        const float v = (q * acc) * scale;
+        de[j][i] = v;
+        // owner: the tile whose interior holds the cell clamped to the interior
+        const int oy = min(max(ay, 2), M.ny + 1), ox = min(max(ax, 2), M.nx + 1);
+        if (oy >= by && oy < by + kBalBH && ox >= bx && ox < bx + kBalBW) C[(long long)ay * cp + ax] = v;
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % kBalBW, ty = threadIdx.x / kBalBW;
+    const int ti = bx + tx, tj = by + ty;
+    if (ti >= M.nx + 2 || tj >= M.ny + 2) return;
+    const float d = de[ty + 2][tx + 2];
+    const float ex = (de[ty + 2][tx + 3] - de[ty + 2][tx + 1]) / (2.0f * M.dx);
+    const float ey = (de[ty + 3][tx + 2] - de[ty + 1][tx + 2]) / (2.0f * M.dy);
+    balance_cell(L, M, ti, tj, d, ex, ey, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -661,6 +731,47 @@ int gocean_noise_balance(const gocean_noise_t* nz, const float* coarse, float* e
     return check_launch("k_noise_balance");
 }
 
+int gocean_noise_soar_balance(const gocean_noise_t* nz, float q0, float scale, const float* random,
+                              float* coarse, float* eta, float* hu, float* hv, const float* Hi,
+                              const gocean_fields_t* fields, void* stream) {
+    clear_error();
+    Sizes S;
+    if (int e = validate_noise(nz, true, S)) return e;
+    if (nz->interpolation_factor != 1)
+        return set_error(GOCEAN_EINVAL, "the fused SOAR + balance needs interpolation factor 1 (got %d)",
+                         nz->interpolation_factor);
+    if (!random || !coarse || !eta || !hu || !hv || !Hi) return set_error(GOCEAN_EINVAL, "a buffer is NULL");
+    const bool have_f = fields && fields->f;
+    const bool have_n = fields && (fields->north_x || fields->north_y);
+    if (have_n && !(fields->north_x && fields->north_y))
+        return set_error(GOCEAN_EINVAL, "north_x and north_y fields must be given together");
+    if (int e = check_coriolis(nz, have_f)) return e;
+    NoiseLaunch L{};
+    fill_member(nz, S, L.one);
+    L.one.q0 = q0;
+    int tx, ty;
+    bal_tiles(L.one, tx, ty);
+    L.one.bal_tiles_x = tx;
+    L.eta = eta;
+    L.hu = hu;
+    L.hv = hv;
+    L.Hi = Hi;
+    L.random = (float*)random;
+    L.coarse = coarse;
+    if (have_f) {
+        L.ff = fields->f;
+        L.one.f_off = 0;
+    }
+    if (have_n) {
+        L.fnx = fields->north_x;
+        L.fny = fields->north_y;
+        L.one.nxf_off = L.one.nyf_off = 0;
+    }
+    hipLaunchKernelGGL(k_noise_fused1<false>, dim3(tx * ty), dim3(kBalBW * kBalBH), 0, (hipStream_t)stream,
+                       L, 1.0f, scale);
+    return check_launch("k_noise_fused1");
+}
+
 int gocean_noise_norms(const gocean_noise_t* nz, const float* xi, const float* nu, float* out,
                        void* stream) {
     clear_error();
@@ -698,6 +809,7 @@ struct gocean_noise_batch {
     int* d_bal_map = nullptr;
     int n_rng = 0, n_soar = 0, n_bal = 0;
     bool any_field = false;
+    bool all_f1 = true;     // every member at interpolation factor 1: fused SOAR + balance
     long long totals[3] = {0, 0, 0};
 };
 
@@ -743,7 +855,8 @@ int gocean_noise_batch_create(const gocean_noise_member_t* members, int n, int k
         rand_off += ((long long)M.rand_nx * M.rand_ny + 63) / 64 * 64;
         coarse_off += ((long long)(M.cnx + 4) * (M.cny + 4) + 63) / 64 * 64;
         M.rng_begin = (int)rng_map.size();
-        rng_map.insert(rng_map.end(), std::max(rng_blocks(M, kind), perp_blocks(M)), m);
+        rng_map.insert(rng_map.end(), rng_blocks(M, kind), m);
+        if (M.factor != 1) B->all_f1 = false;
         int tx, ty;
         soar_tiles(M, tx, ty);
         M.soar_begin = (int)soar_map.size();
@@ -839,13 +952,19 @@ int gocean_noise_batch_perturb(gocean_noise_batch_t* B, float* state, const floa
     NoiseLaunch L = batch_launch(B);
     L.random = random;
     L.coarse = coarse;
+    L.eta = L.hu = L.hv = state;
+    L.Hi = Hi;
+    L.ff = L.fnx = L.fny = fields;
+    if (B->all_f1) {
+        L.map = B->d_bal_map;
+        hipLaunchKernelGGL(k_noise_fused1<true>, dim3(B->n_bal), dim3(kBalBW * kBalBH), 0, s, L, q0_scale,
+                           perturbation_scale);
+        return check_launch("model error (batch, fused)");
+    }
     L.map = B->d_soar_map;
     hipLaunchKernelGGL(k_noise_soar<true>, dim3(B->n_soar), dim3(kSoarBW * kSoarBH), 0, s, L, q0_scale,
                        perturbation_scale, 0);
     L.map = B->d_bal_map;
-    L.eta = L.hu = L.hv = state;
-    L.Hi = Hi;
-    L.ff = L.fnx = L.fny = fields;
     hipLaunchKernelGGL(k_noise_balance<true>, dim3(B->n_bal), dim3(kBalBW * kBalBH), 0, s, L, 0, 0);
     return check_launch("model error (batch)");
 }

@@ -334,6 +334,15 @@ int gocean_noise_balance(const gocean_noise_t* nz, const float* coarse, float* e
                          float* hv, const float* Hi, const gocean_fields_t* fields, int offset_i,
                          int offset_j, void* stream);
 
+/* The SOAR kernel followed by geostrophicBalance in one launch, for
+ * interpolation factor 1 (ocean_noise.cu:264-505): the same coarse buffer
+ * and the same state as gocean_noise_soar (not additive) then
+ * gocean_noise_balance with zero offsets, without the coarse buffer's round
+ * trip through memory between the two. */
+int gocean_noise_soar_balance(const gocean_noise_t* nz, float q0, float scale, const float* random,
+                              float* coarse, float* eta, float* hu, float* hv, const float* Hi,
+                              const gocean_fields_t* fields, void* stream);
+
 /* out = [sum xi^2, sum nu^2, sum xi nu] over the random field (squareSum /
  * squareSumDouble, reductions.cu:27-185); nu == NULL writes out[0] only. */
 int gocean_noise_norms(const gocean_noise_t* nz, const float* xi, const float* nu, float* out,
@@ -373,7 +382,8 @@ int gocean_noise_batch_layout(const gocean_noise_batch_t* batch, int64_t* offset
 int gocean_noise_batch_generate(gocean_noise_batch_t* batch, uint64_t* seed, float* random,
                                 uint64_t key, uint64_t* counter, void* stream);
 /* perturbOceanState for every member: [generate,] SOAR with amplitude
- * soar_q0 * q0_scale, balance with zero alignment offsets. */
+ * soar_q0 * q0_scale, balance with zero alignment offsets (one fused launch
+ * when every member has interpolation factor 1). */
 int gocean_noise_batch_perturb(gocean_noise_batch_t* batch, float* state, const float* Hi,
                                const float* fields, float q0_scale, float perturbation_scale,
                                int update_random, uint64_t* seed, float* random, float* coarse,

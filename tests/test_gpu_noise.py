@@ -505,3 +505,40 @@ def test_graph_replays_with_model_error_match_eager():
         for a, b in zip(got, ens.state):
             assert torch.equal(a, b), rep
         assert not torch.equal(got[0], s0[0])
+
+
+@pytest.mark.parametrize("nx,ny", [(45, 38), (130, 9)])
+@pytest.mark.parametrize("pns,pew", BCS)
+def test_fused_soar_balance_equals_two_launches(nx, ny, pns, pew):
+    """gocean_noise_soar_balance == gocean_noise_soar then
+    gocean_noise_balance, bit for bit: the state and every coarse-buffer cell
+    (interior and both ghost rings), with land and per-cell fields."""
+    G, lib = _lib()
+    torch = _torch()
+    _say("fused soar+balance", nx, ny, pns, pew)
+    rs = np.random.RandomState(9)
+    H, eta, hu, hv = _state(rs, nx, ny, [(5, 6), (ny, nx // 2)])
+    cnx, cny, rnx, rny, *_ = O.noise_sizes(nx, ny, 1, pns, pew)
+    xi = _dev(rs.randn(rny, rnx).astype(np.float32))
+    d = _desc(nx, ny, 1, pns, pew, beta=2e-6)
+    ang = 0.2 * rs.rand(ny + 4, nx + 4)
+    keep = [_dev(a.astype(np.float32)) for a in (F + 1e-3 * rs.rand(ny + 4, nx + 4), np.sin(ang), np.cos(ang))]
+    fields = G.Fields()
+    fields.f, fields.north_x, fields.north_y = (t.data_ptr() for t in keep)
+    q0 = float(np.float32(d.soar_q0 * 0.9))
+    outs = []
+    for fused in (False, True):
+        st = [_dev(a) for a in (eta, hu, hv, H)]
+        coarse = torch.full((cny + 4, cnx + 4), -3.0, dtype=torch.float32, device="cuda")
+        if fused:
+            G.check(lib.gocean_noise_soar_balance(d, q0, 0.8, xi.data_ptr(), coarse.data_ptr(), st[0].data_ptr(),
+                                                  st[1].data_ptr(), st[2].data_ptr(), st[3].data_ptr(), fields, None))
+        else:
+            G.check(lib.gocean_noise_soar(d, q0, 0.8, xi.data_ptr(), coarse.data_ptr(), 0, None))
+            G.check(lib.gocean_noise_balance(d, coarse.data_ptr(), st[0].data_ptr(), st[1].data_ptr(),
+                                             st[2].data_ptr(), st[3].data_ptr(), fields, 0, 0, None))
+        torch.cuda.synchronize()
+        outs.append([t.cpu().numpy() for t in st[:3]] + [coarse.cpu().numpy()])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert not np.any(outs[1][3] == -3.0), "every coarse cell written"
