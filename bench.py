@@ -415,6 +415,36 @@ def bench_model_error(ctx, inp, reps=50, ens_members=100, ens_n=512, ens_steps=2
     return out
 
 
+def bench_etkf(ctx, members=100, n=512, n_drifters=64, reps=5):
+    """Global ETKF analysis (ETKFOcean.py:74-301) of the C3 ensemble on the
+    device: batched observation of every member at the true drifters, the
+    ensemble-space algebra in float64 on the host, and the state update as one
+    GEMM.  Algorithmic bytes: the state read twice (mean, update) and written
+    once, 3 x N x 3 (ny+4)(nx+4) x 4 B."""
+    import torch
+    from SWESimulators import BathymetryAndICs as B, CDKLM16Ensemble, ETKFOcean
+    inps = B.config3(members=members, n=n)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, inps, courant_number=0.7)
+    ens.enable_drifters(n_drifters)
+    rs = np.random.default_rng(9)
+    pos = np.column_stack([rs.uniform(0, n * inps[0].dx, n_drifters), rs.uniform(0, n * inps[0].dy, n_drifters)])
+    obs = np.column_stack([pos, 0.05 * rs.standard_normal((n_drifters, 2))])
+    et = ETKFOcean.ETKFOcean(ens, np.diag([0.01, 0.01]))
+    et.ETKF(obs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        et.ETKF(obs)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    state_bytes = members * 3 * (n + 4) * (n + 4) * 4
+    ens.cleanUp()
+    return {"workload": f"{members} members x {n}x{n}, {n_drifters} drifters (hu, hv observed)",
+            "analysis_ms": round(t * 1e3, 2),
+            "state_MB": round(state_bytes / 1e6, 1),
+            "algorithmic_GBs": round(3 * state_bytes / t / 1e9, 1)}
+
+
 def measured_traffic(cells):
     """HBM bytes per substep launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy), when
@@ -497,6 +527,11 @@ def main():
             log("[bench] batched uneven ensemble with adjoint C5")
             extra["ensemble_adjoint"] = bench_c5(ctx)
             log(f"[bench] ensemble adjoint: {extra['ensemble_adjoint']}")
+
+    if not a.no_ensemble:
+        log("[bench] ETKF analysis on the C3 ensemble")
+        extra["etkf"] = bench_etkf(ctx)
+        log(f"[bench] etkf: {extra['etkf']}")
 
     if not a.no_model_error:
         log("[bench] model error (OceanStateNoise) on the headline domain and the ensemble")

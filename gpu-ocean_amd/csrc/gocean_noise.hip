@@ -36,6 +36,7 @@ constexpr int kCut = 2;                    // config.soar_cutoff
 constexpr int kRngNT = 256;
 constexpr int kSoarBW = 32, kSoarBH = 8;   // SOAR tile over the coarse buffer
 constexpr int kBalBW = 64, kBalBH = 4;     // balance tile over fine interior cells
+constexpr int kFuBW = 64, kFuBH = 16;      // fused SOAR + balance tile (factor 1)
 constexpr int kNormNT = 1024;
 
 struct NoiseMember {
@@ -48,7 +49,7 @@ struct NoiseMember {
     int rand_nx, rand_ny, seed_nx, seed_ny;
     long long seed_off, rand_off, coarse_off;
     float w[(2 * kCut + 1) * (2 * kCut + 1)];   // (1 + d/L) exp(-d/L), row-major (dy, dx)
-    int rng_begin, soar_begin, soar_tiles_x, bal_begin, bal_tiles_x;
+    int rng_begin, soar_begin, soar_tiles_x, bal_begin, bal_tiles_x, fu_begin, fu_tiles_x;
     int id;
 };
 
@@ -418,9 +419,9 @@ __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_balance(NoiseLaunch L
 // it owns (the tile's interior, plus the ghost rings next to it on the
 // domain's edges: each coarse cell once), and balances its interior cells.
 template <bool BATCH>
-__global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_fused1(NoiseLaunch L, float q0_scale,
+__global__ __launch_bounds__(kFuBW * kFuBH) void k_noise_fused1(NoiseLaunch L, float q0_scale,
                                                                   float scale) {
-    constexpr int DW = kBalBW + 4, DH = kBalBH + 4;             // d_eta: tile + 2-cell ring
+    constexpr int DW = kFuBW + 4, DH = kFuBH + 4;             // d_eta: tile + 2-cell ring
     constexpr int RW = DW + 2 * kCut, RH = DH + 2 * kCut;       // random values: + SOAR halo
     constexpr int NW = (2 * kCut + 1) * (2 * kCut + 1);
     __shared__ float xi[RH][RW + 1];
@@ -429,12 +430,12 @@ __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_fused1(NoiseLaunch L,
     float w[NW];
 #pragma unroll
     for (int k = 0; k < NW; ++k) w[k] = M.w[k];
-    const int lt = blockIdx.x - M.bal_begin;
-    const int by = 2 + (lt / M.bal_tiles_x) * kBalBH;
-    const int bx = 2 + (lt % M.bal_tiles_x) * kBalBW;
+    const int lt = blockIdx.x - M.fu_begin;
+    const int by = 2 + (lt / M.fu_tiles_x) * kFuBH;
+    const int bx = 2 + (lt % M.fu_tiles_x) * kFuBW;
     const int ay0 = by - 2, ax0 = bx - 2;     // coarse (= fine, ghosted) index of de[0][0]
     const float* rnd = L.random + M.rand_off;
-    for (int k = threadIdx.x; k < RH * RW; k += kBalBW * kBalBH) {
+    for (int k = threadIdx.x; k < RH * RW; k += kFuBW * kFuBH) {
         const int j = k / RW, i = k - j * RW;
         const int gy = M.periodic_ns ? pmod(ay0 + j - 2 * kCut, M.cny) : min(ay0 + j, M.rand_ny - 1);
         const int gx = M.periodic_ew ? pmod(ax0 + i - 2 * kCut, M.cnx) : min(ax0 + i, M.rand_nx - 1);
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_fused1(NoiseLaunch L,
     const float q = M.q0 * q0_scale;
     const int cp = M.cnx + 4;
     float* C = L.coarse + M.coarse_off;
-    for (int k = threadIdx.x; k < DH * DW; k += kBalBW * kBalBH) {
+    for (int k = threadIdx.x; k < DH * DW; k += kFuBW * kFuBH) {
         const int j = k / DW, i = k - j * DW;
         const int ay = ay0 + j, ax = ax0 + i;
         if (ay > M.cny + 3 || ax > M.cnx + 3) continue;
@@ -458,10 +459,10 @@ __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_fused1(NoiseLaunch L,
         de[j][i] = v;
         // owner: the tile whose interior holds the cell clamped to the interior
         const int oy = min(max(ay, 2), M.ny + 1), ox = min(max(ax, 2), M.nx + 1);
-        if (oy >= by && oy < by + kBalBH && ox >= bx && ox < bx + kBalBW) C[(long long)ay * cp + ax] = v;
+        if (oy >= by && oy < by + kFuBH && ox >= bx && ox < bx + kFuBW) C[(long long)ay * cp + ax] = v;
     }
     __syncthreads();
-    const int tx = threadIdx.x % kBalBW, ty = threadIdx.x / kBalBW;
+    const int tx = threadIdx.x % kFuBW, ty = threadIdx.x / kFuBW;
     const int ti = bx + tx, tj = by + ty;
     if (ti >= M.nx + 2 || tj >= M.ny + 2) return;
     const float d = de[ty + 2][tx + 2];
@@ -619,6 +620,10 @@ void bal_tiles(const NoiseMember& M, int& tx, int& ty) {
     tx = (M.nx + kBalBW - 1) / kBalBW;
     ty = (M.ny + kBalBH - 1) / kBalBH;
 }
+void fu_tiles(const NoiseMember& M, int& tx, int& ty) {
+    tx = (M.nx + kFuBW - 1) / kFuBW;
+    ty = (M.ny + kFuBH - 1) / kFuBH;
+}
 
 int check_kind(int kind) {
     if (kind < GOCEAN_RNG_LCG_NORMAL || kind > GOCEAN_RNG_PHILOX_UNIFORM)
@@ -750,8 +755,8 @@ int gocean_noise_soar_balance(const gocean_noise_t* nz, float q0, float scale, c
     fill_member(nz, S, L.one);
     L.one.q0 = q0;
     int tx, ty;
-    bal_tiles(L.one, tx, ty);
-    L.one.bal_tiles_x = tx;
+    fu_tiles(L.one, tx, ty);
+    L.one.fu_tiles_x = tx;
     L.eta = eta;
     L.hu = hu;
     L.hv = hv;
@@ -767,7 +772,7 @@ int gocean_noise_soar_balance(const gocean_noise_t* nz, float q0, float scale, c
         L.fny = fields->north_y;
         L.one.nxf_off = L.one.nyf_off = 0;
     }
-    hipLaunchKernelGGL(k_noise_fused1<false>, dim3(tx * ty), dim3(kBalBW * kBalBH), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_noise_fused1<false>, dim3(tx * ty), dim3(kFuBW * kFuBH), 0, (hipStream_t)stream,
                        L, 1.0f, scale);
     return check_launch("k_noise_fused1");
 }
@@ -807,7 +812,8 @@ struct gocean_noise_batch {
     int* d_rng_map = nullptr;
     int* d_soar_map = nullptr;
     int* d_bal_map = nullptr;
-    int n_rng = 0, n_soar = 0, n_bal = 0;
+    int* d_fu_map = nullptr;
+    int n_rng = 0, n_soar = 0, n_bal = 0, n_fu = 0;
     bool any_field = false;
     bool all_f1 = true;     // every member at interpolation factor 1: fused SOAR + balance
     long long totals[3] = {0, 0, 0};
@@ -822,7 +828,7 @@ int gocean_noise_batch_create(const gocean_noise_member_t* members, int n, int k
     B->n = n;
     B->kind = kind;
     B->host.resize(n);
-    std::vector<int> rng_map, soar_map, bal_map;
+    std::vector<int> rng_map, soar_map, bal_map, fu_map;
     long long seed_off = 0, rand_off = 0, coarse_off = 0;
     for (int m = 0; m < n; ++m) {
         const gocean_noise_member_t& G = members[m];
@@ -866,6 +872,10 @@ int gocean_noise_batch_create(const gocean_noise_member_t* members, int n, int k
         M.bal_begin = (int)bal_map.size();
         M.bal_tiles_x = tx;
         bal_map.insert(bal_map.end(), tx * ty, m);
+        fu_tiles(M, tx, ty);
+        M.fu_begin = (int)fu_map.size();
+        M.fu_tiles_x = tx;
+        fu_map.insert(fu_map.end(), tx * ty, m);
     }
     B->totals[0] = seed_off;
     B->totals[1] = rand_off;
@@ -873,15 +883,18 @@ int gocean_noise_batch_create(const gocean_noise_member_t* members, int n, int k
     B->n_rng = (int)rng_map.size();
     B->n_soar = (int)soar_map.size();
     B->n_bal = (int)bal_map.size();
+    B->n_fu = (int)fu_map.size();
     bool ok = hipMalloc(&B->d_members, sizeof(NoiseMember) * n) == hipSuccess &&
               hipMalloc(&B->d_rng_map, sizeof(int) * rng_map.size()) == hipSuccess &&
               hipMalloc(&B->d_soar_map, sizeof(int) * soar_map.size()) == hipSuccess &&
-              hipMalloc(&B->d_bal_map, sizeof(int) * bal_map.size()) == hipSuccess;
+              hipMalloc(&B->d_bal_map, sizeof(int) * bal_map.size()) == hipSuccess &&
+              hipMalloc(&B->d_fu_map, sizeof(int) * fu_map.size()) == hipSuccess;
     ok = ok &&
          hipMemcpy(B->d_members, B->host.data(), sizeof(NoiseMember) * n, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B->d_rng_map, rng_map.data(), sizeof(int) * rng_map.size(), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(B->d_soar_map, soar_map.data(), sizeof(int) * soar_map.size(), hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(B->d_bal_map, bal_map.data(), sizeof(int) * bal_map.size(), hipMemcpyHostToDevice) == hipSuccess;
+         hipMemcpy(B->d_bal_map, bal_map.data(), sizeof(int) * bal_map.size(), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(B->d_fu_map, fu_map.data(), sizeof(int) * fu_map.size(), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
         gocean_noise_batch_destroy(B);
         (void)hipGetLastError();
@@ -897,6 +910,7 @@ int gocean_noise_batch_destroy(gocean_noise_batch_t* B) {
     (void)hipFree(B->d_rng_map);
     (void)hipFree(B->d_soar_map);
     (void)hipFree(B->d_bal_map);
+    (void)hipFree(B->d_fu_map);
     delete B;
     return 0;
 }
@@ -956,8 +970,8 @@ int gocean_noise_batch_perturb(gocean_noise_batch_t* B, float* state, const floa
     L.Hi = Hi;
     L.ff = L.fnx = L.fny = fields;
     if (B->all_f1) {
-        L.map = B->d_bal_map;
-        hipLaunchKernelGGL(k_noise_fused1<true>, dim3(B->n_bal), dim3(kBalBW * kBalBH), 0, s, L, q0_scale,
+        L.map = B->d_fu_map;
+        hipLaunchKernelGGL(k_noise_fused1<true>, dim3(B->n_fu), dim3(kFuBW * kFuBH), 0, s, L, q0_scale,
                            perturbation_scale);
         return check_launch("model error (batch, fused)");
     }
