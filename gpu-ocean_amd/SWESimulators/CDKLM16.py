@@ -14,8 +14,9 @@ Differences from the reference, by design:
   * runs of equal time steps are issued with one library call
     (``gocean_rk_steps``); ``step_n`` and ``capture_graph`` expose fixed-dt
     stepping without any host synchronisation;
-  * open boundaries (3, 4), gridded wind and NetCDF output are not part of
-    this build and raise ``NotImplementedError``.
+  * open boundaries (3, 4) and gridded wind are not part of this build and
+    raise ``NotImplementedError``; NetCDF output is NetCDF-3 (scipy.io)
+    since netCDF4 is not available.
 """
 
 from __future__ import annotations
@@ -25,7 +26,7 @@ import logging
 
 import numpy as np
 
-from . import Common, OceanStateNoise, Simulator, WindStress
+from . import Common, OceanStateNoise, SimReader, SimWriter, Simulator, WindStress
 from . import _gocean as G
 
 
@@ -67,8 +68,6 @@ class CDKLM16(Simulator.Simulator):
         if any(b not in (Common.WALL, Common.PERIODIC) for b in bc_codes):
             raise NotImplementedError("open boundary conditions (3: flow relaxation, "
                                       "4: linear interpolation) are not part of this build")
-        if write_netcdf:
-            raise NotImplementedError("NetCDF output is not part of this build")
         if max_wind_direction_perturbation > 0:
             raise NotImplementedError("wind direction perturbation needs the legacy wind classes")
         if not wind_stress.is_uniform():
@@ -165,6 +164,12 @@ class CDKLM16(Simulator.Simulator):
         self.total_time_steps = 0
         if model_time_step is None:
             self.model_time_step = self.dt
+        if self.write_netcdf:
+            # NetCDF output after each step() (CDKLM16.py:375-377)
+            self.sim_writer = SimWriter.SimNetCDFWriter(self, super_dir_name=super_dir_name,
+                                                        filename=netcdf_filename,
+                                                        ignore_ghostcells=self.ignore_ghostcells,
+                                                        offset_x=self.offset_x, offset_y=self.offset_y)
         if self.dt <= 0:
             self.updateDt()
 
@@ -242,6 +247,8 @@ class CDKLM16(Simulator.Simulator):
 
     # ------------------------------------------------------------------
     def cleanUp(self):
+        if getattr(self, "sim_writer", None) is not None:
+            self.closeNetCDF()
         if getattr(self, "gpu_data", None) is not None:
             self.gpu_data.release()
         if getattr(self, "bathymetry", None) is not None:
@@ -252,10 +259,45 @@ class CDKLM16(Simulator.Simulator):
         self.gpu_ctx = None
         gc.collect()
 
+    def closeNetCDF(self):
+        if self.write_netcdf and self.sim_writer is not None:
+            self.sim_writer.close()
+            self.write_netcdf = False
+
     @classmethod
     def fromfilename(cls, gpu_ctx, filename, cont_write_netcdf=True, use_lcg=False,
                      xorwow_seed=None, new_netcdf_filename=None, time0=None):
-        raise NotImplementedError("NetCDF hot start is not part of this build")
+        """Hot start from the last (or the ``time0``) record of a file written
+        by SimNetCDFWriter, with its parameters (CDKLM16.py:410-490)."""
+        reader = SimReader.SimNetCDFReader(filename, ignore_ghostcells=False)
+        sim_name = str(reader.get('simulator_short'))
+        assert sim_name == cls.__name__, \
+            "Trying to initialize a " + cls.__name__ + " simulator with netCDF file based on " + sim_name + " results."
+        H = reader.getH()
+        if time0 is None:
+            eta0, hu0, hv0, time0 = reader.getLastTimeStep()
+        else:
+            eta0, hu0, hv0, time0 = reader.getStateAtTime(time0)
+        if len(H.shape) == 3:
+            H = H[0, :, :]
+        params = dict(gpu_ctx=gpu_ctx, eta0=eta0, hu0=hu0, hv0=hv0, H=H,
+                      nx=reader.get("nx"), ny=reader.get("ny"), dx=reader.get("dx"), dy=reader.get("dy"),
+                      dt=reader.get("dt"), g=reader.get("g"), f=reader.get("coriolis_force"),
+                      r=reader.get("bottom_friction_r"), t=float(time0), theta=reader.get("minmod_theta"),
+                      rk_order=reader.get("time_integrator"), coriolis_beta=reader.get("coriolis_beta"),
+                      write_netcdf=cont_write_netcdf, use_lcg=use_lcg, xorwow_seed=xorwow_seed,
+                      netcdf_filename=new_netcdf_filename, wind_stress=WindStress.WindStress(),
+                      boundary_conditions=reader.getBC())
+        if reader.has('small_scale_perturbation'):
+            params['small_scale_perturbation'] = reader.get('small_scale_perturbation') == 'True'
+            if params['small_scale_perturbation']:
+                params['small_scale_perturbation_amplitude'] = reader.get('small_scale_perturbation_amplitude')
+                params['small_scale_perturbation_interpolation_factor'] = reader.get(
+                    'small_scale_perturbation_interpolation_factor')
+        if reader.has('model_time_step'):
+            params['model_time_step'] = reader.get('model_time_step')
+        reader.close()
+        return cls(**params)
 
     def step(self, t_end=0.0, apply_stochastic_term=True, write_now=True, update_dt=False):
         """Advance by ``t_end`` seconds (CDKLM16.py:494-598).  Same loop:
@@ -287,6 +329,8 @@ class CDKLM16(Simulator.Simulator):
                     self.t += np.float64(dts[q])
                     self.num_iterations += 1
                 k = m
+            if self.write_netcdf and write_now:
+                self.sim_writer.writeTimestep(self)
             return self.t
         while t_now < t_end:
             if update_dt:
@@ -301,6 +345,9 @@ class CDKLM16(Simulator.Simulator):
             self.t += np.float64(local_dt)
             t_now += np.float64(local_dt)
             self.num_iterations += 1
+        # NetCDF record after the step (CDKLM16.py:595-596)
+        if self.write_netcdf and write_now:
+            self.sim_writer.writeTimestep(self)
         return self.t
 
     def step_n(self, n_steps, dt=None, dt_dev=None):
@@ -370,12 +417,15 @@ class CDKLM16(Simulator.Simulator):
             self.total_time_steps += 1
             if self.total_time_steps % 5 == 0:
                 self.updateDt(courant_number=courant_number)
+        if self.write_netcdf and write_now:
+            self.sim_writer.writeTimestep(self)
         assert round(observation_time) == round(self.t), \
             'The simulation time is not the same as observation time after dataAssimilationStep! \n' + \
             '(self.t, observation_time, diff): ' + str((self.t, observation_time, self.t - observation_time))
 
     def writeState(self):
-        pass
+        if self.write_netcdf:
+            self.sim_writer.writeTimestep(self)
 
     def max_dt_device(self, courant_number=None):
         """Launch the CFL reduction; returns the scratch tensor whose [0] is

@@ -284,7 +284,8 @@ class BoundaryConditions:
               ", east: " + self._toString(self.east) + \
               ", south: " + self._toString(self.south) + \
               ", west: " + self._toString(self.west)
-        return msg + ", spongeCells: " + str(self.spongeCells)
+        # plain ints, as numpy 1.x printed them (numpy 2 prints np.int32(...))
+        return msg + ", spongeCells: " + str({k: int(v) for k, v in self.spongeCells.items()})
 
     @classmethod
     def fromstring(cls, bc_string):
