@@ -176,26 +176,41 @@ def bench_adjoint(ctx, n=512, steps=500, K=50, n_obs=64, seed=44):
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
 
-    def fwd_adj():
-        q = [a.clone().requires_grad_(True) for a in q0]
-        out = A.CDKLM16RK2.apply(*q, adj, steps, dt)
+    def fwd_adj(a):
+        q = [x.clone().requires_grad_(True) for x in q0]
+        out = A.CDKLM16RK2.apply(*q, a, steps, dt)
         J = A.point_misfit(out[0], ox, oy, y, inp.dx, inp.dy)
         J.backward()
         return J, q
 
-    fwd_adj()                       # warm-up (allocations, first launches)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    J, q = fwd_adj()
-    torch.cuda.synchronize()
-    t_tot = time.perf_counter() - t0
+    def timed(a):
+        fwd_adj(a)                  # warm-up (allocations, first launches)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        J, q = fwd_adj(a)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, J, q
+
+    # records kept from the forward run (default when they fit the budget)
+    t_tot, J, q = timed(adj)
+    kept = adj.records is not None
+    rec_gb = adj.records_bytes(steps) / 1e9
+    adj.records = None
+    # checkpointed: the backward recomputes each segment
+    adj_rc = A.simulator_adjoint(sim, checkpoint_every=K)
+    adj_rc.keep_records = False
+    t_rc, J_rc, q_rc = timed(adj_rc)
+    same = all(torch.equal(a.grad, b.grad) for a, b in zip(q, q_rc))
     cells = n * n
     g = [float(x.grad.norm()) for x in q]
     sim.cleanUp()
     return {"workload": f"{n}x{n} walls, random depth, 8 eddies, fixed dt {dt:.3f} s (CFL 0.7), "
                         f"{steps} RK2 steps, checkpoint every {K}, {n_obs} point observations",
             "fwd_s": round(t_fwd, 4), "fwd_adj_s": round(t_tot, 4),
-            "fwd_adj_over_fwd": round(t_tot / t_fwd, 2),
+            "fwd_adj_over_fwd": round(t_tot / t_fwd, 2), "records_kept": kept,
+            "records_gb": round(rec_gb, 2),
+            "fwd_adj_recompute_s": round(t_rc, 4), "fwd_adj_recompute_over_fwd": round(t_rc / t_fwd, 2),
+            "kept_vs_recompute_grad_identical": bool(same),
             "fwd_mcell_steps_s": round(cells * steps / t_fwd / 1e6, 1),
             "fwd_adj_mcell_steps_s": round(cells * steps / t_tot / 1e6, 1),
             "J": float(J), "grad_norms_eta_hu_hv": [round(v, 6) for v in g]}

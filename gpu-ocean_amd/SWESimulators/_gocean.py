@@ -115,6 +115,12 @@ PROTOTYPES.extend([
      [ctypes.POINTER(Domain), ctypes.POINTER(Wind), c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
       c_void_p, ctypes.c_longlong, c_void_p]),
+    ("gocean_adjoint_record", c_int,
+     [ctypes.POINTER(Domain), ctypes.POINTER(Wind), c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, ctypes.c_longlong, c_void_p]),
+    ("gocean_adjoint_reverse", c_int,
+     [ctypes.POINTER(Domain), c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_longlong, c_void_p]),
 ])
 
 

@@ -20,6 +20,13 @@ Memory: (n / K + 1) checkpoints of 3 planes plus (16 K + 9) planes of segment
 work, plane = (ny + 4) * pitch floats.  Work: one extra forward run plus the
 adjoint kernels.
 
+With ``keep_records`` the forward itself runs the recording variant
+(``gocean_adjoint_record``, one work buffer per segment) and the backward
+only applies the substep adjoints (``gocean_adjoint_reverse``): no second
+forward run, for about 16 planes of memory per step.  ``"auto"`` (default)
+keeps the records when they take at most ``record_budget`` of the free
+device memory at the time of the forward run.
+
 Scope (enforced by the library): north = (0, 1), constant f, r == 0,
 rk_order 2, fixed dt, constant wind.  Ghost cells: an output ghost cell the
 forward rewrites (periodic images) passes its adjoint to the interior cell it
@@ -47,7 +54,8 @@ class RK2Adjoint(object):
     (``gocean_bathymetry_init``).  ``wind``: ``gocean_wind_t`` or None.
     """
 
-    def __init__(self, domain, Hi, stream=None, wind=None, checkpoint_every=50, tile_config=-1):
+    def __init__(self, domain, Hi, stream=None, wind=None, checkpoint_every=50, tile_config=-1,
+                 keep_records="auto", record_budget=0.25):
         torch = _torch()
         self.lib = G.load()
         self.d = domain
@@ -60,6 +68,11 @@ class RK2Adjoint(object):
         self.fields = G.Fields()
         self.K = max(1, int(checkpoint_every))
         self.tile_config = int(tile_config)
+        if keep_records not in (True, False, "auto"):
+            raise ValueError("keep_records must be True, False or 'auto'")
+        self.keep_records = keep_records
+        self.record_budget = float(record_budget)
+        self.records = None
         self.shape = (domain.ny + 4, domain.pitch)
         n_scratch = int(self.lib.gocean_adjoint_scratch_floats(domain))
         with torch.cuda.stream(self.stream):
@@ -116,15 +129,68 @@ class RK2Adjoint(object):
             None, None, None, G.ptr(self.Hi), self.fields, self.wind, self.tile_config,
             G.stream_handle(self.stream)), "gocean_rk_steps")
 
+    def _segments(self):
+        n, K = self.n_steps, self.K
+        return [min(K, n - s) for s in range(0, n, K)]
+
+    def records_bytes(self, n_steps):
+        """Device bytes the kept records of an ``n_steps`` run take."""
+        n, K = int(n_steps), self.K
+        return sum(4 * int(self.lib.gocean_adjoint_segment_work_floats(self.d, min(K, n - s)))
+                   for s in range(0, n, K))
+
+    def _use_records(self):
+        if self.keep_records != "auto":
+            return bool(self.keep_records)
+        free, _ = _torch().cuda.mem_get_info(self.dev)
+        return self.records_bytes(self.n_steps) <= self.record_budget * free
+
+    def _forward_records(self, q):
+        """Recording forward, one work buffer per segment; returns the final
+        state (views into the last buffer)."""
+        torch = _torch()
+        pl = self.shape[0] * self.shape[1]
+        self.records = []
+        start = q
+        for k in self._segments():
+            nw = int(self.lib.gocean_adjoint_segment_work_floats(self.d, k))
+            w = torch.empty(nw, dtype=torch.float32, device=self.dev)
+            G.check(self.lib.gocean_adjoint_record(
+                self.d, self.wind, k, self.dt, None, *(G.ptr(a) for a in start), G.ptr(self.Hi),
+                G.ptr(w), nw, G.stream_handle(self.stream)), "gocean_adjoint_record")
+            self.records.append((start, w, k))
+            o = 3 * (k - 1) * pl
+            start = tuple(w[o + f * pl:o + (f + 1) * pl].view(self.shape) for f in range(3))
+        return start
+
     # ------------------------------------------------------------------
     def forward(self, q0, n_steps, dt):
         """Run ``n_steps`` fixed-dt RK2 steps from padded (eta, hu, hv) device
-        tensors; returns new tensors and keeps the checkpoints."""
+        tensors; returns new tensors and keeps the checkpoints (or the
+        records, see ``keep_records``)."""
         torch = _torch()
         self.n_steps, self.dt = int(n_steps), float(np.float32(dt))
+        self.records = None
+        self.checkpoints = None
         cur = self._enter()
         with torch.cuda.stream(self.stream):
             q = tuple(a.detach().to(torch.float32).contiguous().clone() for a in q0)
+            if self._use_records():
+                fin = self._forward_records(q)
+                # cells a step does not write (wall ghosts, pitch padding)
+                # keep the input's values, as in the in-place forward
+                nx4 = self.d.nx + 4
+                out = []
+                for a, f in zip(q, fin):
+                    o = a.clone()
+                    o[:, :nx4] = f[:, :nx4]
+                    o.reshape(-1)[self.keep] = a.reshape(-1)[self.keep]
+                    out.append(o)
+                q = tuple(out)
+                self.checkpoints = [r[0] for r in self.records]
+                self.final = q
+                self._leave(cur, q)
+                return q
             tmp = self._alloc3()
             self.checkpoints = []
             done = 0
@@ -153,7 +219,15 @@ class RK2Adjoint(object):
             cur = tuple(lam)
             nxt = self._alloc3()
             n_seg = len(self.checkpoints)
-            if self.work is None:
+            if self.records is not None:
+                for start, w, k in reversed(self.records):
+                    G.check(self.lib.gocean_adjoint_reverse(
+                        self.d, k, self.dt, None, G.ptr(start[0]), *(G.ptr(a) for a in cur),
+                        *(G.ptr(a) for a in nxt), G.ptr(self.Hi), G.ptr(w), w.numel(),
+                        G.stream_handle(self.stream)), "gocean_adjoint_reverse")
+                    cur, nxt = nxt, cur
+                n_seg = 0
+            if self.work is None and n_seg:
                 nw = int(self.lib.gocean_adjoint_segment_work_floats(self.d, self.K))
                 self.work = torch.empty(nw, dtype=torch.float32, device=self.dev)
             for s in range(n_seg - 1, -1, -1):

@@ -187,13 +187,15 @@ struct AdjSmem {
     static constexpr int W = BW + 6, H = BH + 6;
     static constexpr int NF = (BH * W > H * BW) ? BH * W : H * BW;
     float Hi[H + 1][W + 1];
-    float E[H][W], U[H][W], V[H][W], T[H][W], S[H][W], h[H][W];
+    float E[H][W], U[H][W], V[H][W];
     uint32_t code[H][W];                 // record word; bits 28-29: conversion branch
     float W0[H][W], W1[H][W], W2[H][W];  // lambda at interior cells, clamp-masked
     float A0[H][W], A1[H][W], A2[H][W];  // gathered d/d(E, u, v) of output cells
     float FD[6][NF];                     // face -> (E, u, v) of its minus and plus cell
-    float FS[6][NF];                     // face -> (T, u slope, v slope) of minus / plus
-    float CE[9][NF];                     // slope centre -> (E, u, v) of cells -1, 0, +1
+    // face -> (T, u slope, v slope) of minus / plus in rows 0-5 until the
+    // slope-centre stage has read them, then slope centre -> (E, u, v) of
+    // cells -1, 0, +1 in rows 0-8 (written after a barrier)
+    float FC[9][NF];
 };
 
 enum { kBrLand = 0, kBrDry = 1, kBrDesing = 2, kBrWet = 3 };
@@ -248,6 +250,16 @@ struct AdjTile {
         return out_eta[o] <= -Hreg + A.cutoff;
     }
 
+    // cell depth from the four corner depths in LDS, land corners left out
+    // (the forward's operations, so h = eta + Hm is the forward's h)
+    __host__ __device__ float cell_hm(int lr, int lc) const {
+        const float a = S.Hi[lr][lc], b = S.Hi[lr + 1][lc], c = S.Hi[lr][lc + 1], d = S.Hi[lr + 1][lc + 1];
+        const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag) - (float)(c == kDryFlag)
+                          - (float)(d == kDryFlag);
+        const float sum = ((a + b) + c) + d;
+        return (wet == 4.0f) ? sum * 0.25f : (wet == 0.0f ? A.land : sum / wet);
+    }
+
     // ---- stage 0: corner depths, land flagged
     __host__ __device__ void stage_hi(int it) {
         const int lr = it / (W + 1), lc = it - lr * (W + 1);
@@ -263,7 +275,7 @@ struct AdjTile {
         const int lr = it / W, lc = it - lr * W;
         const int R = Rof(lr), C = Cof(lc);
         const int nx = A.nx, ny = A.ny;
-        float E = 0.0f, U = 0.0f, V = 0.0f, T = 0.0f, Sv = 0.0f, h = 0.0f;
+        float E = 0.0f, U = 0.0f, V = 0.0f;
         float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
         uint32_t code = 0u;
         const bool in_band = (lr >= 3 && lr < BH + 3) || (lc >= 3 && lc < BW + 3);
@@ -275,12 +287,8 @@ struct AdjTile {
             if (C < 2) sc = (A.bc_w == kWall) ? 3 - C : C + nx;
             else if (C > nx + 1) sc = (A.bc_e == kWall) ? 2 * nx + 3 - C : C - nx;
             const float eta = A.eta[(size_t)sr * A.pitch + sc];
-            const float a = S.Hi[lr][lc], b = S.Hi[lr + 1][lc], c = S.Hi[lr][lc + 1], d = S.Hi[lr + 1][lc + 1];
-            const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag) - (float)(c == kDryFlag)
-                              - (float)(d == kDryFlag);
-            const float sum = ((a + b) + c) + d;
-            const float Hm = (wet == 4.0f) ? sum * 0.25f : (wet == 0.0f ? A.land : sum / wet);
-            h = eta + Hm;
+            const float Hm = cell_hm(lr, lc);
+            const float h = eta + Hm;
             uint32_t br;
             if (fabsf(Hm - A.land) <= kDryEps) { E = kDryFlag; br = kBrLand; }
             else if (h < A.desing) {
@@ -292,14 +300,13 @@ struct AdjTile {
             V = A.tV[o];
             const bool ri = !ghost_r(R), ci = !ghost_c(C);
             code = br << 28;
-            if (ri && C >= 1 && C <= nx + 2) { T = A.tT[o]; code |= A.tC[o] & 0x001F00FFu; }
-            if (ci && R >= 1 && R <= ny + 2) { Sv = A.tS[o]; code |= A.tC[o] & 0x03E0FF00u; }
+            if (ri && C >= 1 && C <= nx + 2) code |= A.tC[o] & 0x001F00FFu;
+            if (ci && R >= 1 && R <= ny + 2) code |= A.tC[o] & 0x03E0FF00u;
             if (ri && ci) {
                 if (!clamped(A.w_clamp, lr, lc, o)) { w0 = A.w_eta[o]; w1 = A.w_hu[o]; w2 = A.w_hv[o]; }
             }
         }
-        S.E[lr][lc] = E; S.U[lr][lc] = U; S.V[lr][lc] = V; S.T[lr][lc] = T; S.S[lr][lc] = Sv;
-        S.h[lr][lc] = h; S.code[lr][lc] = code;
+        S.E[lr][lc] = E; S.U[lr][lc] = U; S.V[lr][lc] = V; S.code[lr][lc] = code;
         S.W0[lr][lc] = w0; S.W1[lr][lc] = w1; S.W2[lr][lc] = w2;
         S.A0[lr][lc] = 0.0f; S.A1[lr][lc] = 0.0f; S.A2[lr][lc] = 0.0f;
     }
@@ -331,8 +338,10 @@ struct AdjTile {
                 const float Rpx = S.U[lr][p] - 0.5f * Sup, Rpy = S.V[lr][p] - 0.5f * Svp;
                 const float Rmx = S.U[lr][m] + 0.5f * Sum, Rmy = S.V[lr][m] + 0.5f * Svm;
                 const float Hf = 0.5f * (S.Hi[lr][p] + S.Hi[lr + 1][p]);
-                const float hp_raw = (Ep + Hf) - S.T[lr][p];
-                const float hm_raw = (Em + Hf) + S.T[lr][m];
+                // face depth terms from the record: rows 2..ny+1, columns 1..nx+2
+                const size_t om = (size_t)Rof(lr) * A.pitch + cm;
+                const float hp_raw = (Ep + Hf) - A.tT[om + 1];
+                const float hm_raw = (Em + Hf) + A.tT[om];
                 const float hp = fmaxf(0.0f, hp_raw), hm = fmaxf(0.0f, hm_raw);
                 float ghm, gnm, gtm, ghp, gnp, gtp;
                 flux_vjp(hm, Rmx, Rmy, hp, Rpx, Rpy, (km >> kCodeFx) & 31u, A.g, A.cutoff, l0, l1, l2,
@@ -346,20 +355,20 @@ struct AdjTile {
             }
         }
         const int k = j * W + fi;
-        for (int q = 0; q < 6; ++q) { S.FD[q][k] = out[q]; S.FS[q][k] = out[6 + q]; }
+        for (int q = 0; q < 6; ++q) { S.FD[q][k] = out[q]; S.FC[q][k] = out[6 + q]; }
     }
 
     // slope centre c (lc = fi, row lr): gather from its two faces, apply the
     // u / v slope VJPs and the depth-term VJP (K with minmod / adjustment)
-    __host__ __device__ void stage_xcentre(int it) {
+    __host__ __device__ void stage_xcentre(int it, float (&out)[9]) const {
         const int j = it / W, fi = it - j * W;
         const int lr = j + 3, c = Cof(fi);
-        float out[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < 9; ++q) out[q] = 0.0f;
         if (x_row_ok(j) && c >= fx0 && c <= fx1 + 1 && fi >= 1 && fi + 1 < W) {
             const int kl = j * W + fi - 1, kr = j * W + fi;   // face to the left (c is plus), right (c is minus)
-            const float gT = S.FS[3][kl] + S.FS[0][kr];
-            const float gSu = S.FS[4][kl] + S.FS[1][kr];
-            const float gSv = S.FS[5][kl] + S.FS[2][kr];
+            const float gT = S.FC[3][kl] + S.FC[0][kr];
+            const float gSu = S.FC[4][kl] + S.FC[1][kr];
+            const float gSv = S.FC[5][kl] + S.FC[2][kr];
             const uint32_t code = S.code[lr][fi];
             const float th = A.theta;
             float cl, cc, cr;
@@ -376,8 +385,10 @@ struct AdjTile {
             out[0] += gE[0]; out[3] += gE[1]; out[6] += gE[2];
             out[2] += gt[0]; out[5] += gt[1]; out[8] += gt[2];
         }
-        const int k = j * W + fi;
-        for (int q = 0; q < 9; ++q) S.CE[q][k] = out[q];
+    }
+    // centre item it's results, stored once every centre has read FC
+    __host__ __device__ void store_centre(int it, const float (&out)[9]) {
+        for (int q = 0; q < 9; ++q) S.FC[q][it] = out[q];
     }
 
     __host__ __device__ void stage_xgather(int it) {
@@ -387,9 +398,9 @@ struct AdjTile {
         if (!is_output(Rof(lr), Cof(fi))) return;
         const int k = j * W + fi;
         // faces: left (cell is plus), right (cell is minus); centres -1, 0, +1
-        S.A0[lr][fi] += ((S.FD[3][k - 1] + S.FD[0][k]) + (S.CE[6][k - 1] + S.CE[3][k])) + S.CE[0][k + 1];
-        S.A1[lr][fi] += ((S.FD[4][k - 1] + S.FD[1][k]) + (S.CE[7][k - 1] + S.CE[4][k])) + S.CE[1][k + 1];
-        S.A2[lr][fi] += ((S.FD[5][k - 1] + S.FD[2][k]) + (S.CE[8][k - 1] + S.CE[5][k])) + S.CE[2][k + 1];
+        S.A0[lr][fi] += ((S.FD[3][k - 1] + S.FD[0][k]) + (S.FC[6][k - 1] + S.FC[3][k])) + S.FC[0][k + 1];
+        S.A1[lr][fi] += ((S.FD[4][k - 1] + S.FD[1][k]) + (S.FC[7][k - 1] + S.FC[4][k])) + S.FC[1][k + 1];
+        S.A2[lr][fi] += ((S.FD[5][k - 1] + S.FD[2][k]) + (S.FC[8][k - 1] + S.FC[5][k])) + S.FC[2][k + 1];
     }
 
     // ---- y pass --------------------------------------------------------
@@ -419,8 +430,10 @@ struct AdjTile {
                 const float Rpx = S.U[p][lc] - 0.5f * Sup, Rpy = S.V[p][lc] - 0.5f * Svp;
                 const float Rmx = S.U[m][lc] + 0.5f * Sum, Rmy = S.V[m][lc] + 0.5f * Svm;
                 const float Hf = 0.5f * (S.Hi[p][lc] + S.Hi[p][lc + 1]);
-                const float hp_raw = (Ep + Hf) - S.S[p][lc];
-                const float hm_raw = (Em + Hf) + S.S[m][lc];
+                // face depth terms from the record: columns 2..nx+1, rows 1..ny+2
+                const size_t om = (size_t)rm * A.pitch + Cof(lc);
+                const float hp_raw = (Ep + Hf) - A.tS[om + A.pitch];
+                const float hm_raw = (Em + Hf) + A.tS[om];
                 const float hp = fmaxf(0.0f, hp_raw), hm = fmaxf(0.0f, hm_raw);
                 float ghm, gnm, gtm, ghp, gnp, gtp;
                 flux_vjp(hm, Rmy, Rmx, hp, Rpy, Rpx, (km >> kCodeFy) & 31u, A.g, A.cutoff, l0, l1, l2,
@@ -435,18 +448,18 @@ struct AdjTile {
             }
         }
         const int k = fj * BW + i;
-        for (int q = 0; q < 6; ++q) { S.FD[q][k] = out[q]; S.FS[q][k] = out[6 + q]; }
+        for (int q = 0; q < 6; ++q) { S.FD[q][k] = out[q]; S.FC[q][k] = out[6 + q]; }
     }
 
-    __host__ __device__ void stage_ycentre(int it) {
+    __host__ __device__ void stage_ycentre(int it, float (&out)[9]) const {
         const int fj = it / BW, i = it - fj * BW;
         const int lc = i + 3, c = Rof(fj);
-        float out[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int q = 0; q < 9; ++q) out[q] = 0.0f;
         if (y_col_ok(i) && c >= fy0 && c <= fy1 + 1 && fj >= 1 && fj + 1 < H) {
             const int kl = (fj - 1) * BW + i, kr = fj * BW + i;
-            const float gT = S.FS[3][kl] + S.FS[0][kr];
-            const float gSu = S.FS[4][kl] + S.FS[1][kr];
-            const float gSv = S.FS[5][kl] + S.FS[2][kr];
+            const float gT = S.FC[3][kl] + S.FC[0][kr];
+            const float gSu = S.FC[4][kl] + S.FC[1][kr];
+            const float gSv = S.FC[5][kl] + S.FC[2][kr];
             const uint32_t code = S.code[fj][lc] >> 8;
             const float th = A.theta;
             float cl, cc, cr;
@@ -463,8 +476,6 @@ struct AdjTile {
             out[0] += gE[0]; out[3] += gE[1]; out[6] += gE[2];
             out[1] += gt[0]; out[4] += gt[1]; out[7] += gt[2];
         }
-        const int k = fj * BW + i;
-        for (int q = 0; q < 9; ++q) S.CE[q][k] = out[q];
     }
 
     __host__ __device__ void stage_ygather(int it) {
@@ -473,9 +484,9 @@ struct AdjTile {
         if (!y_col_ok(i) || fj < 1 || fj + 1 >= H) return;
         if (!is_output(Rof(fj), Cof(lc))) return;
         const int k = fj * BW + i;
-        S.A0[fj][lc] += ((S.FD[3][k - BW] + S.FD[0][k]) + (S.CE[6][k - BW] + S.CE[3][k])) + S.CE[0][k + BW];
-        S.A1[fj][lc] += ((S.FD[4][k - BW] + S.FD[1][k]) + (S.CE[7][k - BW] + S.CE[4][k])) + S.CE[1][k + BW];
-        S.A2[fj][lc] += ((S.FD[5][k - BW] + S.FD[2][k]) + (S.CE[8][k - BW] + S.CE[5][k])) + S.CE[2][k + BW];
+        S.A0[fj][lc] += ((S.FD[3][k - BW] + S.FD[0][k]) + (S.FC[6][k - BW] + S.FC[3][k])) + S.FC[0][k + BW];
+        S.A1[fj][lc] += ((S.FD[4][k - BW] + S.FD[1][k]) + (S.FC[7][k - BW] + S.FC[4][k])) + S.FC[1][k + BW];
+        S.A2[fj][lc] += ((S.FD[5][k - BW] + S.FD[2][k]) + (S.FC[8][k - BW] + S.FC[5][k])) + S.FC[2][k + BW];
     }
 
     // ---- output: base / source VJP (interior cells), conversion VJP, store
@@ -511,7 +522,8 @@ struct AdjTile {
         }
         // conversion (eta, hu, hv) -> (E, u, v)
         const uint32_t br = S.code[lr][lc] >> 28;
-        const float h = S.h[lr][lc];
+        // wet / desingularised cells have E = eta, so h = E + Hm as in the forward
+        const float h = (br == kBrWet || br == kBrDesing) ? E + cell_hm(lr, lc) : 0.0f;
         float g_eta = 0.0f, g_hu = 0.0f, g_hv = 0.0f;
         if (br == kBrWet) {
             const float rh = adj_rcp(h);
@@ -544,12 +556,15 @@ struct AdjTile {
 #define ADJ_SYNC()
 #endif
 
-// every stage of one tile, items spread over NT threads; the host build runs
-// the same body with one "thread" that loops over all items
+// every stage of one tile, items spread over nthreads == NT threads; a host
+// build runs the same body with NT = nthreads = 1, one "thread" that loops
+// over all items
 template <int BW, int BH, int NT>
 __host__ __device__ inline void adj_tile_body(const AdjTileArgs& A, AdjSmem<BW, BH>& S, int tx, int ty,
                                              int tid, int nthreads) {
     using Tl = AdjTile<BW, BH>;
+    // items per thread of the centre stages (nthreads == NT)
+    constexpr int kPer = ((Tl::kXItems > Tl::kYItems ? Tl::kXItems : Tl::kYItems) + NT - 1) / NT;
     Tl T(A, S, tx, ty);
     for (int it = tid; it < Tl::kHiItems; it += nthreads) T.stage_hi(it);
     ADJ_SYNC();
@@ -557,13 +572,29 @@ __host__ __device__ inline void adj_tile_body(const AdjTileArgs& A, AdjSmem<BW, 
     ADJ_SYNC();
     for (int it = tid; it < Tl::kXItems; it += nthreads) T.stage_xface(it);
     ADJ_SYNC();
-    for (int it = tid; it < Tl::kXItems; it += nthreads) T.stage_xcentre(it);
+    {
+        // slope centres overwrite the face rows they read: results are held
+        // in registers until every centre has read them
+        float out[kPer][9];
+        for (int r = 0; r < kPer; ++r)
+            if (tid + r * nthreads < Tl::kXItems) T.stage_xcentre(tid + r * nthreads, out[r]);
+        ADJ_SYNC();
+        for (int r = 0; r < kPer; ++r)
+            if (tid + r * nthreads < Tl::kXItems) T.store_centre(tid + r * nthreads, out[r]);
+    }
     ADJ_SYNC();
     for (int it = tid; it < Tl::kXItems; it += nthreads) T.stage_xgather(it);
     ADJ_SYNC();
     for (int it = tid; it < Tl::kYItems; it += nthreads) T.stage_yface(it);
     ADJ_SYNC();
-    for (int it = tid; it < Tl::kYItems; it += nthreads) T.stage_ycentre(it);
+    {
+        float out[kPer][9];
+        for (int r = 0; r < kPer; ++r)
+            if (tid + r * nthreads < Tl::kYItems) T.stage_ycentre(tid + r * nthreads, out[r]);
+        ADJ_SYNC();
+        for (int r = 0; r < kPer; ++r)
+            if (tid + r * nthreads < Tl::kYItems) T.store_centre(tid + r * nthreads, out[r]);
+    }
     ADJ_SYNC();
     for (int it = tid; it < Tl::kYItems; it += nthreads) T.stage_ygather(it);
     ADJ_SYNC();

@@ -189,33 +189,36 @@ long long gocean_adjoint_segment_work_floats(const gocean_domain_t* d, int n_ste
     return (16LL * n_steps + 9LL) * plane_of(d);
 }
 
-int gocean_adjoint_segment(const gocean_domain_t* d, const gocean_wind_t* wind, int n_steps, float dt,
-                           const float* dt_dev, const float* eta0, const float* hu0, const float* hv0,
-                           const float* l_eta, const float* l_hu, const float* l_hv,
-                           float* o_eta, float* o_hu, float* o_hv, const float* Hi,
-                           float* work, long long work_floats, void* stream) {
-    clear_error();
+// Work layout of a segment of k steps, plane = (ny + 4) * pitch floats:
+// Q_1..Q_k (3 planes each), Q*_0..Q*_{k-1} (3 each), branch records of the
+// 2k substeps (5 each), lambda scratch (9).
+static int seg_check(const gocean_domain_t* d, int n_steps, const void* a, const void* b, const void* c,
+                     const void* Hi, const float* work, long long work_floats) {
     if (int e = adj_validate(d)) return e;
     if (n_steps < 1) return set_error(GOCEAN_EINVAL, "n_steps must be >= 1");
-    if (!eta0 || !hu0 || !hv0 || !l_eta || !l_hu || !l_hv || !o_eta || !o_hu || !o_hv || !Hi || !work)
-        return set_error(GOCEAN_EINVAL, "NULL argument");
+    if (!a || !b || !c || !Hi || !work) return set_error(GOCEAN_EINVAL, "NULL argument");
     if (work_floats < gocean_adjoint_segment_work_floats(d, n_steps))
         return set_error(GOCEAN_EINVAL, "work buffer too small: %lld floats, need %lld", work_floats,
                          gocean_adjoint_segment_work_floats(d, n_steps));
+    return 0;
+}
+
+int gocean_adjoint_record(const gocean_domain_t* d, const gocean_wind_t* wind, int n_steps, float dt,
+                          const float* dt_dev, const float* eta0, const float* hu0, const float* hv0,
+                          const float* Hi, float* work, long long work_floats, void* stream) {
+    clear_error();
+    if (int e = seg_check(d, n_steps, eta0, hu0, hv0, Hi, work, work_floats)) return e;
     hipStream_t s = (hipStream_t)stream;
     const long long pl = plane_of(d);
     const int k = n_steps;
-    // layout: Q_1..Q_k (3 planes each), Q*_0..Q*_{k-1} (3), records 2k (5), lambda x 3 (3)
     float* Qb = work;
     float* Qs = work + 3LL * k * pl;
     float* Tb = work + 6LL * k * pl;
-    float* Lb = work + 16LL * k * pl;
-    auto Q = [&](int i) -> const float* { return i == 0 ? eta0 : Qb + 3LL * (i - 1) * pl; };
     auto Qh = [&](int i, int f) -> const float* {
         if (i == 0) return f == 0 ? eta0 : (f == 1 ? hu0 : hv0);
         return Qb + (3LL * (i - 1) + f) * pl;
     };
-    // recompute: Q*_i = P(Q_i), Q_{i+1} = (Q_i + P(Q*_i)) / 2, with records
+    // Q*_i = P(Q_i), Q_{i+1} = (Q_i + P(Q*_i)) / 2, with the records
     for (int i = 0; i < k; ++i) {
         float* qs = Qs + 3LL * i * pl;
         float* qn = Qb + 3LL * i * pl;       // Q_{i+1}
@@ -226,7 +229,24 @@ int gocean_adjoint_segment(const gocean_domain_t* d, const gocean_wind_t* wind, 
                              qn + pl, qn + 2 * pl, Hi, tape_at(Tb + (10LL * i + 5) * pl, pl), s))
             return e;
     }
-    // reverse
+    return 0;
+}
+
+int gocean_adjoint_reverse(const gocean_domain_t* d, int n_steps, float dt, const float* dt_dev,
+                           const float* eta0, const float* l_eta, const float* l_hu, const float* l_hv,
+                           float* o_eta, float* o_hu, float* o_hv, const float* Hi, float* work,
+                           long long work_floats, void* stream) {
+    clear_error();
+    if (int e = seg_check(d, n_steps, eta0, l_eta, o_eta, Hi, work, work_floats)) return e;
+    if (!l_hu || !l_hv || !o_hu || !o_hv) return set_error(GOCEAN_EINVAL, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    const long long pl = plane_of(d);
+    const int k = n_steps;
+    float* Qb = work;
+    float* Qs = work + 3LL * k * pl;
+    float* Tb = work + 6LL * k * pl;
+    float* Lb = work + 16LL * k * pl;
+    auto Q = [&](int i) -> const float* { return i == 0 ? eta0 : Qb + 3LL * (i - 1) * pl; };
     float* lam_s = Lb;
     float* lam_a = Lb + 3 * pl;
     float* lam_b = Lb + 6 * pl;
@@ -250,6 +270,18 @@ int gocean_adjoint_segment(const gocean_domain_t* d, const gocean_wind_t* wind, 
         le = oe; lu = ou; lv = ov;
     }
     return 0;
+}
+
+int gocean_adjoint_segment(const gocean_domain_t* d, const gocean_wind_t* wind, int n_steps, float dt,
+                           const float* dt_dev, const float* eta0, const float* hu0, const float* hv0,
+                           const float* l_eta, const float* l_hu, const float* l_hv,
+                           float* o_eta, float* o_hu, float* o_hv, const float* Hi,
+                           float* work, long long work_floats, void* stream) {
+    if (int e = gocean_adjoint_record(d, wind, n_steps, dt, dt_dev, eta0, hu0, hv0, Hi, work, work_floats,
+                                      stream))
+        return e;
+    return gocean_adjoint_reverse(d, n_steps, dt, dt_dev, eta0, l_eta, l_hu, l_hv, o_eta, o_hu, o_hv, Hi, work,
+                                  work_floats, stream);
 }
 
 }  // extern "C"

@@ -274,6 +274,30 @@ int gocean_adjoint_segment(const gocean_domain_t* d, const gocean_wind_t* wind, 
                            float* o_eta, float* o_hu, float* o_hv, const float* Hi,
                            float* work, long long work_floats, void* stream);
 
+/* The two halves of gocean_adjoint_segment, for callers that keep the
+ * records of the forward run instead of recomputing them in the reverse
+ * run.  No reference counterpart (the reference has no reverse mode).
+ *
+ * gocean_adjoint_record: n_steps recording RK2 steps from (eta0, hu0, hv0)
+ * into `work`.  Work layout, plane = (ny + 4) * pitch floats: Q_1..Q_n
+ * (eta, hu, hv planes each; Q_n is the state after the segment, the same
+ * bits as gocean_rk_steps on the cells a step writes: interior and periodic
+ * ghost cells; wall ghost cells and pitch padding are left unwritten),
+ * Q*_0..Q*_{n-1} (3 planes each), the branch
+ * records of the 2n substeps (5 planes each), 9 planes of lambda scratch.
+ *
+ * gocean_adjoint_reverse: lambda at the segment's end (l_*) to lambda at its
+ * start (o_*) from a work buffer that gocean_adjoint_record filled for the
+ * same n_steps, dt, start state (eta0 is read) and bathymetry.  The records
+ * stay valid, so the reverse run may be repeated for other l_*. */
+int gocean_adjoint_record(const gocean_domain_t* d, const gocean_wind_t* wind, int n_steps, float dt,
+                          const float* dt_dev, const float* eta0, const float* hu0, const float* hv0,
+                          const float* Hi, float* work, long long work_floats, void* stream);
+int gocean_adjoint_reverse(const gocean_domain_t* d, int n_steps, float dt, const float* dt_dev,
+                           const float* eta0, const float* l_eta, const float* l_hu, const float* l_hv,
+                           float* o_eta, float* o_hu, float* o_hv, const float* Hi, float* work,
+                           long long work_floats, void* stream);
+
 /* ------------------------------------------------------------------------
  * Model error: SOAR-correlated eta perturbation with geostrophically
  * balanced (hu, hv), added to the state (OceanStateNoise,

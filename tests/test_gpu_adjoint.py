@@ -400,6 +400,39 @@ def test_adjoint_ghost_cells(bcname, bc):
                     assert np.array_equal(a, expect), (R, C, fld)
 
 
+@pytest.mark.parametrize("bcname,bc", BCS)
+@pytest.mark.parametrize("kind", ["plain", "land"])
+def test_kept_records_match_recompute(bcname, bc, kind):
+    """Records kept from the forward run (gocean_adjoint_record /
+    gocean_adjoint_reverse) give the same bits as the checkpointed run that
+    recomputes them: forward state (every cell, wall ghosts included) and
+    gradient, for an uneven last segment; a second backward over the same
+    records gives the same gradient again."""
+    _say("kept records vs recompute", bcname, kind)
+    Common, G, A = _mods()
+    nx, ny, dt, n = 37, 29, 3.0, 7
+    p, Hi, eta, hu, hv = _case(nx, ny, bc, 13, kind)
+    rng = np.random.default_rng(5)
+    W = [_dev(w) for w in _weights(rng, nx, ny)]
+    d = _domain(p)
+    res = {}
+    for keep in (False, True):
+        adj = A.RK2Adjoint(d, _dev(Hi), checkpoint_every=3, keep_records=keep)
+        qd = [_dev(a).requires_grad_(True) for a in (eta, hu, hv)]
+        out = A.CDKLM16RK2.apply(*qd, adj, n, dt)
+        assert (adj.records is not None) == keep
+        L = sum((w * o).sum() for w, o in zip(W, out))
+        L.backward()
+        lam = adj.backward(tuple(w.clone() for w in W))
+        torch.cuda.synchronize()
+        res[keep] = ([o.detach().cpu().numpy() for o in out], [x.grad.cpu().numpy() for x in qd],
+                     [x.cpu().numpy() for x in lam])
+    for a, b in zip(res[False][0], res[True][0]):
+        assert np.array_equal(a, b)
+    for a, b, c in zip(res[False][1], res[True][1], res[True][2]):
+        assert np.array_equal(a, b) and np.array_equal(b, c)
+
+
 def test_adjoint_rejects_out_of_scope_domains():
     Common, G, A = _mods()
     p, Hi, eta, hu, hv = _case(12, 10, (1, 1, 1, 1), 1, "plain")
