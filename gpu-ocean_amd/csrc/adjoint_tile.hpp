@@ -39,6 +39,10 @@ struct AdjTileArgs {
     const float* Hi;
     const float* dt_dev;
     float dt, alpha, gamma;
+    // w / x hold the ghost-cell entries of an adjoint output whose fold has
+    // not run: the fold's sums are formed while reading (same order, so the
+    // same bits as folding first)
+    int w_fold, x_fold;
     int nx, ny, pitch, hi_pitch;
     float dx, dy, g, theta, desing, flux_eps, cutoff, land, f;
     int bc_n, bc_e, bc_s, bc_w;
@@ -189,8 +193,11 @@ struct AdjSmem {
     float Hi[H + 1][W + 1];
     float E[H][W], U[H][W], V[H][W];
     uint32_t code[H][W];                 // record word; bits 28-29: conversion branch
-    float W0[H][W], W1[H][W], W2[H][W];  // lambda at interior cells, clamp-masked
-    float A0[H][W], A1[H][W], A2[H][W];  // gathered d/d(E, u, v) of output cells
+    // lambda at interior cells (clamp-masked) and the gathered d/d(E, u, v)
+    // of output cells, only read inside the outer ring: cell (lr, lc) at
+    // [lr - 1][lc - 1]
+    float W0[H - 2][W - 2], W1[H - 2][W - 2], W2[H - 2][W - 2];
+    float A0[H - 2][W - 2], A1[H - 2][W - 2], A2[H - 2][W - 2];
     float FD[6][NF];                     // face -> (E, u, v) of its minus and plus cell
     // face -> (T, u slope, v slope) of minus / plus in rows 0-5 until the
     // slope-centre stage has read them, then slope centre -> (E, u, v) of
@@ -250,6 +257,32 @@ struct AdjTile {
         return out_eta[o] <= -Hreg + A.cutoff;
     }
 
+    // a[R, C] of an interior cell plus the ghost entries whose image it is,
+    // summed as fold_cell sums them (field 1 = hu, 2 = hv: the normal
+    // momentum of a wall image is negated)
+    __host__ __device__ float fold_read(const float* a, int field, int R, int C) const {
+        const int nx = A.nx, ny = A.ny;
+        const size_t pitch = (size_t)A.pitch;
+        float s = a[R * pitch + C];
+        if (R <= 3) {
+            if (A.bc_s == kWall) { const float g = a[(3 - R) * pitch + C]; s += field == 2 ? -g : g; }
+            else s += a[(R + ny) * pitch + C];
+        }
+        if (R >= ny) {
+            if (A.bc_n == kWall) { const float g = a[(2 * ny + 3 - R) * pitch + C]; s += field == 2 ? -g : g; }
+            else s += a[(R - ny) * pitch + C];
+        }
+        if (C <= 3) {
+            if (A.bc_w == kWall) { const float g = a[R * pitch + 3 - C]; s += field == 1 ? -g : g; }
+            else s += a[R * pitch + C + nx];
+        }
+        if (C >= nx) {
+            if (A.bc_e == kWall) { const float g = a[R * pitch + 2 * nx + 3 - C]; s += field == 1 ? -g : g; }
+            else s += a[R * pitch + C - nx];
+        }
+        return s;
+    }
+
     // cell depth from the four corner depths in LDS, land corners left out
     // (the forward's operations, so h = eta + Hm is the forward's h)
     __host__ __device__ float cell_hm(int lr, int lc) const {
@@ -303,12 +336,19 @@ struct AdjTile {
             if (ri && C >= 1 && C <= nx + 2) code |= A.tC[o] & 0x001F00FFu;
             if (ci && R >= 1 && R <= ny + 2) code |= A.tC[o] & 0x03E0FF00u;
             if (ri && ci) {
-                if (!clamped(A.w_clamp, lr, lc, o)) { w0 = A.w_eta[o]; w1 = A.w_hu[o]; w2 = A.w_hv[o]; }
+                if (!clamped(A.w_clamp, lr, lc, o)) {
+                    if (A.w_fold) {
+                        w0 = fold_read(A.w_eta, 0, R, C); w1 = fold_read(A.w_hu, 1, R, C);
+                        w2 = fold_read(A.w_hv, 2, R, C);
+                    } else { w0 = A.w_eta[o]; w1 = A.w_hu[o]; w2 = A.w_hv[o]; }
+                }
             }
         }
         S.E[lr][lc] = E; S.U[lr][lc] = U; S.V[lr][lc] = V; S.code[lr][lc] = code;
-        S.W0[lr][lc] = w0; S.W1[lr][lc] = w1; S.W2[lr][lc] = w2;
-        S.A0[lr][lc] = 0.0f; S.A1[lr][lc] = 0.0f; S.A2[lr][lc] = 0.0f;
+        if (lr >= 1 && lr + 1 < H && lc >= 1 && lc + 1 < W) {
+            S.W0[lr - 1][lc - 1] = w0; S.W1[lr - 1][lc - 1] = w1; S.W2[lr - 1][lc - 1] = w2;
+            S.A0[lr - 1][lc - 1] = 0.0f; S.A1[lr - 1][lc - 1] = 0.0f; S.A2[lr - 1][lc - 1] = 0.0f;
+        }
     }
     static constexpr int kCellItems = H * W;
 
@@ -327,9 +367,9 @@ struct AdjTile {
             if (!(Em == kDryFlag || Ep == kDryFlag)) {
                 const uint32_t km = S.code[lr][m], kp = S.code[lr][p];
                 const float rdx = dt * A.r_dx;
-                const float l0 = (S.W0[lr][p] - S.W0[lr][m]) * rdx;
-                const float l1 = (S.W1[lr][p] - S.W1[lr][m]) * rdx;
-                const float l2 = (S.W2[lr][p] - S.W2[lr][m]) * rdx;
+                const float l0 = (S.W0[lr - 1][p - 1] - S.W0[lr - 1][m - 1]) * rdx;
+                const float l1 = (S.W1[lr - 1][p - 1] - S.W1[lr - 1][m - 1]) * rdx;
+                const float l2 = (S.W2[lr - 1][p - 1] - S.W2[lr - 1][m - 1]) * rdx;
                 const float th = A.theta;
                 const float Sum = slope_val(km & 3u, S.U[lr][m - 1], S.U[lr][m], S.U[lr][m + 1], th);
                 const float Svm = slope_val((km >> 2) & 3u, S.V[lr][m - 1], S.V[lr][m], S.V[lr][m + 1], th);
@@ -398,9 +438,9 @@ struct AdjTile {
         if (!is_output(Rof(lr), Cof(fi))) return;
         const int k = j * W + fi;
         // faces: left (cell is plus), right (cell is minus); centres -1, 0, +1
-        S.A0[lr][fi] += ((S.FD[3][k - 1] + S.FD[0][k]) + (S.FC[6][k - 1] + S.FC[3][k])) + S.FC[0][k + 1];
-        S.A1[lr][fi] += ((S.FD[4][k - 1] + S.FD[1][k]) + (S.FC[7][k - 1] + S.FC[4][k])) + S.FC[1][k + 1];
-        S.A2[lr][fi] += ((S.FD[5][k - 1] + S.FD[2][k]) + (S.FC[8][k - 1] + S.FC[5][k])) + S.FC[2][k + 1];
+        S.A0[lr - 1][fi - 1] += ((S.FD[3][k - 1] + S.FD[0][k]) + (S.FC[6][k - 1] + S.FC[3][k])) + S.FC[0][k + 1];
+        S.A1[lr - 1][fi - 1] += ((S.FD[4][k - 1] + S.FD[1][k]) + (S.FC[7][k - 1] + S.FC[4][k])) + S.FC[1][k + 1];
+        S.A2[lr - 1][fi - 1] += ((S.FD[5][k - 1] + S.FD[2][k]) + (S.FC[8][k - 1] + S.FC[5][k])) + S.FC[2][k + 1];
     }
 
     // ---- y pass --------------------------------------------------------
@@ -419,9 +459,9 @@ struct AdjTile {
                 const uint32_t km = S.code[m][lc], kp = S.code[p][lc];
                 const float rdy = dt * A.r_dy;
                 // flux order (mass, normal = v momentum, tangential = u momentum)
-                const float l0 = (S.W0[p][lc] - S.W0[m][lc]) * rdy;
-                const float l1 = (S.W2[p][lc] - S.W2[m][lc]) * rdy;
-                const float l2 = (S.W1[p][lc] - S.W1[m][lc]) * rdy;
+                const float l0 = (S.W0[p - 1][lc - 1] - S.W0[m - 1][lc - 1]) * rdy;
+                const float l1 = (S.W2[p - 1][lc - 1] - S.W2[m - 1][lc - 1]) * rdy;
+                const float l2 = (S.W1[p - 1][lc - 1] - S.W1[m - 1][lc - 1]) * rdy;
                 const float th = A.theta;
                 const float Sum = slope_val((km >> 8) & 3u, S.U[m - 1][lc], S.U[m][lc], S.U[m + 1][lc], th);
                 const float Svm = slope_val((km >> 10) & 3u, S.V[m - 1][lc], S.V[m][lc], S.V[m + 1][lc], th);
@@ -484,9 +524,9 @@ struct AdjTile {
         if (!y_col_ok(i) || fj < 1 || fj + 1 >= H) return;
         if (!is_output(Rof(fj), Cof(lc))) return;
         const int k = fj * BW + i;
-        S.A0[fj][lc] += ((S.FD[3][k - BW] + S.FD[0][k]) + (S.FC[6][k - BW] + S.FC[3][k])) + S.FC[0][k + BW];
-        S.A1[fj][lc] += ((S.FD[4][k - BW] + S.FD[1][k]) + (S.FC[7][k - BW] + S.FC[4][k])) + S.FC[1][k + BW];
-        S.A2[fj][lc] += ((S.FD[5][k - BW] + S.FD[2][k]) + (S.FC[8][k - BW] + S.FC[5][k])) + S.FC[2][k + BW];
+        S.A0[fj - 1][lc - 1] += ((S.FD[3][k - BW] + S.FD[0][k]) + (S.FC[6][k - BW] + S.FC[3][k])) + S.FC[0][k + BW];
+        S.A1[fj - 1][lc - 1] += ((S.FD[4][k - BW] + S.FD[1][k]) + (S.FC[7][k - BW] + S.FC[4][k])) + S.FC[1][k + BW];
+        S.A2[fj - 1][lc - 1] += ((S.FD[5][k - BW] + S.FD[2][k]) + (S.FC[8][k - BW] + S.FC[5][k])) + S.FC[2][k + BW];
     }
 
     // ---- output: base / source VJP (interior cells), conversion VJP, store
@@ -494,13 +534,13 @@ struct AdjTile {
         const int lr = it / W, lc = it - lr * W;
         const int R = Rof(lr), C = Cof(lc);
         if (lr < 1 || lc < 1 || lr + 1 >= H || lc + 1 >= W || !is_output(R, C)) return;
-        float gE = S.A0[lr][lc], gU = S.A1[lr][lc], gV = S.A2[lr][lc];
+        float gE = S.A0[lr - 1][lc - 1], gU = S.A1[lr - 1][lc - 1], gV = S.A2[lr - 1][lc - 1];
         const size_t o = (size_t)R * A.pitch + C;
         const bool interior = !ghost_r(R) && !ghost_c(C);
         const float E = S.E[lr][lc], U = S.U[lr][lc], V = S.V[lr][lc];
         float xe = 0.0f, xu = 0.0f, xv = 0.0f;
         if (interior) {
-            const float w0 = S.W0[lr][lc], w1 = S.W1[lr][lc], w2 = S.W2[lr][lc];
+            const float w0 = S.W0[lr - 1][lc - 1], w1 = S.W1[lr - 1][lc - 1], w2 = S.W2[lr - 1][lc - 1];
             const float a = S.Hi[lr][lc], b = S.Hi[lr + 1][lc], c = S.Hi[lr][lc + 1], d = S.Hi[lr + 1][lc + 1];
             const float Hreg = 0.25f * (((a + b) + c) + d);
             const float hreg = E + Hreg;
@@ -518,7 +558,12 @@ struct AdjTile {
                 gU += ghu * hreg; gE += ghu * U;
                 gV += ghv * hreg; gE += ghv * V;
             }
-            if (A.x_eta && !clamped(A.x_clamp, lr, lc, o)) { xe = A.x_eta[o]; xu = A.x_hu[o]; xv = A.x_hv[o]; }
+            if (A.x_eta && !clamped(A.x_clamp, lr, lc, o)) {
+                if (A.x_fold) {
+                    xe = fold_read(A.x_eta, 0, R, C); xu = fold_read(A.x_hu, 1, R, C);
+                    xv = fold_read(A.x_hv, 2, R, C);
+                } else { xe = A.x_eta[o]; xu = A.x_hu[o]; xv = A.x_hv[o]; }
+            }
         }
         // conversion (eta, hu, hv) -> (E, u, v)
         const uint32_t br = S.code[lr][lc] >> 28;
@@ -601,7 +646,7 @@ __host__ __device__ inline void adj_tile_body(const AdjTileArgs& A, AdjSmem<BW, 
     for (int it = tid; it < Tl::kCellItems; it += nthreads) T.stage_out(it);
 }
 
-constexpr int kAdjBW = 32, kAdjBH = 16, kAdjNT = 512;
+constexpr int kAdjBW = 16, kAdjBH = 16, kAdjNT = 512;
 
 // Ghost cells back to the interior cells they are images of.  One thread per
 // interior cell within two cells of the boundary; ghost entries are read by

@@ -5,7 +5,9 @@
 //     record (u, v, the x / y face depth terms, and a code word with every
 //     limiter / adjustment / wave-speed choice) for each cell;
 //   * k_adj_tile: one workgroup per output tile (adjoint_tile.hpp);
-//   * k_adj_fold: ghost-cell adjoints back to their interior images.
+//   * k_adj_fold: ghost-cell adjoints back to their interior images, after
+//     the last substep of a call only: a substep adjoint that reads another
+//     one's output folds the ghost entries in while reading.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -82,9 +84,10 @@ static int fwd_tape(const gocean_domain_t* d, const gocean_wind_t* wind, float d
     return launch_substep_tape(A, s);
 }
 
-static int adj_launch(const AdjTileArgs& A, hipStream_t s) {
+static int adj_launch(const AdjTileArgs& A, bool fold_out, hipStream_t s) {
     const int nb = A.tiles_x * A.tiles_y;
     hipLaunchKernelGGL(k_adj_tile, dim3(nb), dim3(kAdjNT), 0, s, A);
+    if (!fold_out) return check_launch("adjoint substep launch");
     FoldArgs F;
     F.e = A.o_eta; F.u = A.o_hu; F.v = A.o_hv;
     F.nx = A.nx; F.ny = A.ny; F.pitch = A.pitch;
@@ -94,12 +97,15 @@ static int adj_launch(const AdjTileArgs& A, hipStream_t s) {
     return check_launch("adjoint substep launch");
 }
 
-// adjoint of one substep at the state whose eta is e_in, with its record T
+// adjoint of one substep at the state whose eta is e_in, with its record T;
+// w_fold / x_fold: w / x is an unfolded output of another substep adjoint;
+// fold_out: fold the ghost entries of this one's output (else left for the
+// reader)
 static int adj_substep(const gocean_domain_t* d, float dt, const float* dt_dev, const float* e_in,
                        const Tape& T, const float* w_e, const float* w_u, const float* w_v,
                        const float* w_clamp, const float* x_e, const float* x_u, const float* x_v,
                        const float* x_clamp, float alpha, float gamma, float* o_e, float* o_u, float* o_v,
-                       const float* Hi, hipStream_t s) {
+                       const float* Hi, int w_fold, int x_fold, bool fold_out, hipStream_t s) {
     AdjTileArgs A;
     adj_fill(d, A);
     A.eta = e_in;
@@ -108,7 +114,8 @@ static int adj_substep(const gocean_domain_t* d, float dt, const float* dt_dev, 
     A.x_eta = x_e; A.x_hu = x_u; A.x_hv = x_v; A.x_clamp = x_clamp;
     A.o_eta = o_e; A.o_hu = o_u; A.o_hv = o_v;
     A.Hi = Hi; A.dt = dt; A.dt_dev = dt_dev; A.alpha = alpha; A.gamma = gamma;
-    return adj_launch(A, s);
+    A.w_fold = w_fold; A.x_fold = x_fold;
+    return adj_launch(A, fold_out, s);
 }
 
 static long long plane_of(const gocean_domain_t* d) { return (long long)(d->ny + 4) * d->pitch; }
@@ -145,7 +152,7 @@ int gocean_adjoint_substep(const gocean_domain_t* d, const gocean_wind_t* wind, 
                          fo + 2 * pl, Hi, T, s))
         return e;
     return adj_substep(d, dt, dt_dev, eta, T, w_eta, w_hu, w_hv, w_clamp ? w_clamp : fo, x_eta, x_hu,
-                       x_hv, x_clamp, alpha, gamma, o_eta, o_hu, o_hv, Hi, s);
+                       x_hv, x_clamp, alpha, gamma, o_eta, o_hu, o_hv, Hi, 0, 0, true, s);
 }
 
 int gocean_adjoint_rk2_step(const gocean_domain_t* d, const gocean_wind_t* wind, float dt,
@@ -177,11 +184,11 @@ int gocean_adjoint_rk2_step(const gocean_domain_t* d, const gocean_wind_t* wind,
     const float* clamp_next = eta_next ? eta_next : qn1;
     // lambda* = 0.5 * J_P(Q*)^T lambda^{n+1}
     if (int e = adj_substep(d, dt, dt_dev, eta_s, T1, l_eta, l_hu, l_hv, clamp_next, nullptr, nullptr,
-                            nullptr, nullptr, 0.5f, 0.0f, ls, ls + pl, ls + 2 * pl, Hi, s))
+                            nullptr, nullptr, 0.5f, 0.0f, ls, ls + pl, ls + 2 * pl, Hi, 0, 0, false, s))
         return e;
     // lambda^n = J_P(Q^n)^T lambda* + 0.5 * lambda^{n+1}
     return adj_substep(d, dt, dt_dev, eta_n, T0, ls, ls + pl, ls + 2 * pl, qs, l_eta, l_hu, l_hv,
-                       clamp_next, 1.0f, 0.5f, o_eta, o_hu, o_hv, Hi, s);
+                       clamp_next, 1.0f, 0.5f, o_eta, o_hu, o_hv, Hi, 1, 0, true, s);
 }
 
 long long gocean_adjoint_segment_work_floats(const gocean_domain_t* d, int n_steps) {
@@ -260,12 +267,16 @@ int gocean_adjoint_reverse(const gocean_domain_t* d, int n_steps, float dt, cons
             float* dst = (le == lam_a) ? lam_b : lam_a;
             oe = dst; ou = dst + pl; ov = dst + 2 * pl;
         }
+        // lambda^{i+1} is the caller's (folded) for i = k-1, else this
+        // loop's unfolded output; only the segment's result is folded
+        const int l_unfolded = i < k - 1;
         if (int e = adj_substep(d, dt, dt_dev, qs, tape_at(Tb + (10LL * i + 5) * pl, pl), le, lu, lv,
                                 clamp_next, nullptr, nullptr, nullptr, nullptr, 0.5f, 0.0f, lam_s,
-                                lam_s + pl, lam_s + 2 * pl, Hi, s))
+                                lam_s + pl, lam_s + 2 * pl, Hi, l_unfolded, 0, false, s))
             return e;
         if (int e = adj_substep(d, dt, dt_dev, Q(i), tape_at(Tb + 10LL * i * pl, pl), lam_s, lam_s + pl,
-                                lam_s + 2 * pl, qs, le, lu, lv, clamp_next, 1.0f, 0.5f, oe, ou, ov, Hi, s))
+                                lam_s + 2 * pl, qs, le, lu, lv, clamp_next, 1.0f, 0.5f, oe, ou, ov, Hi,
+                                1, l_unfolded, i == 0, s))
             return e;
         le = oe; lu = ou; lv = ov;
     }

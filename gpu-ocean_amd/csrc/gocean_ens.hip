@@ -210,7 +210,7 @@ __device__ __forceinline__ long long ens_off(const EnsMember& M, const EnsRef& r
     return r.in_work ? M.work_off + (long long)r.idx * M.plane : M.state_off;
 }
 
-constexpr int kEnsTapeBW = 32, kEnsTapeBH = 16, kEnsTapeNT = 256;
+constexpr int kEnsTapeBW = 32, kEnsTapeBH = 16, kEnsTapeNT = 512;
 
 struct EnsTapeLaunch {
     const EnsMember* members;
@@ -261,6 +261,7 @@ struct EnsAdjLaunch {
     const float* Hi;
     const float* dt;
     float alpha, gamma;
+    int w_fold, x_fold;                            // as AdjTileArgs
 };
 
 __global__ __launch_bounds__(kAdjNT) void k_ens_adj_tile(EnsAdjLaunch L) {
@@ -289,6 +290,7 @@ __global__ __launch_bounds__(kAdjNT) void k_ens_adj_tile(EnsAdjLaunch L) {
     A.Hi = L.Hi + M.hi_off;
     A.dt_dev = L.dt + m; A.dt = 0.0f;
     A.alpha = L.alpha; A.gamma = L.gamma;
+    A.w_fold = L.w_fold; A.x_fold = L.x_fold;
     A.nx = M.nx; A.ny = M.ny; A.pitch = M.pitch; A.hi_pitch = M.hi_pitch;
     A.dx = M.dx; A.dy = M.dy; A.g = M.g; A.theta = M.theta; A.desing = M.desing;
     A.flux_eps = M.flux_eps; A.cutoff = M.cutoff; A.land = M.land; A.f = M.f;
@@ -670,14 +672,16 @@ int gocean_ensemble_adjoint_segment(gocean_ensemble_t* E, int n_steps, const flo
         hipLaunchKernelGGL(k_ens_substep_tape, dim3(E->n_ftiles), dim3(kEnsTapeNT), 0, s, L);
         return check_launch("ensemble recording substep launch");
     };
+    // w_fold / x_fold: w / x is an unfolded output of another substep
+    // adjoint; fold_out: fold this one's output (the segment's result only)
     auto adj = [&](EnsRef eta, EnsRef tape, EnsRef w, EnsRef wc, EnsRef x, EnsRef xc, float alpha,
-                   float gamma, EnsRef o) {
+                   float gamma, EnsRef o, int w_fold, int x_fold, bool fold_out) {
         EnsAdjLaunch L;
         L.members = E->d_members; L.tile_member = E->d_atile_member;
         L.eta = eta; L.tape = tape; L.w = w; L.w_clamp = wc; L.x = x; L.x_clamp = xc; L.o = o;
-        L.Hi = Hi; L.dt = dt; L.alpha = alpha; L.gamma = gamma;
+        L.Hi = Hi; L.dt = dt; L.alpha = alpha; L.gamma = gamma; L.w_fold = w_fold; L.x_fold = x_fold;
         hipLaunchKernelGGL(k_ens_adj_tile, dim3(E->n_atiles), dim3(kAdjNT), 0, s, L);
-        hipLaunchKernelGGL(k_ens_adj_fold, dim3(gx, 9, E->n), dim3(256), 0, s, E->d_members, o);
+        if (fold_out) hipLaunchKernelGGL(k_ens_adj_fold, dim3(gx, 9, E->n), dim3(256), 0, s, E->d_members, o);
         return check_launch("ensemble adjoint launch");
     };
     for (int i = 0; i < k; ++i) {
@@ -692,8 +696,11 @@ int gocean_ensemble_adjoint_segment(gocean_ensemble_t* E, int n_steps, const flo
         EnsRef o;
         if (i == 0) o = F(lam_out);
         else o = (le.in_work && le.idx == lam_a.idx) ? lam_b : lam_a;
-        if (int e = adj(Qs(i), Tp(i, 1), le, next, none, none, 0.5f, 0.0f, lam_s)) return e;
-        if (int e = adj(Q(i), Tp(i, 0), lam_s, Qs(i), le, next, 1.0f, 0.5f, o)) return e;
+        const int l_unfolded = i < k - 1;
+        if (int e = adj(Qs(i), Tp(i, 1), le, next, none, none, 0.5f, 0.0f, lam_s, l_unfolded, 0, false))
+            return e;
+        if (int e = adj(Q(i), Tp(i, 0), lam_s, Qs(i), le, next, 1.0f, 0.5f, o, 1, l_unfolded, i == 0))
+            return e;
         le = o;
     }
     return 0;

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(NT) void k_substep_tape(SubstepArgs A) {
 }
 
 int launch_substep_tape(SubstepArgs A, hipStream_t s) {
-    constexpr int BW = 32, BH = 16, NT = 256;
+    constexpr int BW = 32, BH = 16, NT = 512;
     A.tiles_x = (A.nx + BW - 1) / BW;
     A.tiles_y = (A.ny + BH - 1) / BH;
     hipLaunchKernelGGL((k_substep_tape<BW, BH, NT>), dim3(A.tiles_x * A.tiles_y), dim3(NT), 0, s, A);
