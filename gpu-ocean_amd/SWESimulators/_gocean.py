@@ -110,6 +110,11 @@ PROTOTYPES.extend([
     ("gocean_ensemble_adjoint_segment", c_int,
      [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
       ctypes.c_longlong, c_void_p]),
+    ("gocean_ensemble_adjoint_record", c_int,
+     [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_longlong, c_void_p, c_void_p]),
+    ("gocean_ensemble_adjoint_reverse", c_int,
+     [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+      ctypes.c_longlong, c_void_p]),
     ("gocean_adjoint_segment_work_floats", ctypes.c_longlong, [ctypes.POINTER(Domain), c_int]),
     ("gocean_adjoint_segment", c_int,
      [ctypes.POINTER(Domain), ctypes.POINTER(Wind), c_int, c_float, c_void_p, c_void_p, c_void_p, c_void_p,

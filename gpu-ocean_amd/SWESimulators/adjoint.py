@@ -25,7 +25,8 @@ With ``keep_records`` the forward itself runs the recording variant
 only applies the substep adjoints (``gocean_adjoint_reverse``): no second
 forward run, for about 16 planes of memory per step.  ``"auto"`` (default)
 keeps the records when they take at most ``record_budget`` of the free
-device memory at the time of the forward run.
+device memory (free, plus what torch's allocator holds unused) at the time
+of the forward run.
 
 Scope (enforced by the library): north = (0, 1), constant f, r == 0,
 rk_order 2, fixed dt, constant wind.  Ghost cells: an output ghost cell the
@@ -44,6 +45,14 @@ from . import _gocean as G
 
 def _torch():
     return Common._torch()
+
+
+def _available_bytes(dev):
+    """Device memory a new allocation can use: free in the driver plus what
+    torch's allocator holds in reserve (cached blocks not in use)."""
+    torch = _torch()
+    free, _ = torch.cuda.mem_get_info(dev)
+    return free + torch.cuda.memory_reserved(dev) - torch.cuda.memory_allocated(dev)
 
 
 class RK2Adjoint(object):
@@ -142,26 +151,32 @@ class RK2Adjoint(object):
     def _use_records(self):
         if self.keep_records != "auto":
             return bool(self.keep_records)
-        free, _ = _torch().cuda.mem_get_info(self.dev)
-        return self.records_bytes(self.n_steps) <= self.record_budget * free
+        return self.records_bytes(self.n_steps) <= self.record_budget * _available_bytes(self.dev)
 
     def _forward_records(self, q):
-        """Recording forward, one work buffer per segment; returns the final
-        state (views into the last buffer)."""
+        """Recording forward from the (3, ny + 4, pitch) state ``q``, one work
+        buffer per segment; returns the final state as three new tensors.
+        The last segment's final slot starts as a copy of
+        ``q``, so the cells a step does not write (wall ghosts, pitch
+        padding) keep the input's values, as in the in-place forward."""
         torch = _torch()
         pl = self.shape[0] * self.shape[1]
         self.records = []
-        start = q
-        for k in self._segments():
+        start = tuple(q.unbind(0))
+        segs = self._segments()
+        for n, k in enumerate(segs):
             nw = int(self.lib.gocean_adjoint_segment_work_floats(self.d, k))
             w = torch.empty(nw, dtype=torch.float32, device=self.dev)
+            o = 3 * (k - 1) * pl
+            last = w[o:o + 3 * pl].view(3, *self.shape)
+            if n == len(segs) - 1:
+                last.copy_(q)
             G.check(self.lib.gocean_adjoint_record(
                 self.d, self.wind, k, self.dt, None, *(G.ptr(a) for a in start), G.ptr(self.Hi),
                 G.ptr(w), nw, G.stream_handle(self.stream)), "gocean_adjoint_record")
             self.records.append((start, w, k))
-            o = 3 * (k - 1) * pl
-            start = tuple(w[o + f * pl:o + (f + 1) * pl].view(self.shape) for f in range(3))
-        return start
+            start = tuple(last.unbind(0))
+        return tuple(a.clone() for a in start)
 
     # ------------------------------------------------------------------
     def forward(self, q0, n_steps, dt):
@@ -174,23 +189,14 @@ class RK2Adjoint(object):
         self.checkpoints = None
         cur = self._enter()
         with torch.cuda.stream(self.stream):
-            q = tuple(a.detach().to(torch.float32).contiguous().clone() for a in q0)
             if self._use_records():
-                fin = self._forward_records(q)
-                # cells a step does not write (wall ghosts, pitch padding)
-                # keep the input's values, as in the in-place forward
-                nx4 = self.d.nx + 4
-                out = []
-                for a, f in zip(q, fin):
-                    o = a.clone()
-                    o[:, :nx4] = f[:, :nx4]
-                    o.reshape(-1)[self.keep] = a.reshape(-1)[self.keep]
-                    out.append(o)
-                q = tuple(out)
+                q3 = torch.stack([a.detach().to(torch.float32) for a in q0])
+                q = self._forward_records(q3)
                 self.checkpoints = [r[0] for r in self.records]
                 self.final = q
                 self._leave(cur, q)
                 return q
+            q = tuple(a.detach().to(torch.float32).contiguous().clone() for a in q0)
             tmp = self._alloc3()
             self.checkpoints = []
             done = 0
@@ -211,13 +217,14 @@ class RK2Adjoint(object):
             raise RuntimeError("backward() needs a preceding forward()")
         cur_stream = self._enter()
         with torch.cuda.stream(self.stream):
-            lam = [a.detach().to(torch.float32).contiguous().clone() for a in lam]
-            keep = [a.reshape(-1)[self.keep].clone() for a in lam]
-            for a in lam:
-                f = a.reshape(-1)
-                f.index_add_(0, self.wrap_dst, f[self.wrap_src])
-            cur = tuple(lam)
-            nxt = self._alloc3()
+            # the three fields as one (3, ny + 4, pitch) tensor, so that each
+            # ghost-cell rule is one indexing launch for all fields
+            L = torch.stack([a.detach().to(torch.float32) for a in lam])
+            Lf = L.view(3, -1)
+            keep = Lf[:, self.keep]
+            Lf.index_add_(1, self.wrap_dst, Lf[:, self.wrap_src])
+            cur = tuple(L.unbind(0))
+            nxt = tuple(torch.empty_like(L).unbind(0))
             n_seg = len(self.checkpoints)
             if self.records is not None:
                 for start, w, k in reversed(self.records):
@@ -238,9 +245,9 @@ class RK2Adjoint(object):
                     G.ptr(self.work), self.work.numel(), G.stream_handle(self.stream)),
                     "gocean_adjoint_segment")
                 cur, nxt = nxt, cur
-            out = tuple(a.clone() for a in cur)
-            for a, kv in zip(out, keep):
-                a.reshape(-1)[self.keep] += kv
+            O = torch.stack(cur)
+            O.view(3, -1)[:, self.keep] += keep
+            out = tuple(O.unbind(0))
         self._leave(cur_stream, out)
         return out
 
@@ -314,9 +321,14 @@ class EnsembleAdjoint(object):
     ``gocean_ensemble_adjoint_segment`` per checkpoint segment, each a few
     launches that cover every member.  Adjoints are flat tensors laid out
     like the ensemble state, so member k's gradient is ``ens.views``-shaped
-    at ``ens.state_off[k]``."""
+    at ``ens.state_off[k]``.
 
-    def __init__(self, ens, checkpoint_every=25):
+    ``keep_records`` as in ``RK2Adjoint``: the forward runs the recording
+    variant (``gocean_ensemble_adjoint_record``, one work buffer per segment)
+    and the backward only the reverse passes
+    (``gocean_ensemble_adjoint_reverse``)."""
+
+    def __init__(self, ens, checkpoint_every=25, keep_records="auto", record_budget=0.25):
         torch = _torch()
         self.ens = ens
         self.lib = G.load()
@@ -336,7 +348,22 @@ class EnsembleAdjoint(object):
                 src.append(a + off); dst.append(b + off); keep.append(c + off)
         mk = lambda parts: torch.as_tensor(np.concatenate(parts), device=dev)
         self.wrap_src, self.wrap_dst, self.keep = mk(src), mk(dst), mk(keep)
+        if keep_records not in (True, False, "auto"):
+            raise ValueError("keep_records must be True, False or 'auto'")
+        self.keep_records = keep_records
+        self.record_budget = float(record_budget)
         self.checkpoints = None
+        self.records = None
+
+    def records_bytes(self, n_steps):
+        """Device bytes the kept records of an ``n_steps`` run take."""
+        n_seg = -(-int(n_steps) // self.K)
+        return n_seg * 4 * (self.work.numel() + self.ens.state[0].numel())
+
+    def _use_records(self):
+        if self.keep_records != "auto":
+            return bool(self.keep_records)
+        return self.records_bytes(self.n_steps) <= self.record_budget * _available_bytes(self.work.device)
 
     def forward(self, state0, n_steps):
         """Run n_steps RK2 steps of every member (per-member device dt) from
@@ -344,10 +371,31 @@ class EnsembleAdjoint(object):
         torch = _torch()
         ens = self.ens
         self.n_steps = int(n_steps)
+        self.records = None
         cur = torch.cuda.current_stream(state0.device)
         self.stream.wait_stream(cur)
         with torch.cuda.stream(self.stream):
             ens.state[0].copy_(state0.detach())
+            if self._use_records():
+                self.records = []
+                start = ens.state[0].clone()
+                self.checkpoints = [start]
+                done = 0
+                while done < self.n_steps:
+                    k = min(self.K, self.n_steps - done)
+                    w = torch.empty_like(self.work)
+                    nxt = torch.empty_like(start)
+                    G.check(self.lib.gocean_ensemble_adjoint_record(
+                        ens.handle, k, G.ptr(ens.dt), G.ptr(start), G.ptr(ens.Hi), G.ptr(w), w.numel(),
+                        G.ptr(nxt), G.stream_handle(self.stream)), "gocean_ensemble_adjoint_record")
+                    self.records.append((start, w, k))
+                    start = nxt
+                    done += k
+                ens.state[0].copy_(start)
+                out = start
+                cur.wait_stream(self.stream)
+                out.record_stream(cur)
+                return out
             self.checkpoints = []
             done = 0
             while done < self.n_steps:
@@ -374,6 +422,13 @@ class EnsembleAdjoint(object):
             a.index_add_(0, self.wrap_dst, a[self.wrap_src])
             b = torch.empty_like(a)
             n_seg = len(self.checkpoints)
+            if self.records is not None:
+                for start, w, k in reversed(self.records):
+                    G.check(self.lib.gocean_ensemble_adjoint_reverse(
+                        ens.handle, k, G.ptr(ens.dt), G.ptr(start), G.ptr(a), G.ptr(b), G.ptr(ens.Hi),
+                        G.ptr(w), w.numel(), G.stream_handle(self.stream)), "gocean_ensemble_adjoint_reverse")
+                    a, b = b, a
+                n_seg = 0
             for s in range(n_seg - 1, -1, -1):
                 k = min(self.K, self.n_steps - s * self.K)
                 G.check(self.lib.gocean_ensemble_adjoint_segment(
@@ -403,6 +458,32 @@ def bilinear_eta(eta, obs_x, obs_y, dx, dy):
     e00, e01 = eta[r, c], eta[r, c + 1]
     e10, e11 = eta[r + 1, c], eta[r + 1, c + 1]
     return (1 - wy) * ((1 - wx) * e00 + wx * e01) + wy * ((1 - wx) * e10 + wx * e11)
+
+
+def bilinear_stencil(offset, pitch, obs_x, obs_y, dx, dy):
+    """Flat indices and weights, both (n, 4), of the bilinear interpolation
+    of ``bilinear_eta`` for points (x, y) in the padded plane that starts at
+    element ``offset`` of a flat buffer with row length ``pitch``: eta at the
+    points is ``(flat[index] * weight).sum(1)``.  With the stencils of many
+    domains concatenated, one gather covers every member of an ensemble."""
+    fx = np.asarray(obs_x, dtype=np.float64) / dx - 0.5
+    fy = np.asarray(obs_y, dtype=np.float64) / dy - 0.5
+    i0 = np.floor(fx).astype(np.int64)
+    j0 = np.floor(fy).astype(np.int64)
+    wx = (fx - i0).astype(np.float32)
+    wy = (fy - j0).astype(np.float32)
+    base = int(offset) + (j0 + 2) * int(pitch) + (i0 + 2)
+    index = np.stack([base, base + 1, base + pitch, base + pitch + 1], axis=1)
+    weight = np.stack([(1 - wy) * (1 - wx), (1 - wy) * wx, wy * (1 - wx), wy * wx], axis=1)
+    return index, weight.astype(np.float32)
+
+
+def stencil_misfit(flat, index, weight, y):
+    """J = 0.5 * sum_k ((flat[index_k] . weight_k) - y_k)^2 for device
+    tensors from ``bilinear_stencil``: a few launches for any number of
+    points and domains."""
+    d = (flat[index] * weight).sum(1) - y
+    return 0.5 * (d * d).sum()
 
 
 def point_misfit(eta, obs_x, obs_y, y, dx, dy):

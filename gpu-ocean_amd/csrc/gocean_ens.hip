@@ -303,6 +303,16 @@ __global__ __launch_bounds__(kAdjNT) void k_ens_adj_tile(EnsAdjLaunch L) {
     adj_tile_body<kAdjBW, kAdjBH, kAdjNT>(A, S, tx, ty, threadIdx.x, kAdjNT);
 }
 
+// member blockIdx.y's three state planes from src to dst
+__global__ __launch_bounds__(256) void k_ens_copy3(const EnsMember* members, EnsRef src, EnsRef dst) {
+    const EnsMember& M = members[blockIdx.y];
+    const float* a = src.base + ens_off(M, src);
+    float* b = const_cast<float*>(dst.base) + ens_off(M, dst);
+    const long long n = 3 * M.plane;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
 __global__ __launch_bounds__(256) void k_ens_adj_fold(const EnsMember* members, EnsRef o) {
     const EnsMember& M = members[blockIdx.z];
     FoldArgs F;
@@ -334,6 +344,7 @@ struct gocean_ensemble {
     // reverse mode
     int adj_kmax = 0;
     int n_ftiles = 0, n_atiles = 0, max_side = 0;
+    long long max_plane = 0;
     int* d_ftile_member = nullptr;
     int* d_atile_member = nullptr;
 };
@@ -606,6 +617,7 @@ long long gocean_ensemble_adjoint_setup(gocean_ensemble_t* E, int max_steps) {
     std::vector<int> ftiles, atiles;
     long long off = 0;
     int max_side = 0;
+    long long max_plane = 0;
     for (int k = 0; k < E->n; ++k) {
         EnsMember& M = E->members[k];
         if (M.r != 0.0f) return set_error(GOCEAN_EINVAL, "member %d: the adjoint supports r == 0 only", k);
@@ -624,6 +636,7 @@ long long gocean_ensemble_adjoint_setup(gocean_ensemble_t* E, int max_steps) {
         M.atile_begin = (int)atiles.size();
         for (int t = 0; t < M.atiles_x * ((M.ny + kAdjBH - 1) / kAdjBH); ++t) atiles.push_back(k);
         max_side = std::max(max_side, std::max(M.nx, M.ny));
+        max_plane = std::max(max_plane, M.plane);
     }
     hipFree(E->d_ftile_member);
     hipFree(E->d_atile_member);
@@ -642,68 +655,133 @@ long long gocean_ensemble_adjoint_setup(gocean_ensemble_t* E, int max_steps) {
     E->n_ftiles = (int)ftiles.size();
     E->n_atiles = (int)atiles.size();
     E->max_side = max_side;
+    E->max_plane = max_plane;
     return off;
 }
 
-int gocean_ensemble_adjoint_segment(gocean_ensemble_t* E, int n_steps, const float* dt,
-                                    const float* state0, const float* lam_in, float* lam_out,
-                                    const float* Hi, float* work, long long work_floats, void* stream) {
-    clear_error();
-    if (!E || !dt || !state0 || !lam_in || !lam_out || !Hi || !work)
-        return set_error(GOCEAN_EINVAL, "NULL argument");
+}  // extern "C"
+
+// Work of a reverse-mode segment, per member block (setup's max_steps K):
+// Q_1..Q_K (3 planes each), Q*_0..Q*_{K-1} (3 each), branch records of the
+// 2K substeps (5 each), lambda scratch (9).
+struct EnsSegment {
+    gocean_ensemble_t* E;
+    const float* dt;
+    const float* state0;
+    const float* Hi;
+    float* work;
+    hipStream_t s;
+    int K, k;
+    EnsRef W(int idx) const { EnsRef r; r.base = work; r.in_work = 1; r.idx = idx; return r; }
+    static EnsRef F(const float* b) { EnsRef r; r.base = b; r.in_work = 0; r.idx = 0; return r; }
+    EnsRef Q(int i) const { return i == 0 ? F(state0) : W(3 * (i - 1)); }   // Q_i
+    EnsRef Qs(int i) const { return W(3 * K + 3 * i); }                      // Q*_i
+    EnsRef Tp(int i, int sub) const { return W(6 * K + 10 * i + 5 * sub); } // branch record
+
+    int fwd(int rk_step, EnsRef in, EnsRef qn, EnsRef out, EnsRef tape) const {
+        EnsTapeLaunch L;
+        L.members = E->d_members; L.tile_member = E->d_ftile_member;
+        L.in = in; L.qn = qn; L.out = out; L.tape = tape; L.Hi = Hi; L.dt = dt; L.rk_step = rk_step;
+        hipLaunchKernelGGL(k_ens_substep_tape, dim3(E->n_ftiles), dim3(kEnsTapeNT), 0, s, L);
+        return check_launch("ensemble recording substep launch");
+    }
+    // w_fold / x_fold: w / x is an unfolded output of another substep
+    // adjoint; fold_out: fold this one's output (the segment's result only)
+    int adj(EnsRef eta, EnsRef tape, EnsRef w, EnsRef wc, EnsRef x, EnsRef xc, float alpha, float gamma,
+            EnsRef o, int w_fold, int x_fold, bool fold_out) const {
+        EnsAdjLaunch L;
+        L.members = E->d_members; L.tile_member = E->d_atile_member;
+        L.eta = eta; L.tape = tape; L.w = w; L.w_clamp = wc; L.x = x; L.x_clamp = xc; L.o = o;
+        L.Hi = Hi; L.dt = dt; L.alpha = alpha; L.gamma = gamma; L.w_fold = w_fold; L.x_fold = x_fold;
+        hipLaunchKernelGGL(k_ens_adj_tile, dim3(E->n_atiles), dim3(kAdjNT), 0, s, L);
+        if (fold_out) {
+            const int gx = (E->max_side + 16 + 255) / 256;
+            hipLaunchKernelGGL(k_ens_adj_fold, dim3(gx, 9, E->n), dim3(256), 0, s, E->d_members, o);
+        }
+        return check_launch("ensemble adjoint launch");
+    }
+    int copy3(EnsRef src, EnsRef dst) const {
+        const int gx = (int)std::min<long long>((3 * E->max_plane + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_ens_copy3, dim3(gx, E->n), dim3(256), 0, s, E->d_members, src, dst);
+        return check_launch("ensemble state copy launch");
+    }
+    int record(float* state_out) const {
+        // the final slot starts as the start state, so the cells a step does
+        // not write (wall ghosts, pitch padding) carry over into state_out
+        if (state_out)
+            if (int e = copy3(F(state0), W(3 * (k - 1)))) return e;
+        for (int i = 0; i < k; ++i) {
+            if (int e = fwd(0, Q(i), Qs(i), Qs(i), Tp(i, 0))) return e;
+            if (int e = fwd(1, Qs(i), Q(i), W(3 * i), Tp(i, 1))) return e;
+        }
+        if (state_out) return copy3(W(3 * (k - 1)), F(state_out));
+        return 0;
+    }
+    int reverse(const float* lam_in, float* lam_out) const {
+        EnsRef none; none.base = nullptr; none.in_work = 0; none.idx = 0;
+        const EnsRef lam_s = W(16 * K), lam_a = W(16 * K + 3), lam_b = W(16 * K + 6);
+        EnsRef le = F(lam_in);
+        for (int i = k - 1; i >= 0; --i) {
+            const EnsRef next = W(3 * i);          // Q_{i+1}
+            EnsRef o;
+            if (i == 0) o = F(lam_out);
+            else o = (le.in_work && le.idx == lam_a.idx) ? lam_b : lam_a;
+            const int l_unfolded = i < k - 1;
+            if (int e = adj(Qs(i), Tp(i, 1), le, next, none, none, 0.5f, 0.0f, lam_s, l_unfolded, 0, false))
+                return e;
+            if (int e = adj(Q(i), Tp(i, 0), lam_s, Qs(i), le, next, 1.0f, 0.5f, o, 1, l_unfolded, i == 0))
+                return e;
+            le = o;
+        }
+        return 0;
+    }
+};
+
+static int ens_segment(gocean_ensemble_t* E, int n_steps, const float* dt, const float* state0,
+                       const float* Hi, float* work, long long work_floats, void* stream, EnsSegment& S) {
+    if (!E || !dt || !state0 || !Hi || !work) return set_error(GOCEAN_EINVAL, "NULL argument");
     if (E->adj_kmax < 1) return set_error(GOCEAN_ESTATE, "call gocean_ensemble_adjoint_setup first");
     if (n_steps < 1 || n_steps > E->adj_kmax)
         return set_error(GOCEAN_EINVAL, "n_steps %d not in [1, %d]", n_steps, E->adj_kmax);
     const EnsMember& last = E->members[E->n - 1];
     if (work_floats < last.work_off + (16LL * E->adj_kmax + 9LL) * last.plane)
         return set_error(GOCEAN_EINVAL, "work buffer too small");
-    hipStream_t s = (hipStream_t)stream;
-    const int K = E->adj_kmax, k = n_steps;
-    auto W = [&](int idx) { EnsRef r; r.base = work; r.in_work = 1; r.idx = idx; return r; };
-    auto F = [&](const float* b) { EnsRef r; r.base = b; r.in_work = 0; r.idx = 0; return r; };
-    auto Q = [&](int i) { return i == 0 ? F(state0) : W(3 * (i - 1)); };   // Q_i
-    auto Qs = [&](int i) { return W(3 * K + 3 * i); };                       // Q*_i
-    auto Tp = [&](int i, int sub) { return W(6 * K + 10 * i + 5 * sub); };  // branch record
-    const int gx = (E->max_side + 16 + 255) / 256;
-    auto fwd = [&](int rk_step, EnsRef in, EnsRef qn, EnsRef out, EnsRef tape) {
-        EnsTapeLaunch L;
-        L.members = E->d_members; L.tile_member = E->d_ftile_member;
-        L.in = in; L.qn = qn; L.out = out; L.tape = tape; L.Hi = Hi; L.dt = dt; L.rk_step = rk_step;
-        hipLaunchKernelGGL(k_ens_substep_tape, dim3(E->n_ftiles), dim3(kEnsTapeNT), 0, s, L);
-        return check_launch("ensemble recording substep launch");
-    };
-    // w_fold / x_fold: w / x is an unfolded output of another substep
-    // adjoint; fold_out: fold this one's output (the segment's result only)
-    auto adj = [&](EnsRef eta, EnsRef tape, EnsRef w, EnsRef wc, EnsRef x, EnsRef xc, float alpha,
-                   float gamma, EnsRef o, int w_fold, int x_fold, bool fold_out) {
-        EnsAdjLaunch L;
-        L.members = E->d_members; L.tile_member = E->d_atile_member;
-        L.eta = eta; L.tape = tape; L.w = w; L.w_clamp = wc; L.x = x; L.x_clamp = xc; L.o = o;
-        L.Hi = Hi; L.dt = dt; L.alpha = alpha; L.gamma = gamma; L.w_fold = w_fold; L.x_fold = x_fold;
-        hipLaunchKernelGGL(k_ens_adj_tile, dim3(E->n_atiles), dim3(kAdjNT), 0, s, L);
-        if (fold_out) hipLaunchKernelGGL(k_ens_adj_fold, dim3(gx, 9, E->n), dim3(256), 0, s, E->d_members, o);
-        return check_launch("ensemble adjoint launch");
-    };
-    for (int i = 0; i < k; ++i) {
-        if (int e = fwd(0, Q(i), Qs(i), Qs(i), Tp(i, 0))) return e;
-        if (int e = fwd(1, Qs(i), Q(i), W(3 * i), Tp(i, 1))) return e;
-    }
-    EnsRef none; none.base = nullptr; none.in_work = 0; none.idx = 0;
-    const EnsRef lam_s = W(16 * K), lam_a = W(16 * K + 3), lam_b = W(16 * K + 6);
-    EnsRef le = F(lam_in);
-    for (int i = k - 1; i >= 0; --i) {
-        const EnsRef next = W(3 * i);          // Q_{i+1}
-        EnsRef o;
-        if (i == 0) o = F(lam_out);
-        else o = (le.in_work && le.idx == lam_a.idx) ? lam_b : lam_a;
-        const int l_unfolded = i < k - 1;
-        if (int e = adj(Qs(i), Tp(i, 1), le, next, none, none, 0.5f, 0.0f, lam_s, l_unfolded, 0, false))
-            return e;
-        if (int e = adj(Q(i), Tp(i, 0), lam_s, Qs(i), le, next, 1.0f, 0.5f, o, 1, l_unfolded, i == 0))
-            return e;
-        le = o;
-    }
+    S.E = E; S.dt = dt; S.state0 = state0; S.Hi = Hi; S.work = work; S.s = (hipStream_t)stream;
+    S.K = E->adj_kmax; S.k = n_steps;
     return 0;
+}
+
+extern "C" {
+
+int gocean_ensemble_adjoint_record(gocean_ensemble_t* E, int n_steps, const float* dt, const float* state0,
+                                   const float* Hi, float* work, long long work_floats, float* state_out,
+                                   void* stream) {
+    clear_error();
+    EnsSegment S;
+    if (int e = ens_segment(E, n_steps, dt, state0, Hi, work, work_floats, stream, S)) return e;
+    if (state_out == state0) return set_error(GOCEAN_EINVAL, "state_out must not alias state0");
+    return S.record(state_out);
+}
+
+int gocean_ensemble_adjoint_reverse(gocean_ensemble_t* E, int n_steps, const float* dt, const float* state0,
+                                    const float* lam_in, float* lam_out, const float* Hi, float* work,
+                                    long long work_floats, void* stream) {
+    clear_error();
+    if (!lam_in || !lam_out) return set_error(GOCEAN_EINVAL, "NULL argument");
+    EnsSegment S;
+    if (int e = ens_segment(E, n_steps, dt, state0, Hi, work, work_floats, stream, S)) return e;
+    return S.reverse(lam_in, lam_out);
+}
+
+int gocean_ensemble_adjoint_segment(gocean_ensemble_t* E, int n_steps, const float* dt,
+                                    const float* state0, const float* lam_in, float* lam_out,
+                                    const float* Hi, float* work, long long work_floats, void* stream) {
+    clear_error();
+    if (!lam_in || !lam_out) return set_error(GOCEAN_EINVAL, "NULL argument");
+    EnsSegment S;
+    if (int e = ens_segment(E, n_steps, dt, state0, Hi, work, work_floats, stream, S)) return e;
+    if (int e = S.record(nullptr)) return e;
+    return S.reverse(lam_in, lam_out);
 }
 
 }  // extern "C"

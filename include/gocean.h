@@ -211,6 +211,22 @@ int gocean_ensemble_adjoint_segment(gocean_ensemble_t* ens, int n_steps, const f
                                     const float* state0, const float* lam_in, float* lam_out,
                                     const float* Hi, float* work, long long work_floats,
                                     void* stream);
+/* The two halves of gocean_ensemble_adjoint_segment, for callers that keep
+ * one work buffer per segment from the forward run instead of recomputing
+ * it (as gocean_adjoint_record / gocean_adjoint_reverse for one domain).
+ * record: n_steps recording RK2 steps of every member from state0 into
+ * `work`; when state_out is not NULL (and not state0) it receives the state
+ * after the segment, laid out like state0, with the cells a step does not
+ * write (wall ghosts, pitch padding) taken from state0.  reverse: lambda at
+ * the segment's end to lambda at its start from a work buffer that record
+ * filled for the same n_steps, dt, state0 and Hi; it may be repeated. */
+int gocean_ensemble_adjoint_record(gocean_ensemble_t* ens, int n_steps, const float* dt,
+                                   const float* state0, const float* Hi, float* work,
+                                   long long work_floats, float* state_out, void* stream);
+int gocean_ensemble_adjoint_reverse(gocean_ensemble_t* ens, int n_steps, const float* dt,
+                                    const float* state0, const float* lam_in, float* lam_out,
+                                    const float* Hi, float* work, long long work_floats,
+                                    void* stream);
 
 /* ---- adjoint (reverse mode) of the fixed-dt RK2 step ----
  * No counterpart in the reference, which has no gradients: this serves

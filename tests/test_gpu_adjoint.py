@@ -488,3 +488,38 @@ def test_ensemble_adjoint_matches_single_domain():
             _say("   member", k, name, "bit-identical" if same else
                  f"max |d| {float((a - b).abs().max()):.3e} of {float(a.abs().max()):.3e}")
             assert same, (k, name)
+
+
+def test_ensemble_kept_records_match_recompute():
+    """Batched reverse mode with the records kept from the forward run
+    (gocean_ensemble_adjoint_record / _reverse) against the checkpointed run
+    that recomputes them: same final state (every cell of every member, and
+    the ensemble's own state afterwards) and the same gradient bits, with an
+    uneven last segment."""
+    _say("ensemble kept records vs recompute")
+    from SWESimulators import Common, CDKLM16Ensemble
+    from test_gpu_ensemble import members
+    Common_, G, A = _mods()
+    ms = members(seed=5)[:4]
+    ctx = Common.HIPContext(0)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, rk_order=2, courant_number=0.6)
+    state0 = ens.state[0].clone()
+    lam = torch.zeros_like(state0)
+    rng = np.random.default_rng(8)
+    for k in range(ens.n):
+        o, pl = int(ens.state_off[k]), ens.planes[k]
+        for f in range(3):
+            v = lam[o + f * pl:o + (f + 1) * pl].view(ens.ny[k] + 4, ens.pitch[k])
+            v[2:-2, 2:-2] = torch.from_numpy(rng.standard_normal((ens.ny[k], ens.nx[k])).astype(np.float32))
+    res = {}
+    for keep in (False, True):
+        ea = A.EnsembleAdjoint(ens, checkpoint_every=3, keep_records=keep)
+        out = ea.forward(state0, 8)
+        assert (ea.records is not None) == keep
+        g1 = ea.backward(lam)
+        g2 = ea.backward(lam)
+        torch.cuda.synchronize()
+        res[keep] = (out.clone(), ens.state[0].clone(), g1, g2)
+    for a, b in zip(res[False], res[True]):
+        assert torch.equal(a, b)
+    assert torch.equal(res[True][2], res[True][3])
