@@ -48,11 +48,24 @@ struct SubstepArgs {
     int bc_n, bc_e, bc_s, bc_w;
     int rk_order, rk_step;
     int tiles_x, tiles_y;
+    // derived by substep_derive on the host, once per launch instead of in
+    // every thread: dx/(2g), dy/(2g), 1/(2g), 1/dx, 1/dy (IEEE divisions on
+    // either side, so the same bits)
+    float vdx, udy, r_two_g, r_dx, r_dy;
     // branch record of the substep (written only by the recording variant):
     // per padded cell u, v, the x / y face depth terms and a code word
     float* tape_u; float* tape_v; float* tape_t; float* tape_s;
     uint32_t* tape_code;
 };
+
+inline void substep_derive(SubstepArgs& A) {
+    const float two_g = 2.0f * A.g;
+    A.vdx = A.dx / two_g;
+    A.udy = A.dy / two_g;
+    A.r_two_g = 1.0f / two_g;
+    A.r_dx = 1.0f / A.dx;
+    A.r_dy = 1.0f / A.dy;
+}
 
 // ---------------------------------------------------------------------------
 // scalar helpers
@@ -251,15 +264,15 @@ __device__ __forceinline__ Consts make_consts(const SubstepArgs& A) {
     K.theta_g = A.theta * A.g;
     K.dx = A.dx;
     K.dy = A.dy;
-    K.vdx = A.dx / K.two_g;
-    K.udy = A.dy / K.two_g;
+    K.vdx = A.vdx;
+    K.udy = A.udy;
     K.desing = A.desing_eps;
     K.flux_eps = A.flux_eps;
     K.cutoff = A.cutoff;
     K.land = A.land;
-    K.r_two_g = 1.0f / K.two_g;
-    K.r_dx = 1.0f / A.dx;
-    K.r_dy = 1.0f / A.dy;
+    K.r_two_g = A.r_two_g;
+    K.r_dx = A.r_dx;
+    K.r_dy = A.r_dy;
     return K;
 }
 

@@ -29,12 +29,13 @@ struct EnsMember {
     float f, north_x, north_y;
     float wind_t, wx0, wx1, wy0, wy1;
     int bc_n, bc_e, bc_s, bc_w;
+    float vdx, udy;                // as SubstepArgs (with r_dx, r_dy, r_two_g below)
     // reverse mode (set by gocean_ensemble_adjoint_setup): the member's block
     // of the work buffer and its tiles in the recording-forward (32x16) and
     // adjoint (kAdjBW x kAdjBH) launches
     long long work_off;
     int ftiles_x, ftile_begin, atiles_x, atile_begin;
-    float r_dx, r_dy, r_two_g, Dx, Dy, Ddx, Ddy, g_dx, g_dy;   // as AdjTileArgs
+    float r_dx, r_dy, r_two_g, Dx, Dy, Ddx, Ddy, g_dx, g_dy;   // as AdjTileArgs / SubstepArgs
 };
 
 struct EnsLaunch {
@@ -77,6 +78,7 @@ __global__ __launch_bounds__(NT) void k_ens_substep(EnsLaunch L) {
     A.wind_t = M.wind_t; A.wx0 = M.wx0; A.wx1 = M.wx1; A.wy0 = M.wy0; A.wy1 = M.wy1;
     A.dt = 0.0f;
     A.bc_n = M.bc_n; A.bc_e = M.bc_e; A.bc_s = M.bc_s; A.bc_w = M.bc_w;
+    A.vdx = M.vdx; A.udy = M.udy; A.r_two_g = M.r_two_g; A.r_dx = M.r_dx; A.r_dy = M.r_dy;
     A.rk_order = L.rk_order; A.rk_step = L.rk_step;
     A.tiles_x = M.tiles_x; A.tiles_y = 0;
     const int lt = t - M.tile_begin;
@@ -247,6 +249,7 @@ __global__ __launch_bounds__(kEnsTapeNT) void k_ens_substep_tape(EnsTapeLaunch L
     A.wind_t = M.wind_t; A.wx0 = M.wx0; A.wx1 = M.wx1; A.wy0 = M.wy0; A.wy1 = M.wy1;
     A.dt = 0.0f;
     A.bc_n = M.bc_n; A.bc_e = M.bc_e; A.bc_s = M.bc_s; A.bc_w = M.bc_w;
+    A.vdx = M.vdx; A.udy = M.udy; A.r_two_g = M.r_two_g; A.r_dx = M.r_dx; A.r_dy = M.r_dy;
     A.rk_order = 2; A.rk_step = L.rk_step;
     A.tiles_x = M.ftiles_x; A.tiles_y = 0;
     const int lt = t - M.ftile_begin;
@@ -465,6 +468,13 @@ int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_confi
         M.land = d.land_value; M.f = d.f; M.north_x = d.north_x; M.north_y = d.north_y;
         M.wind_t = m.wind.t; M.wx0 = m.wind.x0; M.wx1 = m.wind.x1; M.wy0 = m.wind.y0; M.wy1 = m.wind.y1;
         M.bc_n = d.bc_north; M.bc_e = d.bc_east; M.bc_s = d.bc_south; M.bc_w = d.bc_west;
+        {
+            SubstepArgs D;
+            std::memset(&D, 0, sizeof(D));
+            D.dx = d.dx; D.dy = d.dy; D.g = d.g;
+            substep_derive(D);
+            M.vdx = D.vdx; M.udy = D.udy; M.r_two_g = D.r_two_g; M.r_dx = D.r_dx; M.r_dy = D.r_dy;
+        }
         if (m.state_offset < 0 || m.hi_offset < 0 || m.hm_offset < 0) {
             delete E;
             return set_error(GOCEAN_EINVAL, "member %d: negative buffer offset", k);

@@ -75,6 +75,7 @@ void fill_args(const gocean_domain_t* d, SubstepArgs& A) {
     A.f = d->f; A.north_x = d->north_x; A.north_y = d->north_y;
     A.bc_n = d->bc_north; A.bc_e = d->bc_east; A.bc_s = d->bc_south; A.bc_w = d->bc_west;
     A.rk_order = d->rk_order;
+    substep_derive(A);
 }
 
 // ---------------------------------------------------------------------------
