@@ -164,9 +164,13 @@ def bench_adjoint(ctx, n=512, steps=500, K=50, n_obs=64, seed=44):
     # twin run: perturbed initial eta
     pert = 0.05 * B.gaussian_random_field((n + 4, n + 4), 16.0, rng).astype(np.float32)
     twin0 = [q0[0] + torch.from_numpy(pert).to(q0[0].device), q0[1], q0[2]]
+    # the observation operator as one bilinear stencil gather over the eta plane
+    sidx, swt = A.bilinear_stencil(0, q0[0].shape[1], ox, oy, inp.dx, inp.dy)
+    sidx = torch.from_numpy(sidx).to(q0[0].device)
+    swt = torch.from_numpy(swt).to(q0[0].device)
     with torch.no_grad():
         tw = adj.forward(twin0, steps, dt)
-        y = A.bilinear_eta(tw[0], ox, oy, inp.dx, inp.dy) + torch.from_numpy(
+        y = (tw[0].reshape(-1)[sidx] * swt).sum(1) + torch.from_numpy(
             rng.normal(0.0, 0.01, n_obs).astype(np.float32)).to(q0[0].device)
     # forward alone: the plain step loop
     sim.step_n(5)
@@ -179,7 +183,7 @@ def bench_adjoint(ctx, n=512, steps=500, K=50, n_obs=64, seed=44):
     def fwd_adj(a):
         q = [x.clone().requires_grad_(True) for x in q0]
         out = A.CDKLM16RK2.apply(*q, a, steps, dt)
-        J = A.point_misfit(out[0], ox, oy, y, inp.dx, inp.dy)
+        J = A.stencil_misfit(out[0].reshape(-1), sidx, swt, y)
         J.backward()
         return J, q
 

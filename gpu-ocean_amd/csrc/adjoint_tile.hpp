@@ -84,10 +84,10 @@ __host__ __device__ inline float adj_sqrt(float x) {
 // value of the minmod candidate `sel` (the forward's minmod_slope result,
 // bit for bit) and its derivative pattern
 __host__ __device__ inline float slope_val(uint32_t sel, float l, float c, float r, float theta) {
-    if (sel == 0u) return (c - l) * theta;
-    if (sel == 1u) return (r - l) * 0.5f;
-    if (sel == 2u) return (r - c) * theta;
-    return 0.0f;
+    // all three candidates and a select: no divergent branches in a wave
+    // whose cells chose different candidates
+    const float a = (c - l) * theta, b = (r - l) * 0.5f, d = (r - c) * theta;
+    return sel == 0u ? a : (sel == 1u ? b : (sel == 2u ? d : 0.0f));
 }
 
 // d(candidate sel)/d(l, c, r): (c-l)*theta, (r-l)/2, (r-c)*theta, or 0
