@@ -87,6 +87,80 @@ def cpu_baseline(inp, n_steps=1):
     return inp.nx * inp.ny * n_steps / s / 1e6, s
 
 
+def _cpu_member_steps(args):
+    """Pool worker: ``n_steps`` RK2 steps of one member with the NumPy oracle
+    (oracle setup included: depth halo, cell depth, CFL dt)."""
+    eta, hu, hv, Hi, nx, ny, dx, dy, f, bc, courant, n_steps = args
+    from oracle import cdklm16_np as O
+    p = O.Params(nx=nx, ny=ny, dx=dx, dy=dy, f=f, bc=bc)
+    sim = O.OracleSim(eta, hu, hv, Hi, p, dt=-1)
+    sim.step_n(n_steps, np.float32(courant * sim.dt / 0.8))
+    return nx * ny * n_steps
+
+
+def _cpu_pool_ready(_):
+    from oracle import cdklm16_np  # noqa: F401
+    return os.getpid()
+
+
+def cpu_baseline_ensemble(sample_members=32, n=512, steps=2, procs=None):
+    """C3 members split across a process pool of the host's cores (at most
+    16), each process running the NumPy oracle on whole members: a bounded
+    sample of ``sample_members`` members x ``steps`` RK2 steps."""
+    import multiprocessing as mp
+    from SWESimulators import BathymetryAndICs as B
+    procs = procs or min(16, os.cpu_count() or 1)
+    inps = B.config3(members=sample_members, n=n)
+    args = [(i.eta, i.hu, i.hv, i.Hi, i.nx, i.ny, i.dx, i.dy, i.f, i.bc, i.courant, steps) for i in inps]
+    # spawned workers: fresh interpreters that never touch the GPU
+    with mp.get_context("spawn").Pool(procs) as pool:
+        pool.map(_cpu_pool_ready, range(procs))
+        t0 = time.perf_counter()
+        done = sum(pool.map(_cpu_member_steps, args, chunksize=1))
+        s = time.perf_counter() - t0
+    return {"value": round(done / s / 1e6, 3), "unit": "Mcell-steps/s", "cores": procs, "kind": "port",
+            "sample": f"NumPy fp32 oracle, {sample_members} C3 members of {n}x{n} x {steps} RK2 steps in a "
+                      f"pool of {procs} spawned processes ({s:.1f} s, oracle setup per member included)"}
+
+
+def cpu_baseline_adjoint(steps=2, n=512, n_obs=64, seed=44):
+    """C4 on the host: the float64 torch restatement of the forward
+    (oracle/cdklm16_torch.py) over ``steps`` RK2 steps of the 512^2 domain and
+    the gradient of the 64-point misfit by torch autograd, timed together."""
+    import torch
+    from oracle import cdklm16_np as O, cdklm16_torch as T
+    from SWESimulators import BathymetryAndICs as B
+    inp = B.config4(n=n)
+    p = O.Params(nx=inp.nx, ny=inp.ny, dx=inp.dx, dy=inp.dy, f=inp.f, bc=inp.bc)
+    Hi = O.bathymetry_halo(inp.Hi, p).astype(np.float64)
+    Hm = O.init_bm(Hi, dtype=np.float64)
+    sim = O.OracleSim(inp.eta, inp.hu, inp.hv, inp.Hi, p, dt=-1)
+    dt = float(np.float32(inp.courant * sim.dt / 0.8))
+    rng = np.random.default_rng(seed)
+    ox = rng.uniform(2.0, n - 2.0, n_obs) * inp.dx
+    oy = rng.uniform(2.0, n - 2.0, n_obs) * inp.dy
+    fx = torch.from_numpy(ox / inp.dx - 0.5)
+    fy = torch.from_numpy(oy / inp.dy - 0.5)
+    i0, j0 = fx.floor().long(), fy.floor().long()
+    wx, wy = fx - i0, fy - j0
+    y = torch.zeros(n_obs, dtype=torch.float64)
+    q = [torch.from_numpy(np.asarray(a, np.float64)).requires_grad_(True) for a in (inp.eta, inp.hu, inp.hv)]
+    Hit, Hmt = torch.from_numpy(Hi), torch.from_numpy(Hm)
+    t0 = time.perf_counter()
+    out = T.rk2_steps(*q, Hit, Hmt, p, dt, steps)
+    e = out[0]
+    r, c = j0 + 2, i0 + 2
+    est = (1 - wy) * ((1 - wx) * e[r, c] + wx * e[r, c + 1]) + wy * ((1 - wx) * e[r + 1, c] + wx * e[r + 1, c + 1])
+    J = 0.5 * ((est - y) ** 2).sum()
+    J.backward()
+    s = time.perf_counter() - t0
+    return {"value": round(n * n * steps / s / 1e6, 4), "unit": "Mcell-steps/s (forward + adjoint)",
+            "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"float64 torch CPU restatement of the forward + autograd gradient of the 64-point "
+                      f"misfit, {steps} RK2 steps of the {n}x{n} C4 domain ({s:.1f} s, "
+                      f"{torch.get_num_threads()} torch threads)"}
+
+
 def bench_ensemble(ctx, members=100, n=512, steps=50, warmup=5, per_member_steps=5):
     """C3: ``members`` domains of n^2 advanced by the batched launch (eager
     and HIP-graph replay), and the same members stepped one member per launch
@@ -572,6 +646,14 @@ def main():
         log(f"[bench] model error: {extra['model_error']}")
 
     cpu = None
+    if not a.no_cpu and not a.no_ensemble:
+        log("[bench] cpu baseline, C3 members over a process pool")
+        extra["cpu_baseline_ensemble"] = cpu_baseline_ensemble()
+        log(f"[bench] cpu ensemble: {extra['cpu_baseline_ensemble']}")
+    if not a.no_cpu and not a.no_adjoint:
+        log("[bench] cpu baseline, C4 forward + adjoint (float64 torch, autograd)")
+        extra["cpu_baseline_adjoint"] = cpu_baseline_adjoint()
+        log(f"[bench] cpu adjoint: {extra['cpu_baseline_adjoint']}")
     if not a.no_cpu:
         log("[bench] cpu baseline (oracle, 1 core, 1 RK2 step)")
         v, s = cpu_baseline(inp, 1)
