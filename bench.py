@@ -167,9 +167,12 @@ def bench_ensemble(ctx, members=100, n=512, steps=50, warmup=5, per_member_steps
     from a Python loop (the reference's ensemble pattern)."""
     import torch
     from SWESimulators import BathymetryAndICs as B, CDKLM16Ensemble
+    from SWESimulators import autotune
     inps = B.config3(members=members, n=n)
     cells = sum(i.nx * i.ny for i in inps)
     ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, inps, courant_number=0.7)
+    best, tune_ms = autotune.tune_ensemble(ens)
+    names = autotune.tile_config_names()
     st = ens.gpu_stream
     ens.step_n(warmup)
     st.synchronize()
@@ -192,9 +195,12 @@ def bench_ensemble(ctx, members=100, n=512, steps=50, warmup=5, per_member_steps
     graph = cells * steps / gwall / 1e6
     ens.cleanUp()
     del ens
-    # one member per launch (plain drop-in class, Python loop over members)
+    # one member per launch (plain drop-in class, Python loop over members),
+    # with the tile the single-domain autotuner picks for these members
     sims = [make_sim(i, ctx) for i in inps]
+    pbest, _ = autotune.tune_simulator(sims[0])
     for sm in sims:
+        sm.tile_config = pbest
         sm.step_n(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -212,6 +218,7 @@ def bench_ensemble(ctx, members=100, n=512, steps=50, warmup=5, per_member_steps
             "batched_graph_mcell_steps_s": round(graph, 1),
             "plain_per_member_mcell_steps_s": round(plain, 1),
             "batched_vs_plain": round(max(eager, graph) / plain, 2),
+            "batched_tile": names[best], "plain_tile": names[pbest],
             "batched_substep_launch_us": round(kernel_s * 1e6, 1),
             "batched_hbm_frac_68B": round(max(eager, graph) * 1e6 * BYTES_PER_CELL_STEP / (HBM_PEAK_GBS * 1e9), 4),
             "steps": steps, "per_member_steps": per_member_steps}
@@ -308,6 +315,8 @@ def bench_c5(ctx, steps=200, K=25, n_obs=16, seed=55, plain_members=4):
     inps = B.config5()
     cells = sum(i.nx * i.ny for i in inps)
     ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, inps, courant_number=0.7)
+    from SWESimulators import autotune
+    autotune.tune_ensemble(ens)
     # the records of this run take ~113 GB: allowed up to 60 % of free HBM
     ea = A.EnsembleAdjoint(ens, checkpoint_every=K, record_budget=0.6)
     state0 = ens.state[0].clone()
@@ -504,6 +513,8 @@ def bench_model_error(ctx, inp, reps=50, ens_members=100, ens_n=512, ens_steps=2
     inps = B.config3(members=ens_members, n=ens_n)
     ecells = sum(i.nx * i.ny for i in inps)
     ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, inps, courant_number=0.7)
+    from SWESimulators import autotune
+    autotune.tune_ensemble(ens)
     ens.enable_model_error(soar_q0=1e-4, seed=1)
     est = ens.gpu_stream
     t_e = _event_time(est, lambda: ens.step_n(ens_steps), 3) / ens_steps

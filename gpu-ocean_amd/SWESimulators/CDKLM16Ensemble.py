@@ -97,6 +97,7 @@ class CDKLM16Ensemble(object):
         G.check(self.lib.gocean_ensemble_create(arr, self.n, -1 if tile_config is None else int(tile_config),
                                                 ctypes.byref(h)), "gocean_ensemble_create")
         self.handle = h
+        self.tile_config = -1 if tile_config is None else int(tile_config)
         G.check(self.lib.gocean_ensemble_bathymetry_init(self.handle, self.Hi.data_ptr(), self.Hm.data_ptr(),
                                                          G.stream_handle(st)), "ensemble bathymetry")
         st.synchronize()
@@ -356,6 +357,15 @@ class CDKLM16Ensemble(object):
             G.check(self.lib.gocean_ensemble_replay(self.handle, G.stream_handle(self.gpu_stream)),
                     "gocean_ensemble_replay")
         self.num_iterations += self.graph_steps
+
+    def set_tile_config(self, tile_config):
+        """Switch the batched substep's tile configuration (an index into
+        ``autotune.tile_config_names()``); a captured graph must be captured
+        again."""
+        G.check(self.lib.gocean_ensemble_set_tile_config(self.handle, int(tile_config)),
+                "gocean_ensemble_set_tile_config")
+        self.tile_config = int(tile_config)
+        self.torch_graph = None
 
     def cells(self):
         return int(sum(nx * ny for nx, ny in zip(self.nx, self.ny)))

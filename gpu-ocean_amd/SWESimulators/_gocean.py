@@ -79,6 +79,7 @@ PROTOTYPES.extend([
     ("gocean_ensemble_create", ctypes.c_int,
      [ctypes.POINTER(Member), ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     ("gocean_ensemble_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("gocean_ensemble_set_tile_config", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("gocean_ensemble_info", ctypes.c_int,
      [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
       ctypes.POINTER(ctypes.c_int)]),

@@ -1,9 +1,9 @@
 """Tile-shape autotuner for the CDKLM16 substep kernels.
 
 Times every tile configuration the library is built with on the caller's own
-domain (a few fixed-dt steps each, HIP events on the simulator's stream),
+domain or ensemble (a few fixed-dt steps each, HIP events on its stream),
 restores the state afterwards, and returns the fastest.  Results are cached
-per (nx, ny, boundary conditions) for the life of the process.
+per (sizes, boundary conditions) for the life of the process.
 """
 
 from __future__ import annotations
@@ -58,5 +58,44 @@ def tune_simulator(sim, steps=8, warmup=2, configs=None, use_cache=True):
     sim.t, sim.num_iterations = t_saved, it_saved
     best = min(times, key=times.get)
     sim.tile_config = best
+    _cache[key] = (best, times)
+    return best, times
+
+
+def tune_ensemble(ens, steps=4, warmup=1, configs=None, use_cache=True):
+    """Return (best_config, {config: ms_per_step}) for a ``CDKLM16Ensemble``:
+    every tile configuration of the batched substep timed on the ensemble's
+    own members (one launch per substep covers them all).  The ensemble's
+    state and iteration count are left unchanged and its tile configuration
+    is set to the winner; a captured graph must be captured again."""
+    import torch
+    key = ("ensemble", tuple(ens.nx), tuple(ens.ny), tuple(tuple(b) for b in ens.bcs), ens.rk_order)
+    if use_cache and key in _cache:
+        ens.set_tile_config(_cache[key][0])
+        return _cache[key]
+    if configs is None:
+        configs = list(range(len(tile_config_names())))
+    st = ens.gpu_stream
+    bufs = list(ens.state) + ([ens.state2] if getattr(ens, "state2", None) is not None else [])
+    with torch.cuda.stream(st):
+        saved = [b.clone() for b in bufs]
+    times = {}
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    for c in configs:
+        ens.set_tile_config(c)
+        # plain RK steps only: no model error, drifters or iteration count
+        ens._rk(warmup)
+        ev0.record(st)
+        ens._rk(steps)
+        ev1.record(st)
+        ev1.synchronize()
+        times[c] = ev0.elapsed_time(ev1) / steps
+        with torch.cuda.stream(st):
+            for b, s in zip(bufs, saved):
+                b.copy_(s)
+    st.synchronize()
+    best = min(times, key=times.get)
+    ens.set_tile_config(best)
     _cache[key] = (best, times)
     return best, times

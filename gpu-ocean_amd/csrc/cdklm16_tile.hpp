@@ -43,11 +43,28 @@ struct SubstepArgs {
     int nx, ny, pitch, hi_pitch;
     float dx, dy, g, r, theta, desing_eps, flux_eps, cutoff, land;
     float f, north_x, north_y;
-    float wind_t, wx0, wx1, wy0, wy1;
+    // wind stress at the substep's time, interpolated on the host
+    // (wind_lerp: the kernel's former expression, so the same bits)
+    float wind_x, wind_y;
     float dt;
-    int bc_n, bc_e, bc_s, bc_w;
-    int rk_order, rk_step;
+    // bits 0-7: north, east, south, west boundary codes (2 bits each); bits
+    // 8-9: rk_order; bits 10-11: rk_step.  One word instead of six keeps the
+    // kernel within the scalar registers of 8 waves per SIMD.
+    uint32_t flags;
     int tiles_x, tiles_y;
+    __host__ __device__ int bc_n() const { return (int)(flags & 3u); }
+    __host__ __device__ int bc_e() const { return (int)((flags >> 2) & 3u); }
+    __host__ __device__ int bc_s() const { return (int)((flags >> 4) & 3u); }
+    __host__ __device__ int bc_w() const { return (int)((flags >> 6) & 3u); }
+    __host__ __device__ int rk_order() const { return (int)((flags >> 8) & 3u); }
+    __host__ __device__ int rk_step() const { return (int)((flags >> 10) & 3u); }
+    __host__ __device__ void set_flags(int n, int e, int s_, int w, int order, int step) {
+        flags = (uint32_t)(n & 3) | ((uint32_t)(e & 3) << 2) | ((uint32_t)(s_ & 3) << 4) |
+                ((uint32_t)(w & 3) << 6) | ((uint32_t)(order & 3) << 8) | ((uint32_t)(step & 3) << 10);
+    }
+    __host__ __device__ void set_rk_step(int step) {
+        flags = (flags & ~(3u << 10)) | ((uint32_t)(step & 3) << 10);
+    }
     // derived by substep_derive on the host, once per launch instead of in
     // every thread: dx/(2g), dy/(2g), 1/(2g), 1/dx, 1/dy (IEEE divisions on
     // either side, so the same bits)
@@ -57,6 +74,9 @@ struct SubstepArgs {
     float* tape_u; float* tape_v; float* tape_t; float* tape_s;
     uint32_t* tape_code;
 };
+
+// wind stress at interpolation weight t between two samples
+inline float wind_lerp(float t, float a0, float a1) { return t * a1 + (1.0f - t) * a0; }
 
 inline void substep_derive(SubstepArgs& A) {
     const float two_g = 2.0f * A.g;
@@ -224,11 +244,12 @@ enum { kModeZero = 0, kModeConst = 1, kModeField = 2 };
 template <int BW, int BH, int MODE, bool TAPE = false>
 struct TileSmem {
     static constexpr bool FIELD = MODE == kModeField;
-    // branch codes of the recording variant (see kCodeX ...)
-    uint32_t CodeX[TAPE ? BH : 1][TAPE ? BW + 2 : 1];
-    uint32_t CodeY[TAPE ? BH + 2 : 1][TAPE ? BW : 1];
-    uint32_t CodeF[TAPE ? BH : 1][TAPE ? BW + 1 : 1];
-    uint32_t CodeG[TAPE ? BH + 1 : 1][TAPE ? BW : 1];
+    // branch codes of the recording variant (see kCodeX ...), 8 bits each
+    // (slope codes) or 5 (flux codes): bytes keep the 32x16 tile under 40 KB
+    uint8_t CodeX[TAPE ? BH : 1][TAPE ? BW + 2 : 1];
+    uint8_t CodeY[TAPE ? BH + 2 : 1][TAPE ? BW : 1];
+    uint8_t CodeF[TAPE ? BH : 1][TAPE ? BW + 1 : 1];
+    uint8_t CodeG[TAPE ? BH + 1 : 1][TAPE ? BW : 1];
     float Hi[BH + 5][BW + 5];          // corner depths, land flagged
     float E[BH + 4][BW + 4];           // eta (dry / land adjusted)
     float U[BH + 4][BW + 4];           // u
@@ -291,9 +312,9 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
     const int by = tile_y * BH;     // padded row of the tile's first halo row
     const Consts K = make_consts(A);
 
-    const bool wN = A.bc_n == kWall, wS = A.bc_s == kWall;
-    const bool wE = A.bc_e == kWall, wW = A.bc_w == kWall;
-    const bool pNS = A.bc_n == kPeriodic, pEW = A.bc_e == kPeriodic;
+    const bool wN = A.bc_n() == kWall, wS = A.bc_s() == kWall;
+    const bool wE = A.bc_e() == kWall, wW = A.bc_w() == kWall;
+    const bool pNS = A.bc_n() == kPeriodic, pEW = A.bc_e() == kPeriodic;
 
     // ---- phase A: corner depths (CDKLM16_kernel.cu:509-525) ----
     for (int it = tid; it < (BH + 5) * (BW + 5); it += NT) {
@@ -445,7 +466,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             const float h_east = (ec + H_east) + t;
             bool adj = false;
             if constexpr (TAPE) {
-                S.CodeX[j][i] = (minmod_sel((uc - ul) * K.theta, (ur - ul) * 0.5f, (ur - uc) * K.theta)
+                S.CodeX[j][i] = (uint8_t)(minmod_sel((uc - ul) * K.theta, (ur - ul) * 0.5f, (ur - uc) * K.theta)
                                  | (minmod_sel((vc_raw - S.V[rr][cc - 1]) * K.theta,
                                                (S.V[rr][cc + 1] - S.V[rr][cc - 1]) * 0.5f,
                                                (S.V[rr][cc + 1] - vc_raw) * K.theta) << 2)
@@ -505,7 +526,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             const float h_north = (ec + H_north) + t;
             bool adj = false;
             if constexpr (TAPE) {
-                S.CodeY[j][i] = (minmod_sel((uc_raw - S.U[rr - 1][cc]) * K.theta,
+                S.CodeY[j][i] = (uint8_t)(minmod_sel((uc_raw - S.U[rr - 1][cc]) * K.theta,
                                             (S.U[rr + 1][cc] - S.U[rr - 1][cc]) * 0.5f,
                                             (S.U[rr + 1][cc] - uc_raw) * K.theta)
                                  | (minmod_sel((vc - vl) * K.theta, (vu - vl) * 0.5f, (vu - vc) * K.theta) << 2)
@@ -573,7 +594,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                 }
             }
             S.F[0][j][i] = Fx; S.F[1][j][i] = Fy; S.F[2][j][i] = Fz;
-            if constexpr (TAPE) S.CodeF[j][i] = fcode;
+            if constexpr (TAPE) S.CodeF[j][i] = (uint8_t)fcode;
             if constexpr (FIELD) { S.F2[0][j][i] = Fx2; S.F2[1][j][i] = Fy2; S.F2[2][j][i] = Fz2; }
         } else {
             const int k = it - kFx;
@@ -625,7 +646,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                 }
             }
             S.G[0][j][i] = Gx; S.G[1][j][i] = Gy; S.G[2][j][i] = Gz;
-            if constexpr (TAPE) S.CodeG[j][i] = gcode;
+            if constexpr (TAPE) S.CodeG[j][i] = (uint8_t)gcode;
             if constexpr (FIELD) { S.G2[0][j][i] = Gx2; S.G2[1][j][i] = Gy2; S.G2[2][j][i] = Gz2; }
         }
     }
@@ -667,14 +688,14 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             // x slope centre (interior rows, tile columns 1..BW+2)
             if (ri && cc >= 1 && cc <= BW + 2 && C >= 1 && C <= nx + 2) {
                 A.tape_t[o] = S.Kx[rr - 2][cc - 1];
-                code |= S.CodeX[rr - 2][cc - 1] << kCodeX;
-                if (C <= nx + 1 && cc <= BW + 1) code |= S.CodeF[rr - 2][cc - 1] << kCodeFx;
+                code |= (uint32_t)S.CodeX[rr - 2][cc - 1] << kCodeX;
+                if (C <= nx + 1 && cc <= BW + 1) code |= (uint32_t)S.CodeF[rr - 2][cc - 1] << kCodeFx;
             }
             // y slope centre (interior columns, tile rows 1..BH+2)
             if (ci && rr >= 1 && rr <= BH + 2 && R >= 1 && R <= ny + 2) {
                 A.tape_s[o] = S.Ly[rr - 1][cc - 2];
-                code |= S.CodeY[rr - 1][cc - 2] << kCodeY;
-                if (R <= ny + 1 && rr <= BH + 1) code |= S.CodeG[rr - 1][cc - 2] << kCodeFy;
+                code |= (uint32_t)S.CodeY[rr - 1][cc - 2] << kCodeY;
+                if (R <= ny + 1 && rr <= BH + 1) code |= (uint32_t)S.CodeG[rr - 1][cc - 2] << kCodeFy;
             }
             A.tape_code[o] = code;
         }
@@ -682,9 +703,8 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
 
     // ---- phase E: sources, RK update, store (:802-1004) ----
     const float dt = A.dt_dev ? *A.dt_dev : A.dt;
-    const float wt = A.wind_t;
-    const float X = wt * A.wx1 + (1.0f - wt) * A.wx0;
-    const float Y = wt * A.wy1 + (1.0f - wt) * A.wy0;
+    const float X = A.wind_x;
+    const float Y = A.wind_y;
     const bool friction = A.r > 0.0f;
     for (int it = tid; it < BW * BH; it += NT) {
         const int j = it / BW, i = it - j * BW;
@@ -777,8 +797,8 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
 
         const size_t o = (size_t)R * pitch + C;
         float ue, uu, uv;
-        if (A.rk_order < 3) {
-            if (A.rk_step == 0) {
+        if (A.rk_order() < 3) {
+            if (A.rk_step() == 0) {
                 ue = ec + dt * L1;
                 uu = hu + dt * L2;
                 uv = hv + dt * L3;
@@ -797,11 +817,11 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
                 }
             }
         } else {
-            if (A.rk_step == 0) {
+            if (A.rk_step() == 0) {
                 ue = ec + dt * L1;
                 uu = hu + dt * L2;
                 uv = hv + dt * L3;
-            } else if (A.rk_step == 1) {
+            } else if (A.rk_step() == 1) {
                 const float ea = A.eta_qn[o], ha = A.hu_qn[o], va = A.hv_qn[o];
                 ue = 0.75f * ea + 0.25f * (ec + dt * L1);
                 uu = 0.75f * ha + 0.25f * (hu + dt * L2);

@@ -72,9 +72,11 @@ static int fwd_tape(const gocean_domain_t* d, const gocean_wind_t* wind, float d
                     hipStream_t s) {
     SubstepArgs A;
     fill_args(d, A);
-    A.dt = dt; A.dt_dev = dt_dev; A.rk_step = rk_step;
+    A.dt = dt; A.dt_dev = dt_dev;
+    A.set_rk_step(rk_step);
     if (wind) {
-        A.wind_t = wind->t; A.wx0 = wind->x0; A.wx1 = wind->x1; A.wy0 = wind->y0; A.wy1 = wind->y1;
+        A.wind_x = wind_lerp(wind->t, wind->x0, wind->x1);
+        A.wind_y = wind_lerp(wind->t, wind->y0, wind->y1);
     }
     A.eta_in = e_in; A.hu_in = u_in; A.hv_in = v_in;
     A.eta_qn = e_qn ? e_qn : e_out; A.hu_qn = u_qn ? u_qn : u_out; A.hv_qn = v_qn ? v_qn : v_out;

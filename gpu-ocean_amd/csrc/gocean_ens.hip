@@ -27,7 +27,7 @@ struct EnsMember {
     int tiles_x, tile_begin;
     float dx, dy, g, r, theta, desing, flux_eps, cutoff, land;
     float f, north_x, north_y;
-    float wind_t, wx0, wx1, wy0, wy1;
+    float wind_x, wind_y;          // wind stress (wind_lerp of the member's samples)
     int bc_n, bc_e, bc_s, bc_w;
     float vdx, udy;                // as SubstepArgs (with r_dx, r_dy, r_two_g below)
     // reverse mode (set by gocean_ensemble_adjoint_setup): the member's block
@@ -75,11 +75,10 @@ __global__ __launch_bounds__(NT) void k_ens_substep(EnsLaunch L) {
     A.dx = M.dx; A.dy = M.dy; A.g = M.g; A.r = M.r; A.theta = M.theta;
     A.desing_eps = M.desing; A.flux_eps = M.flux_eps; A.cutoff = M.cutoff; A.land = M.land;
     A.f = M.f; A.north_x = M.north_x; A.north_y = M.north_y;
-    A.wind_t = M.wind_t; A.wx0 = M.wx0; A.wx1 = M.wx1; A.wy0 = M.wy0; A.wy1 = M.wy1;
+    A.wind_x = M.wind_x; A.wind_y = M.wind_y;
     A.dt = 0.0f;
-    A.bc_n = M.bc_n; A.bc_e = M.bc_e; A.bc_s = M.bc_s; A.bc_w = M.bc_w;
+    A.set_flags(M.bc_n, M.bc_e, M.bc_s, M.bc_w, L.rk_order, L.rk_step);
     A.vdx = M.vdx; A.udy = M.udy; A.r_two_g = M.r_two_g; A.r_dx = M.r_dx; A.r_dy = M.r_dy;
-    A.rk_order = L.rk_order; A.rk_step = L.rk_step;
     A.tiles_x = M.tiles_x; A.tiles_y = 0;
     const int lt = t - M.tile_begin;
     const int ty = lt / M.tiles_x, tx = lt - ty * M.tiles_x;
@@ -246,11 +245,10 @@ __global__ __launch_bounds__(kEnsTapeNT) void k_ens_substep_tape(EnsTapeLaunch L
     A.dx = M.dx; A.dy = M.dy; A.g = M.g; A.r = M.r; A.theta = M.theta;
     A.desing_eps = M.desing; A.flux_eps = M.flux_eps; A.cutoff = M.cutoff; A.land = M.land;
     A.f = M.f; A.north_x = M.north_x; A.north_y = M.north_y;
-    A.wind_t = M.wind_t; A.wx0 = M.wx0; A.wx1 = M.wx1; A.wy0 = M.wy0; A.wy1 = M.wy1;
+    A.wind_x = M.wind_x; A.wind_y = M.wind_y;
     A.dt = 0.0f;
-    A.bc_n = M.bc_n; A.bc_e = M.bc_e; A.bc_s = M.bc_s; A.bc_w = M.bc_w;
+    A.set_flags(M.bc_n, M.bc_e, M.bc_s, M.bc_w, 2, L.rk_step);
     A.vdx = M.vdx; A.udy = M.udy; A.r_two_g = M.r_two_g; A.r_dx = M.r_dx; A.r_dy = M.r_dy;
-    A.rk_order = 2; A.rk_step = L.rk_step;
     A.tiles_x = M.ftiles_x; A.tiles_y = 0;
     const int lt = t - M.ftile_begin;
     const int ty = lt / M.ftiles_x, tx = lt - ty * M.ftiles_x;
@@ -466,7 +464,8 @@ int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_confi
         M.dx = d.dx; M.dy = d.dy; M.g = d.g; M.r = d.r; M.theta = d.theta;
         M.desing = d.desing_eps; M.flux_eps = d.flux_slope_eps; M.cutoff = d.depth_cutoff;
         M.land = d.land_value; M.f = d.f; M.north_x = d.north_x; M.north_y = d.north_y;
-        M.wind_t = m.wind.t; M.wx0 = m.wind.x0; M.wx1 = m.wind.x1; M.wy0 = m.wind.y0; M.wy1 = m.wind.y1;
+        M.wind_x = wind_lerp(m.wind.t, m.wind.x0, m.wind.x1);
+        M.wind_y = wind_lerp(m.wind.t, m.wind.y0, m.wind.y1);
         M.bc_n = d.bc_north; M.bc_e = d.bc_east; M.bc_s = d.bc_south; M.bc_w = d.bc_west;
         {
             SubstepArgs D;
@@ -500,6 +499,43 @@ int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_confi
     }
     *out = E;
     return 0;
+}
+
+int gocean_ensemble_set_tile_config(gocean_ensemble_t* E, int tile_config) {
+    clear_error();
+    if (!E) return set_error(GOCEAN_EINVAL, "ensemble is NULL");
+    if (tile_config < 0) tile_config = kDefaultTileConfig;
+    if (tile_config >= kNumTileConfigs)
+        return set_error(GOCEAN_EINVAL, "tile configuration %d out of range", tile_config);
+    // work already queued may still read the old tables
+    if (hipDeviceSynchronize() != hipSuccess) return check_launch("ensemble tile configuration");
+    E->cfg = tile_config;
+    const TileConfig& tc = (E->mode == kModeZero) ? kTileConfigs[E->cfg] : kTileConfigs[0];
+    E->bw = tc.bw; E->bh = tc.bh; E->nt = tc.nt;
+    std::vector<int> tile_member;
+    for (int k = 0; k < E->n; ++k) {
+        EnsMember& M = E->members[k];
+        M.tiles_x = (M.nx + E->bw - 1) / E->bw;
+        const int tiles_y = (M.ny + E->bh - 1) / E->bh;
+        M.tile_begin = (int)tile_member.size();
+        for (int t = 0; t < M.tiles_x * tiles_y; ++t) tile_member.push_back(k);
+    }
+    int* d_map = nullptr;
+    if (hipMalloc(&d_map, sizeof(int) * tile_member.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(GOCEAN_ENOMEM, "device allocation of the tile map failed");
+    }
+    hipMemcpy(d_map, tile_member.data(), sizeof(int) * tile_member.size(), hipMemcpyHostToDevice);
+    hipMemcpy(E->d_members, E->members.data(), sizeof(EnsMember) * E->n, hipMemcpyHostToDevice);
+    hipFree(E->d_tile_member);
+    E->d_tile_member = d_map;
+    E->n_tiles = (int)tile_member.size();
+    // a captured graph holds the old launch shape: capture again
+    if (E->graph_exec) hipGraphExecDestroy(E->graph_exec);
+    if (E->graph) hipGraphDestroy(E->graph);
+    E->graph_exec = nullptr;
+    E->graph = nullptr;
+    return check_launch("ensemble tile configuration");
 }
 
 int gocean_ensemble_destroy(gocean_ensemble_t* E) {

@@ -167,6 +167,12 @@ typedef struct gocean_ensemble gocean_ensemble_t;
 int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_config,
                            gocean_ensemble_t** out);
 int gocean_ensemble_destroy(gocean_ensemble_t* ens);
+/* Switch the tile configuration of the batched substep (index into
+ * gocean_tile_configs; -1 = default) and rebuild the tile -> member map, for
+ * an autotuner.  Synchronises the device first and drops a captured graph
+ * (capture again).  Members with per-cell fields or a rotated grid always use
+ * the 32x8 tile. */
+int gocean_ensemble_set_tile_config(gocean_ensemble_t* ens, int tile_config);
 /* n_members, total tiles and kernel mode (0 zero angle, 1 constant north,
  * 2 per-cell fields). */
 int gocean_ensemble_info(const gocean_ensemble_t* ens, int* n_members, int* n_tiles, int* mode);

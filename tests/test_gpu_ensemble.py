@@ -147,3 +147,38 @@ def test_ensemble_bathymetry_and_one_substep():
             d = g[2:-2, 2:-2] != r[2:-2, 2:-2]
             assert not d.any(), f"member {k} {name}: {int(d.sum())} cells differ, first {np.argwhere(d)[:3].tolist()}"
     ens.cleanUp()
+
+
+def test_tile_config_switch_and_tuner():
+    """Every tile configuration of the batched substep gives the same bits
+    (one ensemble switched between them with gocean_ensemble_set_tile_config),
+    and the ensemble autotuner leaves the state unchanged and keeps the
+    fastest configuration."""
+    import torch
+    from SWESimulators import Common, CDKLM16Ensemble, autotune
+    _say("ensemble tile switch and tuner")
+    ms = members(seed=11)
+    ctx = Common.HIPContext(0)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, rk_order=2, courant_number=0.7)
+    start = [s.clone() for s in ens.state]
+    ref = None
+    for c in range(len(autotune.tile_config_names())):
+        for s, s0 in zip(ens.state, start):
+            s.copy_(s0)
+        ens.set_tile_config(c)
+        ens.step_n(3)
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = ens.state[0].clone()
+        else:
+            assert torch.equal(ens.state[0], ref), c
+    before = ens.state[0].clone()
+    best, times = autotune.tune_ensemble(ens, use_cache=False)
+    torch.cuda.synchronize()
+    assert torch.equal(ens.state[0], before)
+    assert ens.tile_config == best == min(times, key=times.get)
+    ens.capture(2)
+    ens.replay()
+    ens.step_n(0)
+    torch.cuda.synchronize()
+    ens.cleanUp()
