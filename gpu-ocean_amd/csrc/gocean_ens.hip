@@ -51,8 +51,9 @@ struct EnsLaunch {
     int n_tiles;
 };
 
+// 512-thread tiles: 8 waves per SIMD, as k_substep
 template <int BW, int BH, int NT, int MODE>
-__global__ __launch_bounds__(NT) void k_ens_substep(EnsLaunch L) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 8 : 1))) void k_ens_substep(EnsLaunch L) {
     __shared__ TileSmem<BW, BH, MODE> S;
     const int t = xcd_band(blockIdx.x, gridDim.x);
     const int m = L.tile_member[t];
