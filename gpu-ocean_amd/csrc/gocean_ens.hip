@@ -223,7 +223,8 @@ struct EnsTapeLaunch {
     int rk_step;
 };
 
-__global__ __launch_bounds__(kEnsTapeNT) void k_ens_substep_tape(EnsTapeLaunch L) {
+// 8 waves per SIMD, as k_ens_substep
+__global__ __launch_bounds__(kEnsTapeNT) __attribute__((amdgpu_waves_per_eu(8))) void k_ens_substep_tape(EnsTapeLaunch L) {
     __shared__ TileSmem<kEnsTapeBW, kEnsTapeBH, kModeZero, true> S;
     const int t = xcd_band(blockIdx.x, gridDim.x);
     const int m = L.tile_member[t];
