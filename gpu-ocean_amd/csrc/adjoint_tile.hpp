@@ -19,8 +19,7 @@
 //     momentum negated, periodic wrap).
 // Branch-deciding quantities are read from the record or recomputed with the
 // forward's exact operations; values that only scale derivatives use fast
-// reciprocal / square root.  The stages are __host__ __device__ so that a
-// host build can run one tile with a single "thread".
+// reciprocal / square root.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -251,10 +250,11 @@ struct AdjTile {
     __host__ __device__ float row_flip(int R) const {
         return ((A.bc_s == kWall && R < 2) || (A.bc_n == kWall && R > A.ny + 1)) ? -1.0f : 1.0f;
     }
-    __host__ __device__ bool clamped(const float* out_eta, int lr, int lc, size_t o) const {
-        if (!out_eta) return false;
+    // the forward clamped the cell dry: its output eta (v) sits at the cutoff
+    __host__ __device__ bool clamped_v(bool has, float v, int lr, int lc) const {
+        if (!has) return false;
         const float Hreg = 0.25f * (((S.Hi[lr][lc] + S.Hi[lr + 1][lc]) + S.Hi[lr][lc + 1]) + S.Hi[lr + 1][lc + 1]);
-        return out_eta[o] <= -Hreg + A.cutoff;
+        return v <= -Hreg + A.cutoff;
     }
 
     // a[R, C] of an interior cell plus the ghost entries whose image it is,
@@ -304,22 +304,54 @@ struct AdjTile {
     static constexpr int kHiItems = (H + 1) * (W + 1);
 
     // ---- stage 1: cell data
-    __host__ __device__ void stage_cells(int it) {
+    // The stage's global loads are issued by load_cell before stage 0, so
+    // their latency overlaps the corner-depth stage; stage_cells then needs
+    // only LDS and these registers.
+    struct CellIn {
+        float eta, U, V, wc, w0, w1, w2;
+        uint32_t tc;
+    };
+    __host__ __device__ bool cell_used(int lr, int lc, int R, int C) const {
+        const bool in_band = (lr >= 3 && lr < BH + 3) || (lc >= 3 && lc < BW + 3);
+        return in_band && R >= 0 && R <= A.ny + 3 && C >= 0 && C <= A.nx + 3 && !(ghost_r(R) && ghost_c(C));
+    }
+    // interior cells with a ghost image (the ones fold_read adds to)
+    __host__ __device__ bool near_boundary(int R, int C) const {
+        return R <= 3 || R >= A.ny || C <= 3 || C >= A.nx;
+    }
+    __host__ __device__ void load_cell(int it, CellIn& in) const {
+        const int lr = it / W, lc = it - lr * W;
+        const int R = Rof(lr), C = Cof(lc);
+        const int nx = A.nx, ny = A.ny;
+        in.eta = in.U = in.V = in.wc = in.w0 = in.w1 = in.w2 = 0.0f;
+        in.tc = 0u;
+        if (!cell_used(lr, lc, R, C)) return;
+        // eta of the image cell (wall mirror / periodic wrap)
+        int sr = R, sc = C;
+        if (R < 2) sr = (A.bc_s == kWall) ? 3 - R : R + ny;
+        else if (R > ny + 1) sr = (A.bc_n == kWall) ? 2 * ny + 3 - R : R - ny;
+        if (C < 2) sc = (A.bc_w == kWall) ? 3 - C : C + nx;
+        else if (C > nx + 1) sc = (A.bc_e == kWall) ? 2 * nx + 3 - C : C - nx;
+        in.eta = A.eta[(size_t)sr * A.pitch + sc];
+        const size_t o = (size_t)R * A.pitch + C;
+        in.U = A.tU[o];
+        in.V = A.tV[o];
+        const bool ri = !ghost_r(R), ci = !ghost_c(C);
+        if ((ri && C >= 1 && C <= nx + 2) || (ci && R >= 1 && R <= ny + 2)) in.tc = A.tC[o];
+        if (ri && ci) {
+            if (A.w_clamp) in.wc = A.w_clamp[o];
+            in.w0 = A.w_eta[o]; in.w1 = A.w_hu[o]; in.w2 = A.w_hv[o];
+        }
+    }
+    __host__ __device__ void stage_cells(int it, const CellIn& in) {
         const int lr = it / W, lc = it - lr * W;
         const int R = Rof(lr), C = Cof(lc);
         const int nx = A.nx, ny = A.ny;
         float E = 0.0f, U = 0.0f, V = 0.0f;
         float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f;
         uint32_t code = 0u;
-        const bool in_band = (lr >= 3 && lr < BH + 3) || (lc >= 3 && lc < BW + 3);
-        if (in_band && R >= 0 && R <= ny + 3 && C >= 0 && C <= nx + 3 && !(ghost_r(R) && ghost_c(C))) {
-            // eta of the image cell (wall mirror / periodic wrap)
-            int sr = R, sc = C;
-            if (R < 2) sr = (A.bc_s == kWall) ? 3 - R : R + ny;
-            else if (R > ny + 1) sr = (A.bc_n == kWall) ? 2 * ny + 3 - R : R - ny;
-            if (C < 2) sc = (A.bc_w == kWall) ? 3 - C : C + nx;
-            else if (C > nx + 1) sc = (A.bc_e == kWall) ? 2 * nx + 3 - C : C - nx;
-            const float eta = A.eta[(size_t)sr * A.pitch + sc];
+        if (cell_used(lr, lc, R, C)) {
+            const float eta = in.eta;
             const float Hm = cell_hm(lr, lc);
             const float h = eta + Hm;
             uint32_t br;
@@ -328,19 +360,18 @@ struct AdjTile {
                 if (h <= A.cutoff) { E = -Hm + A.cutoff; br = kBrDry; }
                 else { E = eta; br = kBrDesing; }
             } else { E = eta; br = kBrWet; }
-            const size_t o = (size_t)R * A.pitch + C;
-            U = A.tU[o];
-            V = A.tV[o];
+            U = in.U;
+            V = in.V;
             const bool ri = !ghost_r(R), ci = !ghost_c(C);
             code = br << 28;
-            if (ri && C >= 1 && C <= nx + 2) code |= A.tC[o] & 0x001F00FFu;
-            if (ci && R >= 1 && R <= ny + 2) code |= A.tC[o] & 0x03E0FF00u;
+            if (ri && C >= 1 && C <= nx + 2) code |= in.tc & 0x001F00FFu;
+            if (ci && R >= 1 && R <= ny + 2) code |= in.tc & 0x03E0FF00u;
             if (ri && ci) {
-                if (!clamped(A.w_clamp, lr, lc, o)) {
-                    if (A.w_fold) {
+                if (!clamped_v(A.w_clamp != nullptr, in.wc, lr, lc)) {
+                    if (A.w_fold && near_boundary(R, C)) {
                         w0 = fold_read(A.w_eta, 0, R, C); w1 = fold_read(A.w_hu, 1, R, C);
                         w2 = fold_read(A.w_hv, 2, R, C);
-                    } else { w0 = A.w_eta[o]; w1 = A.w_hu[o]; w2 = A.w_hv[o]; }
+                    } else { w0 = in.w0; w1 = in.w1; w2 = in.w2; }
                 }
             }
         }
@@ -357,11 +388,25 @@ struct AdjTile {
     static constexpr int kXItems = BH * W;
     __host__ __device__ bool x_row_ok(int j) const { return by + 2 + j <= A.ny + 1; }
 
-    __host__ __device__ void stage_xface(int it) {
+    __host__ __device__ bool xface_used(int j, int fi) const {
+        const int cm = Cof(fi);
+        return x_row_ok(j) && cm >= fx0 && cm <= fx1 && fi >= 1 && fi + 2 < W;
+    }
+    // the face's depth terms from the record (rows 2..ny+1, columns
+    // 1..nx+2), loaded before stage 0
+    __host__ __device__ void load_xface(int it, float& tm, float& tp) const {
         const int j = it / W, fi = it - j * W;
-        const int lr = j + 3, cm = Cof(fi);
+        tm = tp = 0.0f;
+        if (!xface_used(j, fi)) return;
+        const size_t om = (size_t)Rof(j + 3) * A.pitch + Cof(fi);
+        tm = A.tT[om];
+        tp = A.tT[om + 1];
+    }
+    __host__ __device__ void stage_xface(int it, float tm, float tp) {
+        const int j = it / W, fi = it - j * W;
+        const int lr = j + 3;
         float out[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (x_row_ok(j) && cm >= fx0 && cm <= fx1 && fi >= 1 && fi + 2 < W) {
+        if (xface_used(j, fi)) {
             const int m = fi, p = fi + 1;
             const float Em = S.E[lr][m], Ep = S.E[lr][p];
             if (!(Em == kDryFlag || Ep == kDryFlag)) {
@@ -378,10 +423,8 @@ struct AdjTile {
                 const float Rpx = S.U[lr][p] - 0.5f * Sup, Rpy = S.V[lr][p] - 0.5f * Svp;
                 const float Rmx = S.U[lr][m] + 0.5f * Sum, Rmy = S.V[lr][m] + 0.5f * Svm;
                 const float Hf = 0.5f * (S.Hi[lr][p] + S.Hi[lr + 1][p]);
-                // face depth terms from the record: rows 2..ny+1, columns 1..nx+2
-                const size_t om = (size_t)Rof(lr) * A.pitch + cm;
-                const float hp_raw = (Ep + Hf) - A.tT[om + 1];
-                const float hm_raw = (Em + Hf) + A.tT[om];
+                const float hp_raw = (Ep + Hf) - tp;
+                const float hm_raw = (Em + Hf) + tm;
                 const float hp = fmaxf(0.0f, hp_raw), hm = fmaxf(0.0f, hm_raw);
                 float ghm, gnm, gtm, ghp, gnp, gtp;
                 flux_vjp(hm, Rmx, Rmy, hp, Rpx, Rpy, (km >> kCodeFx) & 31u, A.g, A.cutoff, l0, l1, l2,
@@ -448,11 +491,25 @@ struct AdjTile {
     static constexpr int kYItems = H * BW;
     __host__ __device__ bool y_col_ok(int i) const { return bx + 2 + i <= A.nx + 1; }
 
-    __host__ __device__ void stage_yface(int it) {
+    __host__ __device__ bool yface_used(int fj, int i) const {
+        const int rm = Rof(fj);
+        return y_col_ok(i) && rm >= fy0 && rm <= fy1 && fj >= 1 && fj + 2 < H;
+    }
+    // the face's depth terms from the record (columns 2..nx+1, rows
+    // 1..ny+2), loaded before stage 0
+    __host__ __device__ void load_yface(int it, float& tm, float& tp) const {
         const int fj = it / BW, i = it - fj * BW;
-        const int lc = i + 3, rm = Rof(fj);
+        tm = tp = 0.0f;
+        if (!yface_used(fj, i)) return;
+        const size_t om = (size_t)Rof(fj) * A.pitch + Cof(i + 3);
+        tm = A.tS[om];
+        tp = A.tS[om + A.pitch];
+    }
+    __host__ __device__ void stage_yface(int it, float tm, float tp) {
+        const int fj = it / BW, i = it - fj * BW;
+        const int lc = i + 3;
         float out[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (y_col_ok(i) && rm >= fy0 && rm <= fy1 && fj >= 1 && fj + 2 < H) {
+        if (yface_used(fj, i)) {
             const int m = fj, p = fj + 1;
             const float Em = S.E[m][lc], Ep = S.E[p][lc];
             if (!(Em == kDryFlag || Ep == kDryFlag)) {
@@ -470,10 +527,8 @@ struct AdjTile {
                 const float Rpx = S.U[p][lc] - 0.5f * Sup, Rpy = S.V[p][lc] - 0.5f * Svp;
                 const float Rmx = S.U[m][lc] + 0.5f * Sum, Rmy = S.V[m][lc] + 0.5f * Svm;
                 const float Hf = 0.5f * (S.Hi[p][lc] + S.Hi[p][lc + 1]);
-                // face depth terms from the record: columns 2..nx+1, rows 1..ny+2
-                const size_t om = (size_t)rm * A.pitch + Cof(lc);
-                const float hp_raw = (Ep + Hf) - A.tS[om + A.pitch];
-                const float hm_raw = (Em + Hf) + A.tS[om];
+                const float hp_raw = (Ep + Hf) - tp;
+                const float hm_raw = (Em + Hf) + tm;
                 const float hp = fmaxf(0.0f, hp_raw), hm = fmaxf(0.0f, hm_raw);
                 float ghm, gnm, gtm, ghp, gnp, gtp;
                 flux_vjp(hm, Rmy, Rmx, hp, Rpy, Rpx, (km >> kCodeFy) & 31u, A.g, A.cutoff, l0, l1, l2,
@@ -530,10 +585,26 @@ struct AdjTile {
     }
 
     // ---- output: base / source VJP (interior cells), conversion VJP, store
-    __host__ __device__ void stage_out(int it) {
+    // The x term and its clamp value are loaded before stage 0 (load_out).
+    struct OutIn {
+        float xe, xu, xv, xc;
+    };
+    __host__ __device__ bool out_used(int lr, int lc, int R, int C) const {
+        return lr >= 1 && lc >= 1 && lr + 1 < H && lc + 1 < W && is_output(R, C);
+    }
+    __host__ __device__ void load_out(int it, OutIn& in) const {
         const int lr = it / W, lc = it - lr * W;
         const int R = Rof(lr), C = Cof(lc);
-        if (lr < 1 || lc < 1 || lr + 1 >= H || lc + 1 >= W || !is_output(R, C)) return;
+        in.xe = in.xu = in.xv = in.xc = 0.0f;
+        if (!A.x_eta || !out_used(lr, lc, R, C) || ghost_r(R) || ghost_c(C)) return;
+        const size_t o = (size_t)R * A.pitch + C;
+        if (A.x_clamp) in.xc = A.x_clamp[o];
+        in.xe = A.x_eta[o]; in.xu = A.x_hu[o]; in.xv = A.x_hv[o];
+    }
+    __host__ __device__ void stage_out(int it, const OutIn& in) {
+        const int lr = it / W, lc = it - lr * W;
+        const int R = Rof(lr), C = Cof(lc);
+        if (!out_used(lr, lc, R, C)) return;
         float gE = S.A0[lr - 1][lc - 1], gU = S.A1[lr - 1][lc - 1], gV = S.A2[lr - 1][lc - 1];
         const size_t o = (size_t)R * A.pitch + C;
         const bool interior = !ghost_r(R) && !ghost_c(C);
@@ -558,11 +629,11 @@ struct AdjTile {
                 gU += ghu * hreg; gE += ghu * U;
                 gV += ghv * hreg; gE += ghv * V;
             }
-            if (A.x_eta && !clamped(A.x_clamp, lr, lc, o)) {
-                if (A.x_fold) {
+            if (A.x_eta && !clamped_v(A.x_clamp != nullptr, in.xc, lr, lc)) {
+                if (A.x_fold && near_boundary(R, C)) {
                     xe = fold_read(A.x_eta, 0, R, C); xu = fold_read(A.x_hu, 1, R, C);
                     xv = fold_read(A.x_hv, 2, R, C);
-                } else { xe = A.x_eta[o]; xu = A.x_hu[o]; xv = A.x_hv[o]; }
+                } else { xe = in.xe; xu = in.xu; xv = in.xv; }
             }
         }
         // conversion (eta, hu, hv) -> (E, u, v)
@@ -601,49 +672,68 @@ struct AdjTile {
 #define ADJ_SYNC()
 #endif
 
-// every stage of one tile, items spread over nthreads == NT threads; a host
-// build runs the same body with NT = nthreads = 1, one "thread" that loops
-// over all items
-template <int BW, int BH, int NT>
+// every stage of one tile, one item per thread and stage (NT threads cover
+// the largest stage).  PREFETCH issues every stage's global loads first,
+// before the corner depths, so that no stage after the first barrier waits
+// on memory; it costs scalar registers (the batched kernel, which runs many
+// rounds of tiles, gains from it; one 512^2 domain, a single round, does not)
+template <int BW, int BH, int NT, bool PREFETCH>
 __host__ __device__ inline void adj_tile_body(const AdjTileArgs& A, AdjSmem<BW, BH>& S, int tx, int ty,
-                                             int tid, int nthreads) {
+                                             int tid) {
     using Tl = AdjTile<BW, BH>;
-    // items per thread of the centre stages (nthreads == NT)
-    constexpr int kPer = ((Tl::kXItems > Tl::kYItems ? Tl::kXItems : Tl::kYItems) + NT - 1) / NT;
+    static_assert(NT >= Tl::kCellItems && NT >= Tl::kXItems && NT >= Tl::kYItems,
+                  "one item per thread in every stage");
     Tl T(A, S, tx, ty);
-    for (int it = tid; it < Tl::kHiItems; it += nthreads) T.stage_hi(it);
+    typename Tl::CellIn cin;
+    typename Tl::OutIn oin;
+    float xtm = 0.0f, xtp = 0.0f, ytm = 0.0f, ytp = 0.0f;
+    const bool cell = tid < Tl::kCellItems, xi = tid < Tl::kXItems, yi = tid < Tl::kYItems;
+    if (PREFETCH) {
+        if (cell) { T.load_cell(tid, cin); T.load_out(tid, oin); }
+        if (xi) T.load_xface(tid, xtm, xtp);
+        if (yi) T.load_yface(tid, ytm, ytp);
+    }
+    for (int it = tid; it < Tl::kHiItems; it += NT) T.stage_hi(it);
     ADJ_SYNC();
-    for (int it = tid; it < Tl::kCellItems; it += nthreads) T.stage_cells(it);
+    if (cell) {
+        if (!PREFETCH) T.load_cell(tid, cin);
+        T.stage_cells(tid, cin);
+    }
     ADJ_SYNC();
-    for (int it = tid; it < Tl::kXItems; it += nthreads) T.stage_xface(it);
+    if (xi) {
+        if (!PREFETCH) T.load_xface(tid, xtm, xtp);
+        T.stage_xface(tid, xtm, xtp);
+    }
     ADJ_SYNC();
     {
         // slope centres overwrite the face rows they read: results are held
         // in registers until every centre has read them
-        float out[kPer][9];
-        for (int r = 0; r < kPer; ++r)
-            if (tid + r * nthreads < Tl::kXItems) T.stage_xcentre(tid + r * nthreads, out[r]);
+        float out[9];
+        if (xi) T.stage_xcentre(tid, out);
         ADJ_SYNC();
-        for (int r = 0; r < kPer; ++r)
-            if (tid + r * nthreads < Tl::kXItems) T.store_centre(tid + r * nthreads, out[r]);
+        if (xi) T.store_centre(tid, out);
     }
     ADJ_SYNC();
-    for (int it = tid; it < Tl::kXItems; it += nthreads) T.stage_xgather(it);
+    if (xi) T.stage_xgather(tid);
     ADJ_SYNC();
-    for (int it = tid; it < Tl::kYItems; it += nthreads) T.stage_yface(it);
+    if (yi) {
+        if (!PREFETCH) T.load_yface(tid, ytm, ytp);
+        T.stage_yface(tid, ytm, ytp);
+    }
     ADJ_SYNC();
     {
-        float out[kPer][9];
-        for (int r = 0; r < kPer; ++r)
-            if (tid + r * nthreads < Tl::kYItems) T.stage_ycentre(tid + r * nthreads, out[r]);
+        float out[9];
+        if (yi) T.stage_ycentre(tid, out);
         ADJ_SYNC();
-        for (int r = 0; r < kPer; ++r)
-            if (tid + r * nthreads < Tl::kYItems) T.store_centre(tid + r * nthreads, out[r]);
+        if (yi) T.store_centre(tid, out);
     }
     ADJ_SYNC();
-    for (int it = tid; it < Tl::kYItems; it += nthreads) T.stage_ygather(it);
+    if (yi) T.stage_ygather(tid);
     ADJ_SYNC();
-    for (int it = tid; it < Tl::kCellItems; it += nthreads) T.stage_out(it);
+    if (cell) {
+        if (!PREFETCH) T.load_out(tid, oin);
+        T.stage_out(tid, oin);
+    }
 }
 
 constexpr int kAdjBW = 16, kAdjBH = 16, kAdjNT = 512;

@@ -22,7 +22,7 @@ __global__ __launch_bounds__(kAdjNT) void k_adj_tile(AdjTileArgs A) {
     __shared__ AdjSmem<kAdjBW, kAdjBH> S;
     const int t = xcd_band(blockIdx.x, gridDim.x);
     const int ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
-    adj_tile_body<kAdjBW, kAdjBH, kAdjNT>(A, S, tx, ty, threadIdx.x, kAdjNT);
+    adj_tile_body<kAdjBW, kAdjBH, kAdjNT, false>(A, S, tx, ty, threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_adj_fold(FoldArgs F) {

@@ -267,7 +267,8 @@ struct EnsAdjLaunch {
     int w_fold, x_fold;                            // as AdjTileArgs
 };
 
-__global__ __launch_bounds__(kAdjNT) void k_ens_adj_tile(EnsAdjLaunch L) {
+// prefetching tile body at 8 waves per SIMD (four 40 KB workgroups per CU)
+__global__ __launch_bounds__(kAdjNT) __attribute__((amdgpu_waves_per_eu(8))) void k_ens_adj_tile(EnsAdjLaunch L) {
     __shared__ AdjSmem<kAdjBW, kAdjBH> S;
     const int t = xcd_band(blockIdx.x, gridDim.x);
     const int m = L.tile_member[t];
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(kAdjNT) void k_ens_adj_tile(EnsAdjLaunch L) {
     A.Ddx = M.Ddx; A.Ddy = M.Ddy; A.g_dx = M.g_dx; A.g_dy = M.g_dy;
     const int lt = t - M.atile_begin;
     const int ty = lt / M.atiles_x, tx = lt - ty * M.atiles_x;
-    adj_tile_body<kAdjBW, kAdjBH, kAdjNT>(A, S, tx, ty, threadIdx.x, kAdjNT);
+    adj_tile_body<kAdjBW, kAdjBH, kAdjNT, true>(A, S, tx, ty, threadIdx.x);
 }
 
 // member blockIdx.y's three state planes from src to dst
