@@ -376,6 +376,7 @@ static int ens_launch(const gocean_ensemble* E, EnsLaunch L, hipStream_t s) {
             case 3: ens_launch_t<16, 16, 256, kModeZero>(L, s); break;
             case 4: ens_launch_t<64, 4, 256, kModeZero>(L, s); break;
             case 5: ens_launch_t<32, 16, 512, kModeZero>(L, s); break;
+            case 6: ens_launch_t<32, 14, 512, kModeZero>(L, s); break;
         }
     }
     return check_launch("ensemble substep launch");

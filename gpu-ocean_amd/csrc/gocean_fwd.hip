@@ -87,6 +87,9 @@ const TileConfig kTileConfigs[] = {
     {16, 16, 256, "16x16/256"},
     {64, 4, 256, "64x4/256"},
     {32, 16, 512, "32x16/512"},
+    // 14 rows: the slope and face phases fit two passes of 512 threads
+    // (16 rows need a third pass for 96 / 48 threads)
+    {32, 14, 512, "32x14/512"},
 };
 const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 const int kDefaultTileConfig = 1;
@@ -150,6 +153,7 @@ int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s) {
             case 3: launch_substep_t<16, 16, 256, kModeZero>(A, s); break;
             case 4: launch_substep_t<64, 4, 256, kModeZero>(A, s); break;
             case 5: launch_substep_t<32, 16, 512, kModeZero>(A, s); break;
+            case 6: launch_substep_t<32, 14, 512, kModeZero>(A, s); break;
         }
     }
     return check_launch("cdklm substep launch");

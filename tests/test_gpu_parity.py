@@ -104,7 +104,7 @@ def test_substep_bitexact(bcname, bc, feat, rk):
         assert nbad == 0, f"{name}: {nbad} cells differ, max |d| = {np.max(np.abs(gi - ri))}"
 
 
-@pytest.mark.parametrize("tile", range(6))
+@pytest.mark.parametrize("tile", range(7))
 @pytest.mark.parametrize("bcname,bc", [BCS[0], BCS[1], BCS[3]])
 def test_tile_configs_bitexact(tile, bcname, bc):
     _say("tile config", tile, bcname)
