@@ -843,8 +843,9 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
         A.hu_out[o] = uu;
         A.hv_out[o] = uv;
         // keep the periodic ghost cells of the output consistent, as the
-        // reference's boundary kernels would after this substep
-        if (pNS || pEW) {
+        // reference's boundary kernels would after this substep; only cells
+        // within two rows / columns of a periodic side have ghost images
+        if ((pNS && (R <= 3 || R >= ny)) || (pEW && (C <= 3 || C >= nx))) {
             // ghost rows / columns this cell is the periodic image of:
             // ghost index g in {0,1} copies g+n, g in {n+2,n+3} copies g-n
             int rows[3] = {R, -1, -1}, cols[3] = {C, -1, -1};
