@@ -72,6 +72,16 @@ def _almost(a, b, places):
     assert np.all(d == 0), float(np.max(np.abs(np.asarray(a, np.float64) - b)))
 
 
+def _scale(a):
+    """Scale of a perturbation field for the 5-decimal comparisons: its
+    largest magnitude.  The reference's tests divide by the (signed) maximum;
+    for a zero-mean random field that maximum can be many times smaller than
+    the largest magnitude, which turns the 5-decimal bar into a test of
+    float32 rounding for some draws (the generators are seeded from the clock,
+    as in the reference)."""
+    return float(np.max(np.abs(a)))
+
+
 def _seeds(rs, shape):
     return (rs.rand(*shape) * O.LCG_DENOM).astype(np.uint64)
 
@@ -290,8 +300,8 @@ def _noise_obj(nx, ny, pns, pew, factor=1, use_lcg=True):
 def test_perturb_ocean_as_reference_tests(factor, pns, pew):
     """perturb_ocean / interpolate_perturbation_ocean: the device result
     against the CPU twins with eta = 0, H = 5, f = 0.02, beta = 0, compared
-    after scaling by the maximum at 5 decimals (the reference's bar); the
-    coarse buffer equals the aligned fine cells."""
+    after scaling by the largest magnitude at 5 decimals (the reference's bar,
+    see _scale); the coarse buffer equals the aligned fine cells."""
     from SWESimulators import Common
     nx, ny = (NX, NY) if factor == 1 else (NX * factor, NY * factor) if factor == 3 else (10 * factor, 8 * factor)
     _say("perturb ocean (reference setup)", factor, pns, pew, nx, ny)
@@ -309,7 +319,7 @@ def test_perturb_ocean_as_reference_tests(factor, pns, pew):
     got_e = eta.download(stream)[2:-2, 2:-2]
     got_u = hu.download(stream)[2:-2, 2:-2]
     got_v = hv.download(stream)[2:-2, 2:-2]
-    me, mu = np.max(de), np.max(dhu)
+    me, mu = _scale(de), _scale(dhu)
     _almost(got_e / me, de[2:-2, 2:-2] / me, 5)
     _almost(got_u / mu, dhu / mu, 5)
     _almost(got_v / mu, dhv / mu, 5)
@@ -333,7 +343,7 @@ def test_interpolation_offset_as_reference_test():
     de, _, _ = O.cpu_twin_perturbation(xi, HCPU, nx, ny, DX, DY, factor, float(noise.soar_q0),
                                        float(noise.soar_L), F, 0.0, G_, True, True)
     etaCPU = de[2:-2, 2:-2]
-    mx = np.max(etaCPU)
+    mx = _scale(etaCPU)
     for align, off in zip((6, 7, 8), (1, 0, -1)):
         eta = Common.DeviceArray2D(stream, nx, ny, 2, 2, z)
         hu = Common.DeviceArray2D(stream, nx, ny, 2, 2, z)
