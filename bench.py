@@ -234,8 +234,10 @@ def bench_adjoint(ctx, n=512, steps=500, K=50, n_obs=64, seed=44):
     import torch
     from SWESimulators import BathymetryAndICs as B, adjoint as A
     rng = np.random.default_rng(seed)
+    from SWESimulators import autotune
     inp = B.config4(n=n)
     sim = make_sim(inp, ctx)
+    autotune.tune_simulator(sim)       # the forward it is compared with: the fastest tile
     dt = sim.dt
     st = sim.gpu_stream
     ox = rng.uniform(2.0, n - 2.0, n_obs) * inp.dx
