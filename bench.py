@@ -565,6 +565,81 @@ def bench_etkf(ctx, members=100, n=512, n_drifters=64, reps=5):
             "algorithmic_GBs": round(3 * state_bytes / t / 1e9, 1)}
 
 
+def _double_jet_members(members, nx=500, ny=300, dx=2000.0, seed=12):
+    """Members of the double-jet data-assimilation domain (500 x 300 cells,
+    periodic east-west, walls north-south; DoubleJetCase.py:136-141) with
+    synthetic random states; flat 1000 m depth."""
+    rs = np.random.default_rng(seed)
+    j, i = np.mgrid[0:ny + 4, 0:nx + 4]
+    jet = np.exp(-((j - ny / 4) / 12.0) ** 2) - np.exp(-((j - 3 * ny / 4) / 12.0) ** 2)
+    Hi = np.full((ny + 5, nx + 5), 1000.0, dtype=np.float32)
+    out = []
+    for _ in range(members):
+        out.append(dict(nx=nx, ny=ny, dx=dx, dy=dx, g=9.81, f=1.2e-4, r=0.0, bc=(1, 2, 1, 2), Hi=Hi,
+                        eta=(0.3 * jet + 0.02 * rs.standard_normal((ny + 4, nx + 4))).astype(np.float32),
+                        hu=(50.0 * jet + 5.0 * rs.standard_normal((ny + 4, nx + 4))).astype(np.float32),
+                        hv=(5.0 * rs.standard_normal((ny + 4, nx + 4))).astype(np.float32)))
+    return out
+
+
+def bench_letkf(ctx, members=100, spacing=(25, 50), r=15.0, reps=10, cpu=True):
+    """Localised ETKF (ETKFOcean.LETKF, ETKFOcean.py:583-744) on the double
+    jet data-assimilation setup of the reference's experiments: 100 members of
+    500 x 300, static buoys every 25 cells ("all", 240 buoys in 4 groups) and
+    every 50 cells ("sparse", 60 buoys), r = 15 (boxes of 47 x 47 cells).
+    Algorithmic bytes per analysis: every cell with a nonzero weight, per
+    group, read and written once for all members and fields
+    (2 x 3 x N x 4 B).  The CPU baseline is the oracle (the reference's numpy
+    algorithm: per-buoy eigh and solve on N x N matrices) on one thread."""
+    import torch
+    from SWESimulators import CDKLM16Ensemble, ETKFOcean
+    nx, ny, dx = 500, 300, 2000.0
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, _double_jet_members(members), courant_number=0.7)
+    rs = np.random.default_rng(21)
+    out = {"workload": f"{members} members x {nx}x{ny} (double jet), r = {r}, buoys at cell centres"}
+    for step in spacing:
+        ys, xs = np.mgrid[0:ny:step, 0:nx:step]
+        pos = np.column_stack([(xs.ravel() + 0.5) * dx, (ys.ravel() + 0.5) * dx])
+        obs = np.column_stack([pos, 5.0 * rs.standard_normal((len(pos), 2))])
+        et = ETKFOcean.ETKFOcean(ens, np.diag([1.0, 1.0]))
+        et.LETKF(obs, r_factor=r)                       # geometry set-up and warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            et.LETKF(obs)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        S = et._letkf
+        covered = 0
+        jj, ii = np.nonzero(S["W_loc"])
+        R = S["local_n"] // 2
+        for g in S["groups"]:
+            mask = np.zeros((ny, nx), dtype=bool)
+            for o in g:
+                bx = (int(np.round(pos[o, 0] / dx)) - R) % nx
+                by = (int(np.round(pos[o, 1] / dx)) - R) % ny
+                mask[(by + jj) % ny, (bx + ii) % nx] = True
+            covered += int(mask.sum())
+        alg = 2 * 3 * members * 4 * covered
+        leg = {"buoys": len(pos), "groups": len(S["groups"]), "analysis_ms": round(t * 1e3, 3),
+               "covered_cells": covered, "algorithmic_MB": round(alg / 1e6, 1),
+               "algorithmic_GBs": round(alg / t / 1e9, 1)}
+        if cpu:
+            from threadpoolctl import threadpool_limits
+            from oracle import etkf_np as E
+            X = np.stack([np.stack(ens.download(k, interior_domain_only=True)) for k in range(members)])
+            with threadpool_limits(1):
+                t0 = time.perf_counter()
+                E.letkf(X, obs, pos, np.diag([1.0, 1.0]), r, dx, dx)
+                tc = time.perf_counter() - t0
+            leg["cpu_baseline"] = {"value": round(tc * 1e3, 1), "unit": "ms per analysis", "cores": 1,
+                                   "kind": "port", "sample": "the whole analysis"}
+            leg["gpu_over_cpu"] = round(tc / t, 1)
+        out[f"every_{step}_cells"] = leg
+    ens.cleanUp()
+    return out
+
+
 def measured_traffic(cells):
     """HBM bytes per substep launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy), when
@@ -591,6 +666,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-c5", action="store_true", help="skip the batched uneven ensemble adjoint")
     ap.add_argument("--no-model-error", action="store_true", help="skip the model-error leg")
+    ap.add_argument("--letkf-only", action="store_true",
+                    help="run only the LETKF leg and print its JSON (for profiling it)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -604,6 +681,9 @@ def main():
 
     ctx = Common.HIPContext(0)
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
+    if a.letkf_only:
+        print(json.dumps({"letkf": bench_letkf(ctx, cpu=not a.no_cpu)}), flush=True)
+        return
     inp = B.config2(nx=a.n, ny=a.n)
     log(f"[bench] workload {inp.name}: {inp.nx}x{inp.ny}, steps {a.steps}, warmup {a.warmup}")
     sim = make_sim(inp, ctx, tile_config=a.tile)
@@ -652,6 +732,9 @@ def main():
         log("[bench] ETKF analysis on the C3 ensemble")
         extra["etkf"] = bench_etkf(ctx)
         log(f"[bench] etkf: {extra['etkf']}")
+        log("[bench] LETKF analysis on the double-jet ensemble")
+        extra["letkf"] = bench_letkf(ctx, cpu=not a.no_cpu)
+        log(f"[bench] letkf: {extra['letkf']}")
 
     if not a.no_model_error:
         log("[bench] model error (OceanStateNoise) on the headline domain and the ensemble")

@@ -203,6 +203,26 @@ PROTOTYPES.extend([
 ])
 
 
+class LetkfLayout(ctypes.Structure):
+    """gocean_letkf_layout_t"""
+    _fields_ = [("n_members", ctypes.c_int32), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+                ("pitch", ctypes.c_int32), ("plane", ctypes.c_int64), ("member_stride", ctypes.c_int64)]
+
+
+# sizeof(gocean_letkf_coef_t): two floats, two doubles
+LETKF_COEF_BYTES = 24
+
+PROTOTYPES.extend([
+    ("gocean_letkf_tile_shape", c_int, [ctypes.POINTER(ctypes.c_int32)] * 4),
+    ("gocean_letkf_coefficients", c_int,
+     [ctypes.POINTER(LetkfLayout), c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p, ctypes.c_int32,
+      ctypes.POINTER(ctypes.c_double), ctypes.c_double, c_void_p, c_void_p, c_void_p]),
+    ("gocean_letkf_apply", c_int,
+     [ctypes.POINTER(LetkfLayout), c_void_p, c_void_p, ctypes.c_int32, c_void_p, ctypes.c_int32, c_void_p,
+      c_void_p, c_void_p, ctypes.c_int32, c_void_p, ctypes.c_int32, c_void_p]),
+])
+
+
 
 _lib = None
 _lock = threading.Lock()

@@ -498,6 +498,60 @@ int gocean_drift_batch_step(gocean_drift_batch_t* batch, const float* state, con
 int gocean_observe_flow_batch(gocean_drift_batch_t* batch, const float* state, const float* positions,
                               int shared, float* observations, void* stream);
 
+/* ---- Localised ETKF (ETKFOcean.LETKF, gpu_ocean/SWESimulators/ETKFOcean.py:583-744) ----
+ * The ensemble state is n_members blocks of three (ny+4) x pitch float planes
+ * (eta, hu, hv), plane floats apart, member_stride floats between members;
+ * interior cell (x, y) sits at (y+2) * pitch + x+2 of a plane.  One group of
+ * observations (ETKFOcean.initializeGroups, :478-514) is analysed by
+ * gocean_letkf_coefficients followed by gocean_letkf_apply on one stream;
+ * groups run one after the other, and the caller refreshes periodic ghost
+ * cells afterwards (uploadAnalysis, :796-812). */
+typedef struct gocean_letkf_layout {
+    int32_t n_members, nx, ny, pitch;
+    int64_t plane;
+    int64_t member_stride;
+} gocean_letkf_layout_t;
+
+/* Per observation, written by gocean_letkf_coefficients (device memory). */
+typedef struct gocean_letkf_coef {
+    float a_eta, a_uv;   /* anomaly scale: 1, and ff^-1/2 when hu, hv are inflated */
+    double sv0, sv1;     /* member sums of the two V columns */
+} gocean_letkf_coef_t;
+
+/* Cells [x0, x0 + tile_w) x [y0, y0 + tile_h) (clipped to the domain) and the
+ * observations tile_obs[begin .. begin+count) whose nonzero weights reach them,
+ * in group order. */
+typedef struct gocean_letkf_tile {
+    int32_t x0, y0, begin, count;
+} gocean_letkf_tile_t;
+
+/* The tile shape gocean_letkf_apply expects, the most observations of one group
+ * whose nonzero weights may overlap on a cell, and the largest ensemble. */
+int gocean_letkf_tile_shape(int32_t* tile_w, int32_t* tile_h, int32_t* max_cover, int32_t* max_members);
+
+/* For each observation o = group[i], i < n_group: observe hu, hv of every
+ * member at interior cell cells[2o], cells[2o+1] (observe_particles_from_X_f,
+ * :777-793), innovation D = y[2o..2o+1] - mean, and the local transform of
+ * :651-720 (R^-1 from the 2x2 host array obs_cov, inflation: 1 none, 0 the
+ * adaptive factor of :667-671, else the factor for hu and hv) in rank-2 form:
+ * coef[o][k][0..5] = V_k0, V_k1, B^eta_k0, B^eta_k1, B^uv_k0, B^uv_k1 (float,
+ * n_obs x n_members x 6) and oc[o].  All other pointers are device memory. */
+int gocean_letkf_coefficients(const gocean_letkf_layout_t* layout, const float* state, const int32_t* group,
+                              int32_t n_group, const int32_t* cells, const double* y, int32_t n_obs,
+                              const double* obs_cov, double inflation, float* coef, gocean_letkf_coef_t* oc,
+                              void* stream);
+
+/* In place, for every cell of the tiles: x_e <- W_f x_e + sum over covering
+ * observations o of w_loc(o) (mean + a (x_e - mean) + p . B_e), with w_loc the
+ * local_n x local_n weight stencil (float64) placed at the box origin
+ * boxes[2o], boxes[2o+1] (periodic), W_f = 1 - W / max(W, 1), W the sum of the
+ * covering weights (getCombinedWeights and the blend, :421-443, 464-468,
+ * 722-740). */
+int gocean_letkf_apply(const gocean_letkf_layout_t* layout, float* state, const double* w_loc, int32_t local_n,
+                       const int32_t* boxes, int32_t n_obs, const float* coef, const gocean_letkf_coef_t* oc,
+                       const gocean_letkf_tile_t* tiles, int32_t n_tiles, const int32_t* tile_obs,
+                       int32_t n_tile_obs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -229,3 +229,41 @@ def test_noise_batch_rejects_bad_member():
     assert lib.gocean_noise_batch_create(arr, 2, G.RNG_LCG_NORMAL, ctypes.byref(h)) == -1
     assert "Coriolis" in lib.gocean_last_error().decode()
     assert lib.gocean_noise_batch_create(arr, 2, 9, ctypes.byref(h)) == -1
+
+
+def test_letkf_shape_and_argument_checks():
+    lib = G.load()
+    v = [ctypes.c_int32() for _ in range(4)]
+    assert lib.gocean_letkf_tile_shape(*[ctypes.byref(x) for x in v]) == 0
+    assert [x.value for x in v] == [64, 1, 4, 1024]
+    lay = G.LetkfLayout(8, 30, 20, 34, 24 * 34, 3 * 24 * 34)
+    cov = (ctypes.c_double * 4)(0.1, 0.0, 0.0, 0.2)
+    ok = dict(state=16, group=16, cells=16, y=16, coef=16, oc=16)
+
+    def coef_call(layout=lay, cov=cov, infl=1.0, **kw):
+        a = dict(ok, **kw)
+        return lib.gocean_letkf_coefficients(layout, a["state"], a["group"], 1, a["cells"], a["y"], 4, cov, infl,
+                                             a["coef"], a["oc"], None)
+
+    assert coef_call(layout=None) == -1 and "layout" in lib.gocean_last_error().decode()
+    assert coef_call(layout=G.LetkfLayout(1, 30, 20, 34, 24 * 34, 3 * 24 * 34)) == -1
+    assert "members" in lib.gocean_last_error().decode()
+    assert coef_call(layout=G.LetkfLayout(8, 30, 20, 33, 24 * 34, 3 * 24 * 34)) == -1
+    assert "pitch" in lib.gocean_last_error().decode()
+    assert coef_call(layout=G.LetkfLayout(8, 30, 20, 34, 24 * 34, 2 * 24 * 34)) == -1
+    assert "stride" in lib.gocean_last_error().decode()
+    assert coef_call(state=None) == -1 and "NULL" in lib.gocean_last_error().decode()
+    assert coef_call(cov=(ctypes.c_double * 4)(0.1, 0.3, 0.3, 0.2)) == -1
+    assert "positive definite" in lib.gocean_last_error().decode()
+    assert coef_call(cov=(ctypes.c_double * 4)(0.1, 0.01, 0.0, 0.2)) == -1
+    assert coef_call(infl=-1.0) == -1 and "inflation" in lib.gocean_last_error().decode()
+    lay2 = G.LetkfLayout(2, 30, 20, 34, 24 * 34, 3 * 24 * 34)
+    assert coef_call(layout=lay2, infl=0.0) == -1 and "adaptive" in lib.gocean_last_error().decode()
+
+    def apply_call(local_n=5, n_tiles=1, tiles=16):
+        return lib.gocean_letkf_apply(lay, 16, 16, local_n, 16, 4, 16, 16, tiles, n_tiles, 16, 3, None)
+
+    assert apply_call(local_n=4) == -1 and "odd" in lib.gocean_last_error().decode()
+    assert apply_call(local_n=21) == -1 and "fit" in lib.gocean_last_error().decode()
+    assert apply_call(tiles=None) == -1 and "NULL" in lib.gocean_last_error().decode()
+    assert apply_call(n_tiles=-1) == -1
