@@ -194,13 +194,15 @@ __global__ __launch_bounds__(kCoefNT) void k_letkf_coef(CoefArgs A) {
 
 // One workgroup per (tile row of kTileW cells, field): wave w of the
 // workgroup takes members [w P, (w+1) P), P = ceil(N / kChunks), of every
-// cell of the row.  The member sums of pass 1 are combined across the waves
-// through LDS in a fixed order; with REG the wave's members stay in registers
-// between the passes, so the state is read once and written once.
+// cell of the row.  Three phases, the member sums of the first two combined
+// across the waves through LDS in a fixed order: the member mean (float64);
+// p_j = sum_k (x_k - mean) V_kj (float32: the anomalies are small numbers
+// without cancellation); the blend.  With REG the wave's members stay in
+// registers across the phases, so the state is read once and written once.
 template <bool REG>
 __global__ __launch_bounds__(kTileW* kChunks) void k_letkf_apply(ApplyArgs A) {
-    constexpr int kSums = 1 + 2 * kMaxCover;
-    __shared__ double red[kChunks][kSums][kTileW];
+    __shared__ double red_s[kChunks][kTileW];
+    __shared__ float red_p[kChunks][2 * kMaxCover][kTileW];
     const gocean_letkf_tile_t t = A.tiles[blockIdx.x];
     const int f = blockIdx.y;
     const int lane = (int)(threadIdx.x % kTileW), chunk = (int)(threadIdx.x / kTileW);
@@ -238,19 +240,19 @@ __global__ __launch_bounds__(kTileW* kChunks) void k_letkf_apply(ApplyArgs A) {
         wsum += w;
     }
 
-    const float* cf[kMaxCover];
+    const float2* cf[kMaxCover];      // per member: (V0, V1), (B_eta0, B_eta1), (B_uv0, B_uv1)
 #pragma unroll
-    for (int c = 0; c < kMaxCover; ++c) cf[c] = A.coef + (long long)cov[c] * A.n_members * kCoefPerMember;
+    for (int c = 0; c < kMaxCover; ++c)
+        cf[c] = reinterpret_cast<const float2*>(A.coef + (long long)cov[c] * A.n_members * kCoefPerMember);
     const int N = A.n_members;
     const int per = (N + kChunks - 1) / kChunks;
     const int k0 = chunk * per, k1 = min(N, k0 + per);
     const long long ms = A.member_stride;
     float* q = A.state + f * A.plane + (long long)(y + 2) * A.pitch + x + 2;
+    constexpr int kPairs = kCoefPerMember / 2;
 
-    // pass 1: member sums of x, x V_0, x V_1 over this wave's members
-    double s = 0.0, t0[kMaxCover], t1[kMaxCover];
-#pragma unroll
-    for (int c = 0; c < kMaxCover; ++c) t0[c] = t1[c] = 0.0;
+    // phase 1: member mean
+    double s = 0.0;
     float xs[REG ? kRegMembers : 1];
     if (nc > 0) {
         if constexpr (REG) {
@@ -259,69 +261,79 @@ __global__ __launch_bounds__(kTileW* kChunks) void k_letkf_apply(ApplyArgs A) {
                 if (k0 + j < k1) xs[j] = q[(k0 + j) * ms];
 #pragma unroll
             for (int j = 0; j < kRegMembers; ++j)
-                if (k0 + j < k1) {
-                    const double v = xs[j];
-                    s += v;
-#pragma unroll
-                    for (int c = 0; c < kMaxCover; ++c)
-                        if (c < nc) {
-                            t0[c] += v * cf[c][(k0 + j) * kCoefPerMember];
-                            t1[c] += v * cf[c][(k0 + j) * kCoefPerMember + 1];
-                        }
-                }
+                if (k0 + j < k1) s += (double)xs[j];
         } else {
-            for (int k = k0; k < k1; ++k) {
-                const double v = q[k * ms];
-                s += v;
-#pragma unroll
-                for (int c = 0; c < kMaxCover; ++c)
-                    if (c < nc) {
-                        t0[c] += v * cf[c][k * kCoefPerMember];
-                        t1[c] += v * cf[c][k * kCoefPerMember + 1];
-                    }
-            }
+            for (int k = k0; k < k1; ++k) s += (double)q[k * ms];
         }
     }
-    red[chunk][0][lane] = s;
+    red_s[chunk][lane] = s;
+    __syncthreads();
+    s = 0.0;
+#pragma unroll
+    for (int h = 0; h < kChunks; ++h) s += red_s[h][lane];
+    const float mean = (float)(s / N);
+
+    // phase 2: p_j = sum_k (x_k - mean) V_kj
+    float p0[kMaxCover], p1[kMaxCover];
+#pragma unroll
+    for (int c = 0; c < kMaxCover; ++c) p0[c] = p1[c] = 0.0f;
+    auto project = [&](int k, float v) {
+        const float d = v - mean;
+#pragma unroll
+        for (int c = 0; c < kMaxCover; ++c)
+            if (c < nc) {
+                const float2 V = cf[c][k * kPairs];
+                p0[c] += d * V.x;
+                p1[c] += d * V.y;
+            }
+    };
+    if (nc > 0) {
+        if constexpr (REG) {
+#pragma unroll
+            for (int j = 0; j < kRegMembers; ++j)
+                if (k0 + j < k1) project(k0 + j, xs[j]);
+        } else {
+            for (int k = k0; k < k1; ++k) project(k, q[k * ms]);
+        }
+    }
 #pragma unroll
     for (int c = 0; c < kMaxCover; ++c) {
-        red[chunk][1 + 2 * c][lane] = t0[c];
-        red[chunk][2 + 2 * c][lane] = t1[c];
+        red_p[chunk][2 * c][lane] = p0[c];
+        red_p[chunk][2 * c + 1][lane] = p1[c];
     }
     __syncthreads();
     if (nc == 0) return;                                // outside every box: forecast kept
-    s = 0.0;
 #pragma unroll
-    for (int c = 0; c < kMaxCover; ++c) t0[c] = t1[c] = 0.0;
+    for (int c = 0; c < kMaxCover; ++c) p0[c] = p1[c] = 0.0f;
 #pragma unroll
     for (int h = 0; h < kChunks; ++h) {
-        s += red[h][0][lane];
 #pragma unroll
         for (int c = 0; c < kMaxCover; ++c) {
-            t0[c] += red[h][1 + 2 * c][lane];
-            t1[c] += red[h][2 + 2 * c][lane];
+            p0[c] += red_p[h][2 * c][lane];
+            p1[c] += red_p[h][2 * c + 1][lane];
         }
     }
-    const double wf = 1.0 - wsum / fmax(wsum, 1.0);     // ETKFOcean.py:465-468
-    const double mean = s / N;
-    double p0[kMaxCover], p1[kMaxCover], a[kMaxCover];
+
+    // phase 3: x_e <- W_f x_e + sum_o w_o (mean + a (x_e - mean) + p . B_e)
+    const float wf = (float)(1.0 - wsum / fmax(wsum, 1.0));   // ETKFOcean.py:465-468
+    float a[kMaxCover], w[kMaxCover];
 #pragma unroll
     for (int c = 0; c < kMaxCover; ++c) {
         const gocean_letkf_coef_t r = A.oc[cov[c]];
-        p0[c] = t0[c] - mean * r.sv0;                   // sum_k (x_k - mean) V_k0
-        p1[c] = t1[c] - mean * r.sv1;
         a[c] = f == 0 ? r.a_eta : r.a_uv;
+        w[c] = (float)wl[c];
     }
-    const int b = f == 0 ? 2 : 4;
-    // pass 2: blend the local analyses into the forecast, this wave's members
-    auto blend = [&](int k, double v) {
-        double acc = wf * v;
+    const int b = f == 0 ? 1 : 2;
+    auto blend = [&](int k, float v) {
+        const float d = v - mean;
+        float acc = wf * v;
 #pragma unroll
         for (int c = 0; c < kMaxCover; ++c)
-            if (c < nc)
-                acc += wl[c] * (mean + a[c] * (v - mean) + p0[c] * cf[c][k * kCoefPerMember + b] +
-                                p1[c] * cf[c][k * kCoefPerMember + b + 1]);
-        q[k * ms] = (float)acc;
+            if (c < nc) {
+                const float2 B = cf[c][k * kPairs + b];
+                acc += w[c] * (mean + a[c] * d + p0[c] * B.x + p1[c] * B.y);
+            }
+        q[k * ms] = acc;
     };
     if constexpr (REG) {
 #pragma unroll

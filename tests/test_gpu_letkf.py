@@ -41,6 +41,7 @@ CASES = [
     ("random_many_groups", 24, 64, 48, 3.0, 20, 2.0, 1.3),
     ("random_adaptive", 40, 50, 50, 2.5, 15, 1.0, 0.0),
     ("lattice", 20, 60, 45, 2.0, 6, 1.0, 1.0),
+    ("many_members", 300, 32, 30, 2.0, 4, 1.0, 0.0),   # members beyond the register path and one coef sweep
 ]
 
 
