@@ -209,6 +209,20 @@ class LetkfLayout(ctypes.Structure):
                 ("pitch", ctypes.c_int32), ("plane", ctypes.c_int64), ("member_stride", ctypes.c_int64)]
 
 
+PROTOTYPES.extend([
+    ("gocean_iewpf_dots", c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p]),
+    ("gocean_iewpf_make_perpendicular", c_int,
+     [c_void_p, c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p]),
+    ("gocean_iewpf_axpby", c_int,
+     [c_void_p, c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_float, c_void_p]),
+    ("gocean_iewpf_local_svd", c_int,
+     [c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p, ctypes.c_int32,
+      c_void_p]),
+    ("gocean_iewpf_kalman_patch", c_int,
+     [c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p, ctypes.c_int32,
+      ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_void_p]),
+])
+
 # sizeof(gocean_letkf_coef_t): two floats, two doubles
 LETKF_COEF_BYTES = 24
 

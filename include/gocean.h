@@ -498,6 +498,41 @@ int gocean_drift_batch_step(gocean_drift_batch_t* batch, const float* state, con
 int gocean_observe_flow_batch(gocean_drift_batch_t* batch, const float* state, const float* positions,
                               int shared, float* observations, void* stream);
 
+/* ---- Two-stage IEWPF (IEWPFOcean.iewpf_2stage, gpu_ocean/SWESimulators/IEWPFOcean.py:195-329) ----
+ * Coarse random fields of every member of an ensemble: member m's field is n
+ * floats (coarse_ny rows of coarse_nx) at offsets[m] (device int64) of a flat
+ * buffer, the layout of gocean_noise_batch_layout for doubly periodic
+ * domains.  One launch covers every member where the reference launches per
+ * particle. */
+
+/* out[3m..3m+2] = |xi|^2, |nu|^2, xi . nu of member m, float64
+ * (squareSumDouble, reductions.cu; OceanStateNoise.findDoubleNormAndDot). */
+int gocean_iewpf_dots(const float* xi, const float* nu, const int64_t* offsets, int32_t n_members, int32_t n,
+                      double* out, void* stream);
+/* nu <- sqrt(|nu|^2 / |nu_perp|^2) (nu - (xi.nu / |xi|^2) xi) with dots from
+ * gocean_iewpf_dots (makePerpendicular, ocean_noise.cu:206-243). */
+int gocean_iewpf_make_perpendicular(const float* xi, float* nu, const int64_t* offsets, int32_t n_members,
+                                    int32_t n, const double* dots, void* stream);
+/* xi <- a[m] xi + b nu, a per member on the device (blas_xaxpby,
+ * iewpf_kernels.cu:47-66, as addBetaNuIntoAlphaXi uses it, IEWPFOcean.py:522-546). */
+int gocean_iewpf_axpby(float* xi, const float* nu, const int64_t* offsets, int32_t n_members, int32_t n,
+                       const float* a, float b, void* stream);
+/* For each drifter in order, the 7 x 7 window of xi centred on coarse cell
+ * cells[2d], cells[2d+1] (periodic) <- svd (49 x 49, row-major) x window
+ * (localSVDOnGlobalXi, iewpf_kernels.cu:213-258; applySVDtoPerpendicular,
+ * IEWPFOcean.py:693-711). */
+int gocean_iewpf_local_svd(float* xi, const int64_t* offsets, int32_t n_members, int32_t coarse_nx,
+                           int32_t coarse_ny, const float* svd, const int32_t* cells, int32_t n_drifters,
+                           void* stream);
+/* Member m's field <- zero except the 7 x 7 window centred on coarse cell
+ * (cx, cy) (periodic), which gets e[m][drifter][0] patch[0] + e[m][drifter][1]
+ * patch[1] (patch: 2 x 49 floats, e: n_members x n_drifters x 2 floats)
+ * (setNoiseBufferToZero + halfTheKalmanGain, IEWPFOcean.py:509-520, 548-595;
+ * iewpf_kernels.cu:34-45, 115-176). */
+int gocean_iewpf_kalman_patch(float* xi, const int64_t* offsets, int32_t n_members, int32_t coarse_nx,
+                              int32_t coarse_ny, const float* patch, const float* e, int32_t n_drifters,
+                              int32_t drifter, int32_t cx, int32_t cy, void* stream);
+
 /* ---- Localised ETKF (ETKFOcean.LETKF, gpu_ocean/SWESimulators/ETKFOcean.py:583-744) ----
  * The ensemble state is n_members blocks of three (ny+4) x pitch float planes
  * (eta, hu, hv), plane floats apart, member_stride floats between members;
