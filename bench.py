@@ -640,6 +640,64 @@ def bench_letkf(ctx, members=100, spacing=(25, 50), r=15.0, reps=10, cpu=True):
     return out
 
 
+def bench_iewpf(ctx, members=100, nx=500, ny=300, factor=5, n_drifters=64, reps=5, cpu=True, cpu_particles=2):
+    """Two-stage IEWPF (IEWPFOcean.iewpf_2stage, IEWPFOcean.py:195-329) on
+    100 particles of 500 x 300 with model error interpolated by 5 (the
+    double-jet experiments' sizes), doubly periodic and of constant depth
+    (the filter's assumptions), 64 drifters at random positions, synthetic
+    states.  The Kalman-gain basis is built in the warm-up call.  The CPU
+    baseline is the oracle (the reference's algorithm in numpy) on
+    ``cpu_particles`` particles, per particle."""
+    import torch
+    from SWESimulators import CDKLM16Ensemble, IEWPFOcean
+    dx = 2000.0
+    rs = np.random.default_rng(31)
+    Hi = np.full((ny + 5, nx + 5), 1000.0, dtype=np.float32)
+    ms = [dict(nx=nx, ny=ny, dx=dx, dy=dx, g=9.81, f=1.2e-4, r=0.0, bc=(2, 2, 2, 2), Hi=Hi,
+               eta=(0.05 * rs.standard_normal((ny + 4, nx + 4))).astype(np.float32),
+               hu=(5.0 * rs.standard_normal((ny + 4, nx + 4))).astype(np.float32),
+               hv=(5.0 * rs.standard_normal((ny + 4, nx + 4))).astype(np.float32)) for _ in range(members)]
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, courant_number=0.7)
+    q0 = 5e-3
+    ens.enable_model_error(soar_q0=q0, interpolation_factor=factor, seed=3)
+    ens.enable_drifters(n_drifters)
+    pos = np.column_stack([rs.uniform(0, nx * dx, n_drifters), rs.uniform(0, ny * dx, n_drifters)])
+    R = np.eye(2) * 0.5 ** 2
+    iw = IEWPFOcean.IEWPFOcean(ens, R)
+    HX = iw.observeParticles(pos)
+    obs = np.column_stack([pos, HX.mean(axis=0) + 0.5 * rs.standard_normal((n_drifters, 2))])
+    t0 = time.perf_counter()
+    iw.iewpf_2stage(obs)                                # builds the Kalman-gain basis
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        iw.iewpf_2stage(obs)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    out = {"workload": f"{members} particles x {nx}x{ny}, model error interpolated by {factor}, "
+                       f"{n_drifters} drifters, doubly periodic, constant depth",
+           "analysis_ms": round(t * 1e3, 2), "per_particle_ms": round(t * 1e3 / members, 4),
+           "first_call_with_basis_ms": round(t_setup * 1e3, 1)}
+    if cpu:
+        from oracle import iewpf_np as I
+        p = I.Params(nx=nx, ny=ny, dx=dx, dy=dx, factor=factor, q0=q0, L=0.75 * np.float32(dx * factor),
+                     f=1.2e-4, g=9.81, H=1000.0, R=R)
+        states = np.stack([np.stack(ens.download(k)) for k in range(cpu_particles)]).astype(np.float64)
+        cn = (nx // factor, ny // factor)
+        xi = [rs.standard_normal((cn[1], cn[0])) for _ in range(cpu_particles)]
+        nu = [rs.standard_normal((cn[1], cn[0])) for _ in range(cpu_particles)]
+        inn = 0.5 * rs.standard_normal((cpu_particles, n_drifters, 2))
+        t0 = time.perf_counter()
+        I.iewpf_2stage(p, iw.S_host, iw.localSVD_host, states, Hi.astype(np.float64), pos, inn, xi, nu)
+        tc = (time.perf_counter() - t0) / cpu_particles
+        out["cpu_baseline"] = {"value": round(tc * 1e3, 1), "unit": "ms per particle", "cores": 1, "kind": "port",
+                               "sample": f"the oracle on {cpu_particles} particles of the same workload"}
+        out["gpu_over_cpu_per_particle"] = round(tc / (t / members), 1)
+    ens.cleanUp()
+    return out
+
+
 def measured_traffic(cells):
     """HBM bytes per substep launch from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy), when
@@ -668,6 +726,8 @@ def main():
     ap.add_argument("--no-model-error", action="store_true", help="skip the model-error leg")
     ap.add_argument("--letkf-only", action="store_true",
                     help="run only the LETKF leg and print its JSON (for profiling it)")
+    ap.add_argument("--iewpf-only", action="store_true",
+                    help="run only the IEWPF leg and print its JSON (for profiling it)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -683,6 +743,9 @@ def main():
     log(f"[bench] device {torch.cuda.get_device_name(0)}")
     if a.letkf_only:
         print(json.dumps({"letkf": bench_letkf(ctx, cpu=not a.no_cpu)}), flush=True)
+        return
+    if a.iewpf_only:
+        print(json.dumps({"iewpf": bench_iewpf(ctx, cpu=not a.no_cpu)}), flush=True)
         return
     inp = B.config2(nx=a.n, ny=a.n)
     log(f"[bench] workload {inp.name}: {inp.nx}x{inp.ny}, steps {a.steps}, warmup {a.warmup}")
@@ -735,6 +798,9 @@ def main():
         log("[bench] LETKF analysis on the double-jet ensemble")
         extra["letkf"] = bench_letkf(ctx, cpu=not a.no_cpu)
         log(f"[bench] letkf: {extra['letkf']}")
+        log("[bench] two-stage IEWPF on a 100-particle ensemble")
+        extra["iewpf"] = bench_iewpf(ctx, cpu=not a.no_cpu)
+        log(f"[bench] iewpf: {extra['iewpf']}")
 
     if not a.no_model_error:
         log("[bench] model error (OceanStateNoise) on the headline domain and the ensemble")
