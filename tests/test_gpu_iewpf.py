@@ -20,14 +20,15 @@ Q0 = 0.5 * DT * F / (G * H)
 def _ensemble(N, n, factor, seed):
     from SWESimulators import CDKLM16Ensemble, Common
     rs = np.random.RandomState(seed)
-    j, i = np.mgrid[0:n + 4, 0:n + 4]
+    nx, ny = (n, n) if np.isscalar(n) else n
+    j, i = np.mgrid[0:ny + 4, 0:nx + 4]
     members = []
     for _ in range(N):
-        eta = (0.01 * np.sin(2 * np.pi * i / n) * np.cos(2 * np.pi * j / n) + 1e-3 * rs.randn(n + 4, n + 4))
-        members.append(dict(nx=n, ny=n, dx=DX, dy=DX, g=G, f=F, r=0.0, bc=(2, 2, 2, 2),
-                            eta=eta.astype(np.float32), hu=(0.01 * rs.randn(n + 4, n + 4)).astype(np.float32),
-                            hv=(0.01 * rs.randn(n + 4, n + 4)).astype(np.float32),
-                            Hi=np.full((n + 5, n + 5), H, dtype=np.float32)))
+        eta = (0.01 * np.sin(2 * np.pi * i / nx) * np.cos(2 * np.pi * j / ny) + 1e-3 * rs.randn(ny + 4, nx + 4))
+        members.append(dict(nx=nx, ny=ny, dx=DX, dy=DX, g=G, f=F, r=0.0, bc=(2, 2, 2, 2),
+                            eta=eta.astype(np.float32), hu=(0.01 * rs.randn(ny + 4, nx + 4)).astype(np.float32),
+                            hv=(0.01 * rs.randn(ny + 4, nx + 4)).astype(np.float32),
+                            Hi=np.full((ny + 5, nx + 5), H, dtype=np.float32)))
     ctx = Common.HIPContext(0)
     ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, members, rk_order=2, courant_number=0.7)
     ens.enable_model_error(soar_q0=Q0, interpolation_factor=factor, seed=seed)
@@ -65,6 +66,7 @@ CASES = [
     ("factor1", 6, 40, 1, [[120.0, 120.0], [40.0, 120.0], [80.0, 40.0]]),
     ("factor1_wrap_overlap", 5, 40, 1, [[2.0, 150.0], [10.0, 158.0], [90.0, 3.0]]),
     ("factor3", 5, 45, 3, [[30.0, 160.0], [100.0, 50.0], [170.0, 170.0], [5.0, 5.0]]),
+    ("factor5_bench_domain", 4, (500, 300), 5, "random64"),   # the bench's domain and drifters, 4 particles
 ]
 
 
@@ -73,6 +75,10 @@ def test_iewpf_2stage_matches_oracle(name, N, n, factor, pos):
     import torch
     from SWESimulators import IEWPFOcean
     print("case", name, flush=True)
+    nx, ny = (n, n) if np.isscalar(n) else n
+    if isinstance(pos, str):
+        prs = np.random.RandomState(64)
+        pos = np.column_stack([prs.uniform(0, nx * DX, 64), prs.uniform(0, ny * DX, 64)])
     pos = np.array(pos)
     Nd = len(pos)
     ens = _ensemble(N, n, factor, seed=N + factor)
@@ -84,9 +90,9 @@ def test_iewpf_2stage_matches_oracle(name, N, n, factor, pos):
     gamma, nu_norm = iw.samplePerpendicular()
     torch.cuda.synchronize()
     xi = [ens.noise_views(k)[0].to("cpu").numpy().copy() for k in range(N)]
-    cnx = n // factor
+    cnx, cny = nx // factor, ny // factor
     offs = ens.noise_offsets[:, 1]
-    nu = [iw.nu[int(o):int(o) + cnx * cnx].view(cnx, cnx).to("cpu").numpy().copy() for o in offs]
+    nu = [iw.nu[int(o):int(o) + cnx * cny].view(cny, cnx).to("cpu").numpy().copy() for o in offs]
     for k in range(N):                                  # the sample is perpendicular, norms kept
         assert abs(np.sum(xi[k].astype(np.float64) * nu[k])) < 1e-4 * np.sum(xi[k].astype(np.float64) ** 2)
     rs = np.random.RandomState(3)
@@ -95,14 +101,14 @@ def test_iewpf_2stage_matches_oracle(name, N, n, factor, pos):
     diag = iw.iewpf_2stage(obs, innovations=inn, sample=False)
     torch.cuda.synchronize()
     after = np.stack([np.stack(ens.download(k)) for k in range(N)]).astype(np.float64)
-    p = I.Params(nx=n, ny=n, dx=DX, dy=DX, factor=factor, q0=Q0, L=0.75 * np.float32(DX * factor),
+    p = I.Params(nx=nx, ny=ny, dx=DX, dy=DX, factor=factor, q0=Q0, L=0.75 * np.float32(DX * factor),
                  f=F, g=G, H=H, R=R)
-    Hi = np.full((n + 5, n + 5), H)
+    Hi = np.full((ny + 5, nx + 5), H)
     want, wd = I.iewpf_2stage(p, iw.S_host, iw.localSVD_host, before, Hi, pos, inn, xi, nu)
     for key in ("phi", "gamma", "nu_norm", "alpha"):
         np.testing.assert_allclose(diag[key], wd[key], rtol=1e-5, err_msg=key)
     assert abs(diag["beta"] - wd["beta"]) < 1e-6 * max(1.0, abs(wd["beta"]))
-    inner = (slice(None), slice(None), slice(2, n + 2), slice(2, n + 2))
+    inner = (slice(None), slice(None), slice(2, ny + 2), slice(2, nx + 2))
     incr = want[inner] - before[inner]
     err = np.abs(after[inner] - want[inner])
     for f, fname in enumerate(("eta", "hu", "hv")):

@@ -122,3 +122,34 @@ def test_letkf_repeated_calls_reuse_the_geometry():
     for f in range(3):
         incr = np.max(np.abs(want[:, f] - X_f[:, f]))
         assert np.max(np.abs(got[:, f] - want[:, f])) <= 1e-5 * incr
+
+
+def test_letkf_double_jet_full_size_matches_oracle():
+    """The bench's workload itself: 100 members of 500 x 300, r = 15,
+    240 cell-centred buoys every 25 cells (4 groups)."""
+    import torch
+    from SWESimulators import CDKLM16Ensemble, Common, ETKFOcean
+    import bench
+    print("case double jet full size", flush=True)
+    nx, ny, dx = 500, 300, 2000.0
+    ctx = Common.HIPContext(0)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, bench._double_jet_members(100), courant_number=0.7)
+    ys, xs = np.mgrid[0:ny:25, 0:nx:25]
+    pos = np.column_stack([(xs.ravel() + 0.5) * dx, (ys.ravel() + 0.5) * dx])
+    rs = np.random.RandomState(8)
+    obs = np.column_stack([pos, 5.0 * rs.randn(len(pos), 2)])
+    torch.cuda.synchronize()
+    X_f = np.stack([np.stack(ens.download(k, interior_domain_only=True)) for k in range(100)]).astype(np.float32)
+    R2 = np.diag([1.0, 1.0])
+    want = E.letkf(X_f, obs, pos, R2, 15.0, dx, dx)
+    et = ETKFOcean.ETKFOcean(ens, R2)
+    et.LETKF(obs, r_factor=15.0)
+    torch.cuda.synchronize()
+    assert len(et.groups) == 4
+    got = np.stack([np.stack(ens.download(k, interior_domain_only=True)) for k in range(100)])
+    for f in range(3):
+        incr = np.max(np.abs(want[:, f] - X_f[:, f]))
+        err = np.max(np.abs(got[:, f] - want[:, f]))
+        print("field", f, "max error / max increment", err / incr, flush=True)
+        assert err <= 1e-5 * incr
+    ens.cleanUp()
