@@ -314,29 +314,22 @@ class IEWPFOcean(object):
             HX = ens.observe(d_pos)[:, :pos.shape[0], :].to("cpu").numpy().astype(np.float64)
         return HX
 
-    def iewpf_2stage(self, observations, innovations=None, sample=True):
-        """Two-stage IEWPF analysis of every particle in place
-        (IEWPFOcean.iewpf_2stage, IEWPFOcean.py:195-329; the final model
-        step before it is the caller's).
+    def _innovations(self, obs, innovations):
+        pos = obs[:, :2]
+        N, Nd = self.ensemble.n, pos.shape[0]
+        if innovations is None:
+            innovations = obs[None, :, 2:4] - self.observeParticles(pos)
+        return pos, np.asarray(innovations, dtype=np.float64).reshape(N, Nd, 2)
 
-        observations: (N_d, 4) [x, y, hu, hv] of the true drifters
-        (observeTrueState).  innovations: (N, N_d, 2) y - H x of each
-        particle; default: computed here from the particles' observations at
-        the true drifter positions.  sample: draw new perpendicular xi, nu
-        (False: use the ones samplePerpendicular left)."""
+    def _add_kalman_gain(self, pos, d):
+        """Every particle's state plus the Kalman-gain term of every drifter;
+        returns phi = sum_d d^T S d per particle (addKalmanGain,
+        IEWPFOcean.py:548-595)."""
         torch = Common._torch()
         ens = self.ensemble
         st = ens.gpu_stream
         s = G.stream_handle(st)
-        obs = np.asarray(observations, dtype=np.float64).reshape(-1, 4)
-        pos = obs[:, :2]
-        N, Nd = ens.n, pos.shape[0]
-        if innovations is None:
-            innovations = obs[None, :, 2:4] - self.observeParticles(pos)
-        d = np.asarray(innovations, dtype=np.float64).reshape(N, Nd, 2)
-        w_rest = -np.log(1.0 / N) * np.ones(N)
-
-        # stage 1: Kalman gain, e = S d per particle and drifter, phi = d . e
+        N, Nd = d.shape[0], d.shape[1]
         S = self.S_host.astype(np.float64)
         e = np.einsum("ij,pdj->pdi", S, d)
         phi = np.einsum("pdi,pdi->p", d, e)
@@ -370,6 +363,30 @@ class IEWPFOcean(object):
             with torch.cuda.stream(st):
                 E = torch.from_numpy(coef.astype(np.float32)).to(dev)
                 X.addmm_(E, B)
+        return phi
+
+    def iewpf_2stage(self, observations, innovations=None, sample=True):
+        """Two-stage IEWPF analysis of every particle in place
+        (IEWPFOcean.iewpf_2stage, IEWPFOcean.py:195-329; the final model
+        step before it is the caller's).
+
+        observations: (N_d, 4) [x, y, hu, hv] of the true drifters
+        (observeTrueState).  innovations: (N, N_d, 2) y - H x of each
+        particle; default: computed here from the particles' observations at
+        the true drifter positions.  sample: draw new perpendicular xi, nu
+        (False: use the ones samplePerpendicular left)."""
+        torch = Common._torch()
+        ens = self.ensemble
+        st = ens.gpu_stream
+        s = G.stream_handle(st)
+        obs = np.asarray(observations, dtype=np.float64).reshape(-1, 4)
+        pos, d = self._innovations(obs, innovations)
+        N, Nd = d.shape[0], d.shape[1]
+        dev = ens.state[0].device
+        w_rest = -np.log(1.0 / N) * np.ones(N)
+
+        # stage 1: Kalman gain
+        phi = self._add_kalman_gain(pos, d)
         if sample:
             gamma, nu_norm = self.samplePerpendicular()
         else:
@@ -406,4 +423,60 @@ class IEWPFOcean(object):
         # P^1/2 (alpha^1/2 xi + beta^1/2 nu) added to every particle, ghosts refreshed
         ens.perturb(update_random_field=False)
         self.last = dict(phi=phi, gamma=gamma, nu_norm=nu_norm, c=c, w_target=w_target, beta=beta, alpha=alpha)
+        return self.last
+
+    # ------------------------------------------------------------------
+    # one-stage IEWPF (IEWPFOcean.iewpf, IEWPFOcean.py:331-394)
+    # ------------------------------------------------------------------
+    def sampleXi(self):
+        """xi ~ N(0, I) on every particle's coarse grid (generateNormalDistribution)."""
+        ens = self.ensemble
+        G.check(self.lib.gocean_noise_batch_generate(ens.noise, G.ptr(ens.noise_seed), ens.noise_random.data_ptr(),
+                                                     ens.noise_key, ens.noise_counter.data_ptr(),
+                                                     G.stream_handle(ens.gpu_stream)), "gocean_noise_batch_generate")
+
+    def iewpf(self, observations, innovations=None, sample=True):
+        """One-stage IEWPF analysis of every particle in place
+        (IEWPFOcean.iewpf, IEWPFOcean.py:331-394): the Kalman gain, the target
+        weight max(c) (obtainTargetWeight, :742-762), xi ~ N(0, I) with gamma
+        = |xi|^2 scaled (sampleFromP, :597-627), alpha from the Lambert W, and
+        the perturbation P^1/2 xi scaled by sqrt(alpha) (the scale is applied
+        to xi, the same by linearity).  sample=False uses the xi sampleXi
+        left."""
+        torch = Common._torch()
+        ens = self.ensemble
+        st = ens.gpu_stream
+        s = G.stream_handle(st)
+        obs = np.asarray(observations, dtype=np.float64).reshape(-1, 4)
+        pos, d = self._innovations(obs, innovations)
+        N, Nd = d.shape[0], d.shape[1]
+        dev = ens.state[0].device
+        w_rest = -np.log(1.0 / N) * np.ones(N)
+        phi = self._add_kalman_gain(pos, d)
+        c = phi + w_rest
+        w_target = np.max(c)
+        if sample:
+            self.sampleXi()
+        G.check(self.lib.gocean_iewpf_dots(ens.noise_random.data_ptr(), ens.noise_random.data_ptr(),
+                                           self.offsets.data_ptr(), N, self.n_coarse, self.dots.data_ptr(), s),
+                "gocean_iewpf_dots")
+        gamma = self._norms()[0]
+        c_star = w_target - c
+        arg = -(gamma / self.Nx) * np.exp(-gamma / self.Nx) * np.exp(-c_star / self.Nx)
+        alpha = -(self.Nx / gamma) * np.real(lambertw(arg, k=0))
+        if not np.all(np.isfinite(alpha)):
+            raise RuntimeError("IEWPF: non-finite alpha")
+        cells = self._coarse_cells(pos)
+        with torch.cuda.stream(st):
+            a = torch.from_numpy(np.sqrt(alpha).astype(np.float32)).to(dev)
+            d_cells = torch.from_numpy(cells).to(dev)
+            G.check(self.lib.gocean_iewpf_local_svd(ens.noise_random.data_ptr(), self.offsets.data_ptr(), N,
+                                                    self.coarse_nx, self.coarse_ny,
+                                                    self.localSVD_device.data_ptr(), d_cells.data_ptr(), Nd, s),
+                    "gocean_iewpf_local_svd")
+            G.check(self.lib.gocean_iewpf_axpby(ens.noise_random.data_ptr(), ens.noise_random.data_ptr(),
+                                                self.offsets.data_ptr(), N, self.n_coarse, a.data_ptr(), 0.0, s),
+                    "gocean_iewpf_axpby")
+        ens.perturb(update_random_field=False)
+        self.last = dict(phi=phi, gamma=gamma, c=c, w_target=w_target, alpha=alpha)
         return self.last

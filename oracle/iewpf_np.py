@@ -1,6 +1,7 @@
-"""NumPy restatement of gpu-ocean's two-stage implicit equal-weights particle
-filter (IEWPFOcean.iewpf_2stage, gpu_ocean/SWESimulators/IEWPFOcean.py:
-195-329, with the kernels of gpu_kernels/iewpf_kernels.cu), float64.
+"""NumPy restatement of gpu-ocean's implicit equal-weights particle filter,
+two-stage and one-stage (IEWPFOcean.iewpf_2stage and iewpf,
+gpu_ocean/SWESimulators/IEWPFOcean.py:195-394, with the kernels of
+gpu_kernels/iewpf_kernels.cu), float64.
 
 TEST INFRASTRUCTURE ONLY: only ``tests/`` import this module; the product's
 filter (SWESimulators/IEWPFOcean.py) never calls it.
@@ -280,4 +281,31 @@ def iewpf_2stage(p, S, svd, states, Hi, positions, innovations, xi, nu, beta=Non
             apply_local_svd(z, svd, int(np.floor(positions[d, 0] / p.cdx)), int(np.floor(positions[d, 1] / p.cdy)))
         res.append(perturb(p, z, *out[k], Hi))
     diag = dict(phi=phi, gamma=gamma, nu_norm=nu_norm, c=c, w_target=w_target, beta=b, alpha=alphas)
+    return np.array(res), diag
+
+
+def iewpf_1stage(p, S, svd, states, Hi, positions, innovations, xi):
+    """One-stage IEWPF update of every particle (IEWPFOcean.iewpf, :331-394)
+    given each particle's N(0, I) coarse field xi: Kalman gain, target weight
+    max(c) (obtainTargetWeight, :742-762), gamma = |xi|^2 scaled, alpha from
+    the Lambert W, the local SVD blocks at the drifters (sampleFromP,
+    :597-627) and the perturbation scaled by sqrt(alpha)."""
+    N = len(states)
+    w_rest = -np.log(1.0 / N) * np.ones(N)
+    out, phi = [], np.zeros(N)
+    for k in range(N):
+        s, phi[k] = kalman_gain(p, S, states[k], Hi, positions, innovations[k])
+        out.append(s)
+    c = phi + w_rest
+    w_target = np.max(c)
+    gamma = np.array([np.sum(np.asarray(x, np.float64) ** 2) for x in xi]) * p.ratio
+    alphas = np.zeros(N)
+    res = []
+    for k in range(N):
+        alphas[k] = solve_alpha(gamma[k], p.Nx, w_target - c[k])
+        z = np.array(xi[k], dtype=np.float64)
+        for d in range(positions.shape[0]):
+            apply_local_svd(z, svd, int(np.floor(positions[d, 0] / p.cdx)), int(np.floor(positions[d, 1] / p.cdy)))
+        res.append(perturb(p, np.sqrt(alphas[k]) * z, *out[k], Hi))
+    diag = dict(phi=phi, gamma=gamma, c=c, w_target=w_target, alpha=alphas)
     return np.array(res), diag

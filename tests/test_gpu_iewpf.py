@@ -143,3 +143,39 @@ def test_iewpf_default_innovations_and_resampling_run():
     torch.cuda.synchronize()
     HX_a = iw.observeParticles(pos)
     assert np.linalg.norm(HX_a.mean(axis=0) - y) < np.linalg.norm(HX.mean(axis=0) - y)
+
+
+@pytest.mark.parametrize("factor,n", [(1, 40), (3, 45)])
+def test_iewpf_one_stage_matches_oracle(factor, n):
+    """One-stage IEWPF (IEWPFOcean.iewpf, IEWPFOcean.py:331-394) against the
+    oracle, given the device's own xi."""
+    import torch
+    from SWESimulators import IEWPFOcean
+    print("case one stage", factor, n, flush=True)
+    N = 5
+    pos = np.array([[30.0, 150.0], [100.0, 50.0], [150.0, 120.0]])
+    ens = _ensemble(N, n, factor, seed=21 + factor)
+    ens.enable_drifters(len(pos))
+    R = np.eye(2) * 0.01 ** 2
+    iw = IEWPFOcean.IEWPFOcean(ens, R)
+    torch.cuda.synchronize()
+    before = np.stack([np.stack(ens.download(k)) for k in range(N)]).astype(np.float64)
+    iw.sampleXi()
+    torch.cuda.synchronize()
+    xi = [ens.noise_views(k)[0].to("cpu").numpy().copy() for k in range(N)]
+    inn = 0.02 * np.random.RandomState(6).randn(N, len(pos), 2)
+    diag = iw.iewpf(np.column_stack([pos, np.zeros((len(pos), 2))]), innovations=inn, sample=False)
+    torch.cuda.synchronize()
+    after = np.stack([np.stack(ens.download(k)) for k in range(N)]).astype(np.float64)
+    p = I.Params(nx=n, ny=n, dx=DX, dy=DX, factor=factor, q0=Q0, L=0.75 * np.float32(DX * factor),
+                 f=F, g=G, H=H, R=R)
+    want, wd = I.iewpf_1stage(p, iw.S_host, iw.localSVD_host, before, np.full((n + 5, n + 5), H), pos, inn, xi)
+    for key in ("phi", "gamma", "alpha"):
+        np.testing.assert_allclose(diag[key], wd[key], rtol=1e-5, err_msg=key)
+    inner = (slice(None), slice(None), slice(2, n + 2), slice(2, n + 2))
+    incr = want[inner] - before[inner]
+    err = np.abs(after[inner] - want[inner])
+    for f, fname in enumerate(("eta", "hu", "hv")):
+        ratio = err[:, f].max() / np.abs(incr[:, f]).max()
+        print(fname, "max error / max increment", ratio, flush=True)
+        assert ratio < 2e-5

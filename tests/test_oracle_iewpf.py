@@ -119,3 +119,22 @@ def test_two_stage_target_and_beta():
     assert abs(w - c.mean()) < 1e-12
     c_star = w - c - (b - 1) * nu
     assert np.all(c_star >= -1e-9) and abs(c_star.min()) < 1e-9
+
+
+def test_one_stage_particle_with_largest_c_keeps_alpha_one():
+    """In the one-stage filter c* = max(c) - c: the particle with the largest
+    c has c* = 0, whose principal-branch root is alpha = 1 while gamma <= N_x."""
+    p = ref_params()
+    S = I.create_S(p)
+    svd = I.local_svd(p, S)
+    rs = np.random.RandomState(5)
+    N, n = 4, 40
+    states = 0.01 * rs.randn(N, 3, n + 4, n + 4)
+    pos = np.array([[60.0, 80.0], [120.0, 20.0]])
+    inn = 0.02 * rs.randn(N, 2, 2)
+    xi = [0.9 * rs.randn(n, n) for _ in range(N)]          # gamma below N_x
+    _, d = I.iewpf_1stage(p, S, svd, states, np.full((n + 5, n + 5), 10.0), pos, inn, xi)
+    k = int(np.argmax(d["c"]))
+    assert d["gamma"][k] <= p.Nx
+    assert abs(d["alpha"][k] - 1.0) < 1e-9
+    assert np.all(d["alpha"] > 0)
