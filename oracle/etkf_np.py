@@ -240,3 +240,27 @@ def letkf(X_f, observations, positions, R2, r, dx, dy, scale_w=1.0, inflation=1.
                 X_a[:, i, mask] += wx.reshape(n_loc, N).T
         X = (W_forecast[None, None] * X + X_a).astype(np.float32)   # :733-740
     return X
+
+
+# ---------------------------------------------------------------------------
+# Stochastic EnKF (EnKFOcean.EnKF, gpu_ocean/SWESimulators/EnKFOcean.py:64-296)
+# ---------------------------------------------------------------------------
+
+def enkf(X_f, HX_f, innovations, Y_pert, R2, inflation=1.0):
+    """X_a (n_state, N) of the reference's EnKF with perturbed observations:
+    HX_f (2 N_d, N) rows [hu_1..hu_D, hv_1..hv_D] (ETKFOcean layout),
+    innovations (2 N_d, N) y - H x of each member, Y_pert (2 N_d, N) the
+    observation perturbations (multivariate normal N(0, R) in the
+    reference, :186-221); HPHT = infl^2/(N-1) HX' HX'^T, F = HPHT + R,
+    E = HX'^T F^-1 (D), X_a = X_f + infl^2/(N-1) X' E (:172-293)."""
+    X_f = np.asarray(X_f, dtype=np.float64)
+    HX_f = np.asarray(HX_f, dtype=np.float64)
+    N = X_f.shape[1]
+    Nd = HX_f.shape[0] // 2
+    R = np.linalg.inv(construct_rinv(np.asarray(R2, dtype=np.float64), Nd))
+    HXp = HX_f - np.nanmean(HX_f, axis=1)[:, None]
+    F = inflation ** 2 / (N - 1) * HXp @ HXp.T + R
+    D = np.asarray(innovations, dtype=np.float64) + np.asarray(Y_pert, dtype=np.float64)
+    E = HXp.T @ (np.linalg.inv(F) @ D)
+    Xp = X_f - X_f.mean(axis=1)[:, None]
+    return X_f + inflation ** 2 / (N - 1) * Xp @ E

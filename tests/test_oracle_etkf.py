@@ -60,3 +60,37 @@ def test_rinv_block_layout():
     R = np.linalg.inv(E.construct_rinv(R2, 3))
     assert R[1, 1] == 2.0 and R[4, 4] == 1.0 and R[1, 4] == 0.5 and R[4, 1] == 0.5
     assert R[0, 1] == 0.0 and R[0, 4] == 0.0
+
+
+@pytest.mark.parametrize("seed", [0, 4])
+def test_enkf_mean_is_the_kalman_update_without_observation_noise(seed):
+    """With Y_pert = 0 every member takes K (y - H x_e), K = P_f H^T (H P_f
+    H^T + R)^-1 on the ensemble covariance: the mean moves by the Kalman
+    update, and the anomalies become (I - K H) X'."""
+    X, H, R2, y = _case(seed)
+    N, Nd = X.shape[1], H.shape[0] // 2
+    HX = H @ X
+    inn = y[:, None] - HX
+    Xa = E.enkf(X, HX, inn, np.zeros_like(inn), R2)
+    m = X.mean(axis=1)
+    Xp = X - m[:, None]
+    Pf = Xp @ Xp.T / (N - 1)
+    R = _full_R(R2, Nd)
+    K = Pf @ H.T @ np.linalg.inv(H @ Pf @ H.T + R)
+    np.testing.assert_allclose(Xa.mean(axis=1), m + K @ (y - H @ m), rtol=1e-10, atol=1e-10)
+    np.testing.assert_allclose(Xa - Xa.mean(axis=1)[:, None], (np.eye(len(m)) - K @ H) @ Xp,
+                               rtol=1e-8, atol=1e-10)
+
+
+def test_enkf_inflation_scales_the_prior_covariance():
+    """inflation f: the gain is that of the covariance f^2 P_f."""
+    X, H, R2, y = _case(2)
+    N, Nd = X.shape[1], H.shape[0] // 2
+    HX = H @ X
+    inn = y[:, None] - HX
+    f = 1.4
+    Xa = E.enkf(X, HX, inn, np.zeros_like(inn), R2, inflation=f)
+    Xp = X - X.mean(axis=1)[:, None]
+    Pf = f * f * Xp @ Xp.T / (N - 1)
+    K = Pf @ H.T @ np.linalg.inv(H @ Pf @ H.T + _full_R(R2, Nd))
+    np.testing.assert_allclose(Xa, X + K @ inn, rtol=1e-9, atol=1e-9)
