@@ -191,6 +191,108 @@ class CDKLM16Ensemble(object):
     # ------------------------------------------------------------------
     # model error (OceanStateNoise for every member in batched launches)
     # ------------------------------------------------------------------
+    # ------------------------------------------------------------------
+    # time-targeted steps (CDKLM16.step / dataAssimilationStep for every member)
+    # ------------------------------------------------------------------
+    def _dt_sequences(self, dt64, t_span):
+        """Per member, the local time steps of CDKLM16.step(t_span) with the
+        member's dt (CDKLM16.py:508-593): local_dt = float32(min(dt,
+        float32(t_span - t_now))), t_now accumulated in float64."""
+        seqs = []
+        for d in dt64:
+            t_now, seq = 0.0, []
+            while t_now < t_span:
+                ld = np.float32(min(d, np.float32(t_span - t_now)))
+                seq.append(ld)
+                t_now += np.float64(ld)
+            seqs.append(seq)
+        return seqs
+
+    def update_dt_host(self, courant_number=None):
+        """Per-member dt as the simulator's updateDt sets it: courant times
+        the float32 CFL minimum, in float64 (CDKLM16.py:712-739); the device
+        dt gets the float32 value."""
+        c = self.courant_number if courant_number is None else courant_number
+        G.check(self.lib.gocean_ensemble_max_dt(self.handle, self.state[0].data_ptr(), self.Hm.data_ptr(),
+                                                0.0, self.dt.data_ptr(), G.stream_handle(self.gpu_stream)),
+                "gocean_ensemble_max_dt")
+        raw = self.dt_host().astype(np.float64)
+        self.dt64 = float(c) * raw
+        self.set_dt(self.dt64.astype(np.float32))
+        return self.dt64
+
+    def step_to(self, t_span, dt64=None):
+        """Advance every member by ``t_span`` seconds as CDKLM16.step(t_span)
+        without model error (CDKLM16.py:494-598): each member's own dt
+        sequence, the last step shortened, members that are done skipped
+        (dt 0 in their column of the launch's dt row).  Drifters, when
+        enabled, move with the same local steps."""
+        torch = Common._torch()
+        dt64 = self.dt64 if dt64 is None else np.asarray(dt64, dtype=np.float64)
+        seqs = self._dt_sequences(dt64, float(t_span))
+        steps = max(len(q) for q in seqs)
+        table = np.zeros((steps, self.n), dtype=np.float32)
+        for m, q in enumerate(seqs):
+            table[:len(q), m] = q
+        st = G.stream_handle(self.gpu_stream)
+        with torch.cuda.stream(self.gpu_stream):
+            d_table = torch.from_numpy(table).to(self.state[0].device)
+        for i in range(steps):
+            row = d_table[i]
+            G.check(self.lib.gocean_ensemble_rk_steps(
+                self.handle, 1, self.state[0].data_ptr(), self.state[1].data_ptr(), self._ptr2(),
+                self.Hi.data_ptr(), None, row.data_ptr(), st), "gocean_ensemble_rk_steps")
+            if self.drifters is not None:
+                G.check(self.lib.gocean_drift_batch_step(self.drifters, self.state[0].data_ptr(),
+                                                         self.Hm.data_ptr(), self.drifter_positions.data_ptr(),
+                                                         row.data_ptr(), st), "gocean_drift_batch_step")
+        for m, q in enumerate(seqs):
+            for ld in q:
+                self.t[m] += np.float64(ld)
+        self.num_iterations += steps
+        return self.t
+
+    def dataAssimilationStep(self, observation_time, model_time_step, model_error_final_step=True,
+                             courant_number=0.8):
+        """Every member to ``observation_time`` in model time steps, the
+        leftover first, the model error after each model step (with
+        q0_scale sqrt(leftover / model_time_step) after the leftover, and
+        optionally not after the last), dt refreshed every 5 model steps:
+        CDKLM16.dataAssimilationStep (CDKLM16.py:650-704) for every member at
+        once, as EnsembleFromFiles.stepToObservation runs it member by member
+        (EnsembleFromFiles.py:302-325).  Members share their time."""
+        t0 = float(self.t[0])
+        assert np.all(self.t == t0), "members must share their time"
+        mts = float(model_time_step)
+        full = int(round(observation_time - t0) / mts)
+        leftover = observation_time - t0 - full * mts
+        if leftover / mts < 0.1 and full > 1:
+            leftover += mts
+            full -= 1
+        if leftover / mts < 0.00001:
+            leftover = 0
+        assert full > 0, "There is less than model_time_step until the observation"
+        if not hasattr(self, "total_time_steps"):
+            self.total_time_steps = 0
+        self.update_dt_host(courant_number)
+        for i in range(full + 1):
+            if i == 0 and leftover == 0:
+                continue
+            if i == 0:
+                self.step_to(leftover)
+                if self.noise is not None:
+                    self.perturb(q0_scale=np.sqrt(leftover / mts))
+            else:
+                self.step_to(mts)
+                if self.noise is not None and ((i < full) or model_error_final_step):
+                    self.perturb()
+            self.total_time_steps += 1
+            if self.total_time_steps % 5 == 0:
+                self.update_dt_host(courant_number)
+        assert round(observation_time) == round(float(self.t[0])), \
+            "the members' time is not the observation time after dataAssimilationStep"
+        return self.t
+
     def enable_model_error(self, soar_q0=None, soar_L=None, interpolation_factor=1, use_lcg=False,
                            seed=None, beta=0.0):
         """Set up the batched model error of every member: the member-wise

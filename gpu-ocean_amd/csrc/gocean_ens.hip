@@ -57,6 +57,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
     __shared__ TileSmem<BW, BH, MODE> S;
     const int t = xcd_band(blockIdx.x, gridDim.x);
     const int m = L.tile_member[t];
+    // dt 0 never occurs as a step length (the step loop only runs while time
+    // is left): it marks a member that has reached its end time and is left
+    // as it is, so members with different dt can share the launches of a
+    // time-targeted step
+    if (L.dt[m] == 0.0f) return;                       // uniform over the workgroup
     const EnsMember& M = L.members[m];
     SubstepArgs A;
     const long long so = M.state_off, pl = M.plane;

@@ -185,7 +185,10 @@ int gocean_ensemble_periodic_bc(gocean_ensemble_t* ens, float* state, void* stre
 int gocean_ensemble_max_dt(gocean_ensemble_t* ens, const float* state, const float* Hm,
                            float courant, float* dt_out, void* stream);
 /* One substep of every member (gocean_cdklm_substep per member), dt per
- * member from the device array. */
+ * member from the device array.  A member whose dt is 0 is left untouched
+ * (its output buffer is not written): members with different dt can then
+ * share the launches of a step to a common end time.  The same holds for
+ * gocean_ensemble_rk_steps and the captured graphs. */
 int gocean_ensemble_substep(gocean_ensemble_t* ens, int rk_step, const float* state_in,
                             const float* state_qn, float* state_out, const float* Hi,
                             const float* fields, const float* dt, void* stream);

@@ -182,3 +182,59 @@ def test_tile_config_switch_and_tuner():
     ens.step_n(0)
     torch.cuda.synchronize()
     ens.cleanUp()
+
+
+def test_data_assimilation_step_matches_each_member():
+    """CDKLM16Ensemble.dataAssimilationStep equals every member's own
+    CDKLM16.dataAssimilationStep (CDKLM16.py:650-704) bit for bit: members of
+    different sizes, depths and boundaries have different CFL dt, so their
+    step counts per model step differ and finished members sit out launches
+    (dt 0).  Two calls: the second crosses the dt refresh every 5 model
+    steps."""
+    from SWESimulators import Common, CDKLM16, CDKLM16Ensemble, WindStress
+    _say("ensemble dataAssimilationStep vs members")
+    ms = members(seed=7)
+    for k, m in enumerate(ms):                  # depths, hence CFL dt, far apart
+        m["Hi"] = (m["Hi"] * (1 + 2 * k)).astype(np.float32)
+    mts = 10.0
+    ctx = Common.HIPContext(0)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, courant_number=0.8)
+    sims = []
+    for m in ms:
+        w = WindStress.WindStress(t=[0.0], X=[np.array([[m["wind_x"]]], np.float32)],
+                                  Y=[np.array([[m["wind_y"]]], np.float32)])
+        sims.append(CDKLM16.CDKLM16(ctx, m["eta"], m["hu"], m["hv"], m["Hi"], m["nx"], m["ny"], m["dx"], m["dy"],
+                                    1.0, m["g"], m["f"], m["r"],
+                                    boundary_conditions=Common.BoundaryConditions(*m["bc"]), wind_stress=w,
+                                    model_time_step=mts))
+    for obs_time in (25.0, 71.0):
+        ens.dataAssimilationStep(obs_time, mts, courant_number=0.8)
+        for sim in sims:
+            sim.dataAssimilationStep(obs_time, courant_number=0.8)
+        steps = [len(q) for q in ens._dt_sequences(ens.dt64, mts)]
+        _say("  obs time", obs_time, "steps per model step", steps)
+        assert len(set(steps)) > 1, "members should differ in step count"
+        for k, sim in enumerate(sims):
+            assert ens.t[k] == sim.t, (k, ens.t[k], sim.t)
+            for u, v in zip(ens.download(k), sim.download()):
+                assert np.array_equal(u[2:-2, 2:-2], v[2:-2, 2:-2]), k
+    ens.cleanUp()
+
+
+def test_data_assimilation_step_with_model_error_runs():
+    """With the batched model error every member reaches the observation time
+    and the perturbation changed the state (scale and order follow the
+    reference's schedule; the random streams differ from a per-member run)."""
+    from SWESimulators import Common, CDKLM16Ensemble
+    ms = members(seed=8, bcs=((2, 2, 2, 2),) * 4)
+    ctx = Common.HIPContext(0)
+    a = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, courant_number=0.8)
+    b = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, courant_number=0.8)
+    a.enable_model_error(soar_q0=1e-3, seed=3)
+    a.dataAssimilationStep(37.0, 10.0)
+    b.dataAssimilationStep(37.0, 10.0)
+    assert np.all(a.t == 37.0) or np.allclose(a.t, 37.0, rtol=0, atol=1e-9)
+    assert np.array_equal(a.t, b.t)
+    assert not np.array_equal(a.download(0)[0], b.download(0)[0])
+    a.cleanUp()
+    b.cleanUp()
