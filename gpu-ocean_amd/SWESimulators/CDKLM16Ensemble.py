@@ -472,6 +472,64 @@ class CDKLM16Ensemble(object):
     def cells(self):
         return int(sum(nx * ny for nx, ny in zip(self.nx, self.ny)))
 
+    # ------------------------------------------------------------------
+    # the getters DA code calls on the reference's ensembles
+    # (BaseOceanStateEnsemble.py:192-296); members are all active here
+    # ------------------------------------------------------------------
+    @property
+    def particlesActive(self):
+        return [True] * self.n
+
+    def getNumParticles(self):
+        return np.int32(self.n)
+
+    def getNumActiveParticles(self):
+        return self.n
+
+    def getNumDrifters(self):
+        return np.int32(self.num_drifters if self.drifters is not None else 0)
+
+    def getNx(self):
+        return np.int32(self.nx[0])
+
+    def getNy(self):
+        return np.int32(self.ny[0])
+
+    def getDx(self):
+        return np.float32(self.dxy[0][0])
+
+    def getDy(self):
+        return np.float32(self.dxy[0][1])
+
+    def getDt(self):
+        return np.float32(self.dt_host()[0])
+
+    def getDomainSizeX(self):
+        return self.getNx() * self.getDx()
+
+    def getDomainSizeY(self):
+        return self.getNy() * self.getDy()
+
+    def downloadParticleOceanState(self, particleNo):
+        return self.download(particleNo)
+
+    def getInnovations(self, observations):
+        """(N, N_d, 2) y - H(x_m): the observed (hu, hv) of the true state
+        minus every member's at the same positions (getInnovations /
+        observeParticles(innovation=True), BaseOceanStateEnsemble.py:80-205).
+        observations: (N_d, 4) [x, y, hu, hv]."""
+        torch = Common._torch()
+        obs = np.asarray(observations, dtype=np.float64).reshape(-1, 4)
+        with torch.cuda.stream(self.gpu_stream):
+            pos = torch.from_numpy(obs[:, :2].astype(np.float32)).to(self.state[0].device)
+            HX = self.observe(pos)[:, :obs.shape[0], :].to("cpu").numpy().astype(np.float64)
+        return obs[None, :, 2:4] - HX
+
+    def getGaussianWeight(self, innovations, observation_cov, normalize=True, R_scale=1.0):
+        """Particle weights from Gaussian observation errors
+        (BaseOceanStateEnsemble.getGaussianWeight, :216-261)."""
+        return gaussian_weights(innovations, observation_cov, normalize, R_scale)
+
     def cleanUp(self):
         if getattr(self, "handle", None) is not None:
             self.lib.gocean_ensemble_destroy(self.handle)
@@ -488,3 +546,25 @@ class CDKLM16Ensemble(object):
             self.cleanUp()
         except Exception:
             pass
+
+
+def gaussian_weights(innovations, observation_cov, normalize=True, R_scale=1.0):
+    """log w_p = -1/2 sum_d d^T (R R_scale)^-1 d per particle; normalised with
+    the max-shift (exp-normalise) when asked, else scaled by the Gaussian
+    density's constant ((2 pi)^N_d det(R)^(N_d/2)) (BaseOceanStateEnsemble.
+    getGaussianWeight, :216-261).  innovations: (N, N_d, 2) or (N,) scalars
+    with observation variance R[0, 0]."""
+    inn = np.asarray(innovations, dtype=np.float64)
+    R = np.asarray(observation_cov, dtype=np.float64).reshape(2, 2) * R_scale
+    if inn.ndim == 1:
+        var = R[0, 0]
+        logw = -(inn ** 2 / (2 * var))
+        if normalize:
+            return np.exp(logw - logw.max()) / np.exp(logw - logw.max()).sum()
+        return (1.0 / np.sqrt(2 * np.pi * var)) * np.exp(logw)
+    Rinv = np.linalg.inv(R)
+    logw = -0.5 * np.einsum("pdi,ij,pdj->p", inn, Rinv, inn)
+    if normalize:
+        return np.exp(logw - logw.max()) / np.exp(logw - logw.max()).sum()
+    nd = inn.shape[1]
+    return (1.0 / ((2 * np.pi) ** nd * np.linalg.det(R) ** (nd / 2.0))) * np.exp(logw)

@@ -238,3 +238,29 @@ def test_data_assimilation_step_with_model_error_runs():
     assert not np.array_equal(a.download(0)[0], b.download(0)[0])
     a.cleanUp()
     b.cleanUp()
+
+
+def test_reference_getters_and_innovations():
+    """The getters DA code calls on the reference's ensembles, and
+    getInnovations = y - H(x_m) from the batched observation kernel."""
+    from SWESimulators import Common, CDKLM16Ensemble
+    from oracle import drifters_np as DR
+    ms = members(seed=9, sizes=((40, 30),) * 3, bcs=((2, 2, 2, 2),) * 3)
+    for m in ms:
+        m["dx"] = 300.0
+    ctx = Common.HIPContext(0)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms)
+    ens.enable_drifters(4)
+    assert ens.getNumParticles() == 3 and ens.getNumActiveParticles() == 3 and ens.getNumDrifters() == 4
+    assert (ens.getNx(), ens.getNy(), ens.getDx(), ens.getDy()) == (40, 30, np.float32(300.0), np.float32(250.0))
+    assert ens.getDomainSizeX() == 40 * 300.0 and ens.particlesActive == [True] * 3
+    pos = np.array([[1000.0, 900.0], [5000.0, 3000.0], [11000.0, 7000.0], [300.0, 7400.0]])
+    obs = np.column_stack([pos, np.arange(8.0).reshape(4, 2)])
+    inn = ens.getInnovations(obs)
+    for k in range(3):
+        eta, hu, hv = ens.download(k)
+        want = obs[:, 2:4] - DR.observe_flow(pos.astype(np.float32), hu, hv, 300.0, 250.0, 4)
+        np.testing.assert_allclose(inn[k], want, rtol=0, atol=1e-6)
+    w = ens.getGaussianWeight(inn, np.eye(2))
+    assert abs(w.sum() - 1) < 1e-12
+    ens.cleanUp()
