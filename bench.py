@@ -244,6 +244,8 @@ def bench_adjoint(ctx, n=512, steps=500, K=50, n_obs=64, seed=44):
     oy = rng.uniform(2.0, n - 2.0, n_obs) * inp.dy
     q0 = [a.tensor.clone() for a in (sim.gpu_data.h0, sim.gpu_data.hu0, sim.gpu_data.hv0)]
     adj = A.simulator_adjoint(sim, checkpoint_every=K)
+    # the recording forward on its own fastest tile (every tile records the same)
+    rec_best, rec_times = autotune.tune_adjoint_record(adj, torch.stack(q0), dt, steps=4)
     # twin run: perturbed initial eta
     pert = 0.05 * B.gaussian_random_field((n + 4, n + 4), 16.0, rng).astype(np.float32)
     twin0 = [q0[0] + torch.from_numpy(pert).to(q0[0].device), q0[1], q0[2]]
@@ -286,6 +288,7 @@ def bench_adjoint(ctx, n=512, steps=500, K=50, n_obs=64, seed=44):
     # checkpointed: the backward recomputes each segment
     adj_rc = A.simulator_adjoint(sim, checkpoint_every=K)
     adj_rc.keep_records = False
+    adj_rc.record_tile_config = rec_best
     t_rc, J_rc, q_rc = timed(adj_rc)
     same = all(torch.equal(a.grad, b.grad) for a, b in zip(q, q_rc))
     cells = n * n
@@ -296,6 +299,9 @@ def bench_adjoint(ctx, n=512, steps=500, K=50, n_obs=64, seed=44):
             "fwd_s": round(t_fwd, 4), "fwd_adj_s": round(t_tot, 4),
             "fwd_adj_over_fwd": round(t_tot / t_fwd, 2), "records_kept": kept,
             "records_gb": round(rec_gb, 2),
+            "record_tile": autotune.tile_config_names()[rec_best],
+            "record_ms_per_step_by_tile": {autotune.tile_config_names()[c]: round(v, 4)
+                                           for c, v in rec_times.items()},
             "fwd_adj_recompute_s": round(t_rc, 4), "fwd_adj_recompute_over_fwd": round(t_rc / t_fwd, 2),
             "kept_vs_recompute_grad_identical": bool(same),
             "fwd_mcell_steps_s": round(cells * steps / t_fwd / 1e6, 1),

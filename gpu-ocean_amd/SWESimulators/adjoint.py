@@ -64,7 +64,7 @@ class RK2Adjoint(object):
     """
 
     def __init__(self, domain, Hi, stream=None, wind=None, checkpoint_every=50, tile_config=-1,
-                 keep_records="auto", record_budget=0.25):
+                 keep_records="auto", record_budget=0.25, record_tile_config=-1):
         torch = _torch()
         self.lib = G.load()
         self.d = domain
@@ -77,6 +77,7 @@ class RK2Adjoint(object):
         self.fields = G.Fields()
         self.K = max(1, int(checkpoint_every))
         self.tile_config = int(tile_config)
+        self.record_tile_config = int(record_tile_config)
         if keep_records not in (True, False, "auto"):
             raise ValueError("keep_records must be True, False or 'auto'")
         self.keep_records = keep_records
@@ -171,6 +172,8 @@ class RK2Adjoint(object):
             last = w[o:o + 3 * pl].view(3, *self.shape)
             if n == len(segs) - 1:
                 last.copy_(q)
+            G.check(self.lib.gocean_adjoint_set_record_tile_config(self.record_tile_config),
+                    "gocean_adjoint_set_record_tile_config")
             G.check(self.lib.gocean_adjoint_record(
                 self.d, self.wind, k, self.dt, None, *(G.ptr(a) for a in start), G.ptr(self.Hi),
                 G.ptr(w), nw, G.stream_handle(self.stream)), "gocean_adjoint_record")
@@ -237,6 +240,9 @@ class RK2Adjoint(object):
             if self.work is None and n_seg:
                 nw = int(self.lib.gocean_adjoint_segment_work_floats(self.d, self.K))
                 self.work = torch.empty(nw, dtype=torch.float32, device=self.dev)
+            if n_seg:
+                G.check(self.lib.gocean_adjoint_set_record_tile_config(self.record_tile_config),
+                        "gocean_adjoint_set_record_tile_config")
             for s in range(n_seg - 1, -1, -1):
                 k = min(self.K, self.n_steps - s * self.K)
                 G.check(self.lib.gocean_adjoint_segment(

@@ -99,3 +99,42 @@ def tune_ensemble(ens, steps=4, warmup=1, configs=None, use_cache=True):
     ens.set_tile_config(best)
     _cache[key] = (best, times)
     return best, times
+
+
+def tune_adjoint_record(adj, q, dt, steps=4, configs=None):
+    """Pick the fastest tile configuration of the recording forward for the
+    RK2Adjoint ``adj``: time ``steps`` recorded steps from the (3, ny+4,
+    pitch) state ``q`` per configuration (the record is the same for every
+    configuration).  Sets ``adj.record_tile_config``; returns (best,
+    {config: ms_per_step})."""
+    import torch
+    from . import _gocean as G
+    lib = adj.lib
+    if configs is None:
+        configs = list(range(len(tile_config_names())))
+    nw = int(lib.gocean_adjoint_segment_work_floats(adj.d, steps))
+    st = adj.stream
+    with torch.cuda.stream(st):
+        w = torch.empty(nw, dtype=torch.float32, device=adj.dev)
+        start = [a.clone() for a in q.unbind(0)]
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    times = {}
+
+    def record():
+        G.check(lib.gocean_adjoint_record(adj.d, adj.wind, steps, float(dt), None, *(G.ptr(a) for a in start),
+                                          G.ptr(adj.Hi), G.ptr(w), nw, G.stream_handle(st)),
+                "gocean_adjoint_record")
+
+    for c in configs:
+        G.check(lib.gocean_adjoint_set_record_tile_config(c), "gocean_adjoint_set_record_tile_config")
+        record()
+        ev0.record(st)
+        record()
+        ev1.record(st)
+        ev1.synchronize()
+        times[c] = ev0.elapsed_time(ev1) / steps
+    best = min(times, key=times.get)
+    adj.record_tile_config = best
+    G.check(lib.gocean_adjoint_set_record_tile_config(best), "gocean_adjoint_set_record_tile_config")
+    return best, times

@@ -122,11 +122,27 @@ __global__ __launch_bounds__(NT) void k_substep_tape(SubstepArgs A) {
     cdklm_tile<BW, BH, NT, kModeZero, true>(A, tx, ty, S);
 }
 
-int launch_substep_tape(SubstepArgs A, hipStream_t s) {
-    constexpr int BW = 32, BH = 16, NT = 512;
+template <int BW, int BH, int NT>
+static void launch_tape_t(SubstepArgs A, hipStream_t s) {
     A.tiles_x = (A.nx + BW - 1) / BW;
     A.tiles_y = (A.ny + BH - 1) / BH;
     hipLaunchKernelGGL((k_substep_tape<BW, BH, NT>), dim3(A.tiles_x * A.tiles_y), dim3(NT), 0, s, A);
+}
+
+// tile configuration of the recording forward (process-wide; the record is
+// laid out per cell, so every configuration writes the same record)
+static int g_record_cfg = 5;
+
+int launch_substep_tape(SubstepArgs A, hipStream_t s) {
+    switch (g_record_cfg) {
+        case 0: launch_tape_t<32, 8, 256>(A, s); break;
+        case 1: launch_tape_t<32, 16, 256>(A, s); break;
+        case 2: launch_tape_t<64, 8, 256>(A, s); break;
+        case 3: launch_tape_t<16, 16, 256>(A, s); break;
+        case 4: launch_tape_t<64, 4, 256>(A, s); break;
+        case 6: launch_tape_t<32, 14, 512>(A, s); break;
+        default: launch_tape_t<32, 16, 512>(A, s); break;
+    }
     return check_launch("recording substep launch");
 }
 
@@ -334,6 +350,17 @@ int gocean_device_count(void) {
     }
     return n;
 }
+
+int gocean_adjoint_set_record_tile_config(int cfg) {
+    clear_error();
+    if (cfg < 0) cfg = 5;
+    if (cfg >= kNumTileConfigs)
+        return set_error(GOCEAN_EINVAL, "tile configuration %d out of range [0,%d)", cfg, kNumTileConfigs);
+    g_record_cfg = cfg;
+    return 0;
+}
+
+int gocean_adjoint_record_tile_config(void) { return g_record_cfg; }
 
 int gocean_tile_configs(const char** names, int max) {
     for (int k = 0; k < kNumTileConfigs && k < max; ++k) names[k] = kTileConfigs[k].name;

@@ -130,6 +130,14 @@ int gocean_max_dt(const gocean_domain_t* d, const float* eta, const float* hu,
                   const float* hv, const float* Hm, float courant, float* out_dev,
                   void* stream);
 
+/* Tile configuration (index into gocean_tile_configs, -1: the default
+ * 32x16/512) of the recording forward substep the adjoint entry points run
+ * (gocean_adjoint_record / _segment).  Process-wide; every configuration
+ * writes the same record.  No counterpart in the reference (the adjoint is
+ * new); it lets a tuner pick the fastest recording tile, as for the forward. */
+int gocean_adjoint_set_record_tile_config(int cfg);
+int gocean_adjoint_record_tile_config(void);
+
 /* Names of the tile configurations the library is built with; returns the
  * count and writes up to max entries of "BWxBH/threads" into names. */
 int gocean_tile_configs(const char** names, int max);

@@ -523,3 +523,31 @@ def test_ensemble_kept_records_match_recompute():
     for a, b in zip(res[False], res[True]):
         assert torch.equal(a, b)
     assert torch.equal(res[True][2], res[True][3])
+
+
+def test_record_tile_configs_record_the_same_gradient():
+    """Every tile configuration of the recording forward writes the same
+    record: forward output and gradient are bit-identical whichever tile
+    records, with the records kept and recomputed."""
+    import torch
+    from SWESimulators import autotune
+    _say("record tile configurations")
+    Common, G, A = _mods()
+    nx, ny, dt, n = 37, 29, 3.0, 7
+    p, Hi, eta, hu, hv = _case(nx, ny, (2, 1, 2, 1), 13, "plain")
+    W = [_dev(w) for w in _weights(np.random.default_rng(5), nx, ny)]
+    d = _domain(p)
+    ref = None
+    for cfg in range(len(autotune.tile_config_names())):
+        for keep in (True, False):
+            adj = A.RK2Adjoint(d, _dev(Hi), checkpoint_every=3, keep_records=keep, record_tile_config=cfg)
+            qd = [_dev(a).requires_grad_(True) for a in (eta, hu, hv)]
+            out = A.CDKLM16RK2.apply(*qd, adj, n, dt)
+            sum((w * o).sum() for w, o in zip(W, out)).backward()
+            torch.cuda.synchronize()
+            got = [o.detach().cpu().numpy() for o in out] + [x.grad.cpu().numpy() for x in qd]
+            if ref is None:
+                ref = got
+            for x, y in zip(ref, got):
+                assert np.array_equal(x, y), (cfg, keep)
+    G.check(G.load().gocean_adjoint_set_record_tile_config(-1), "reset")
