@@ -228,7 +228,11 @@ class CDKLM16Ensemble(object):
         (dt 0 in their column of the launch's dt row).  Drifters, when
         enabled, move with the same local steps."""
         torch = Common._torch()
-        dt64 = self.dt64 if dt64 is None else np.asarray(dt64, dtype=np.float64)
+        if dt64 is None:
+            if getattr(self, "dt64", None) is None:
+                self.update_dt_host()
+            dt64 = self.dt64
+        dt64 = np.asarray(dt64, dtype=np.float64)
         seqs = self._dt_sequences(dt64, float(t_span))
         steps = max(len(q) for q in seqs)
         table = np.zeros((steps, self.n), dtype=np.float32)
