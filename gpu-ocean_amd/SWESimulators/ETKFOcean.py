@@ -282,14 +282,17 @@ def letkf_tiles(group, bx, by, jj, ii, nx, ny, tw, th):
     """Tiles of tw x th cells that the nonzero weights (rows jj, columns ii of
     the stencil) of the group's boxes reach, each with its buoys in group
     order; and the most buoys weighing on one cell."""
-    ntx = (nx + tw - 1) // tw
+    # tiles start two cells left of a multiple of tw: with the two ghost
+    # columns in front, each tile row is then tw-float aligned in memory
+    sh = 2
+    ntx = (nx + sh + tw - 1) // tw
     tids, oids = [], []
     cover = np.zeros(nx * ny, dtype=np.int32)
     for o in group:
         xs = (bx[o] + ii) % nx
         ys = (by[o] + jj) % ny
         cover[ys * nx + xs] += 1
-        t = np.unique((ys // th) * ntx + xs // tw)
+        t = np.unique((ys // th) * ntx + (xs + sh) // tw)
         tids.append(t)
         oids.append(np.full(t.size, o))
     tid = np.concatenate(tids) if tids else np.zeros(0, np.int64)
@@ -297,5 +300,5 @@ def letkf_tiles(group, bx, by, jj, ii, nx, ny, tw, th):
     order = np.argsort(tid, kind="stable")
     tid, oid = tid[order], oid[order]
     uniq, begin, count = np.unique(tid, return_index=True, return_counts=True)
-    tiles = np.column_stack([(uniq % ntx) * tw, (uniq // ntx) * th, begin, count]).astype(np.int32)
+    tiles = np.column_stack([(uniq % ntx) * tw - sh, (uniq // ntx) * th, begin, count]).astype(np.int32)
     return np.ascontiguousarray(tiles), oid.astype(np.int32), int(cover.max()) if cover.size else 0
