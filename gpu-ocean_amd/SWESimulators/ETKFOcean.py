@@ -96,16 +96,23 @@ class ETKFOcean(object):
             X = self.state_matrix()
             X_mean = X.mean(dim=0, keepdim=True)
             X_pert = X - X_mean
-        # ensemble-space algebra (ETKFOcean.py:232-284), float64
+        # ensemble-space algebra (ETKFOcean.py:232-284), float64.  One
+        # eigendecomposition of A = (N-1) I + HX'^T R^-1 HX' gives both
+        # P = A^-1 and the symmetric square root of (N-1) P; R^-1 is built
+        # from the inverse of its 2 x 2 (hu_l, hv_l) blocks.
         HX_mean = HX_f.mean(axis=1)
         HX_pert = HX_f - HX_mean[:, None]
-        Rinv = self._constructRinv(Nd)
+        r = np.linalg.inv(self.R)
+        U, Vv = HX_pert[:Nd], HX_pert[Nd:]
         y = np.concatenate([obs[:, 2], obs[:, 3]])
         D = y - HX_mean
-        P = np.linalg.inv((N - 1) * np.eye(N) + HX_pert.T @ Rinv @ HX_pert)
-        w = P @ (HX_pert.T @ (Rinv @ D))
-        sigma, V = np.linalg.eigh((N - 1) * P)
-        W = V @ np.diag(np.sqrt(np.real(sigma))) @ V.T
+        A = (N - 1) * np.eye(N) + r[0, 0] * (U.T @ U) + r[0, 1] * (U.T @ Vv) + r[1, 0] * (Vv.T @ U) \
+            + r[1, 1] * (Vv.T @ Vv)
+        lam, V = np.linalg.eigh(A)
+        P = (V / lam) @ V.T
+        RD = np.concatenate([r[0, 0] * D[:Nd] + r[0, 1] * D[Nd:], r[1, 0] * D[:Nd] + r[1, 1] * D[Nd:]])
+        w = P @ (HX_pert.T @ RD)
+        W = (V * np.sqrt((N - 1) / lam)) @ V.T
         T = W + w[:, None]
         self.last = {"P": P, "w": w, "W": W, "D": D}
         with torch.cuda.stream(st):
