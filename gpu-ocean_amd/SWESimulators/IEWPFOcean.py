@@ -12,9 +12,10 @@ batched model error) advances together:
     patch Q U_GB^T H^T e, e = S d.  With interpolated model error the
     balance reads H + d_eta, so the term is a fixed quadratic function of e
     (d_eta is linear, the transports carry (H + d_eta) grad d_eta): five basis
-    states per drifter, computed once per drifter set by the model-error
-    kernels, and all particles take their gain in one GEMM, X += E B with E
-    the (N, 5 N_d) matrix of e_x, e_y, e_x^2, e_y^2, e_x e_y.  Without
+    fields per drifter, computed once per drifter set by the model-error
+    kernels and kept on the window of cells where they are non-zero; one
+    gather launch adds every drifter's e_x, e_y, e_x^2, e_y^2, e_x e_y
+    combination of them to every particle (gocean_iewpf_kalman_window).  Without
     interpolation the balance reads H + eta + d_eta, so the term depends on
     the state: the drifters go one after the other as in the reference, each
     as one patch launch and one batched perturbation over all particles.
@@ -40,6 +41,58 @@ from scipy.special import lambertw
 from . import Common
 from . import OceanStateNoise
 from . import _gocean as G
+
+
+SEGMENT = 256    # interior cells per row tile of gocean_iewpf_kalman_window
+
+
+def circular_cover(mask):
+    """(start, length) of the shortest periodic interval of indices holding
+    every True of ``mask``; (0, 0) when there is none."""
+    mask = np.asarray(mask, dtype=bool)
+    n = mask.size
+    idx = np.flatnonzero(mask)
+    if idx.size == 0:
+        return 0, 0
+    gaps = np.diff(np.concatenate([idx, [idx[0] + n]]))
+    k = int(np.argmax(gaps))
+    return int(idx[(k + 1) % idx.size]), int(n - (gaps[k] - 1))
+
+
+def kalman_window_coefficients(coef):
+    """(N, N_d, 5) per-member coefficients -> the (N_d, 5, N rounded up to
+    8) float32 layout gocean_iewpf_kalman_window reads (zero padding)."""
+    coef = np.asarray(coef)
+    n = coef.shape[0]
+    out = np.zeros((coef.shape[1], 5, -(-n // 8) * 8), dtype=np.float32)
+    out[:, :, :n] = coef.transpose(1, 2, 0)
+    return out
+
+
+def kalman_window_tiles(origins, sizes, nx, ny):
+    """The 256-cell interior row segments that drifters' periodic windows meet
+    (window d: sizes[d] = (rows, columns) from origins[d] = (row, column)),
+    each with the drifters meeting it in ascending order.  Returns tiles
+    (T, 2) [row, segment], list_off (T + 1,) and the drifter list."""
+    nseg = -(-nx // SEGMENT)
+    tid, dd = [], []
+    for d, ((r0, c0), (h, w)) in enumerate(zip(origins, sizes)):
+        if h <= 0 or w <= 0:
+            continue
+        rows = (int(r0) + np.arange(h)) % ny
+        segs = np.unique(((int(c0) + np.arange(w)) % nx) // SEGMENT)
+        t = (rows[:, None] * nseg + segs[None, :]).ravel()
+        tid.append(t)
+        dd.append(np.full(t.size, d))
+    if not tid:
+        return np.zeros((0, 2), np.int32), np.zeros(1, np.int32), np.zeros(0, np.int32)
+    tid, dd = np.concatenate(tid), np.concatenate(dd)
+    order = np.lexsort((dd, tid))
+    tid, dd = tid[order], dd[order]
+    ut, first = np.unique(tid, return_index=True)
+    tiles = np.column_stack([ut // nseg, ut % nseg]).astype(np.int32)
+    list_off = np.append(first, tid.size).astype(np.int32)
+    return tiles, list_off, dd.astype(np.int32)
 
 
 class IEWPFOcean(object):
@@ -204,15 +257,21 @@ class IEWPFOcean(object):
     # Kalman-gain basis (device, once per drifter set)
     # ------------------------------------------------------------------
     def _kalman_basis(self, positions):
-        """(5 N_d, 3 plane) device matrix of the Kalman-gain term of every
-        drifter as a function of e = S d (addKalmanGain, IEWPFOcean.py:
-        548-595): the model-error perturbation of the patch e_x P_x + e_y P_y,
+        """The Kalman-gain term of every drifter as a function of e = S d
+        (addKalmanGain, IEWPFOcean.py:548-595): the model-error perturbation of the patch e_x P_x + e_y P_y,
         aligned with the drifter's fine cell, is
             e_x L_x + e_y L_y + e_x^2 Q_xx + e_y^2 Q_yy + e_x e_y Q_xy
         exactly -- d_eta is linear in e and the balanced transports
         -(g/f)(H + d_eta) grad d_eta are quadratic.  The five fields come
         from perturbations at e = (+-s, 0), (0, +-s), (s, s), s of the order
-        of the innovations' S d, so the quadratic parts are well resolved."""
+        of the innovations' S d, so the quadratic parts are well resolved.
+
+        Each drifter's fields are kept on the smallest periodic window of
+        interior cells outside which all five are zero (ghost cells are left
+        out: the analysis ends with the perturbation, which refreshes them).
+        Returns a dict: basis (N_d, 5, 3, wh, ww) device floats, origins
+        (N_d, 2) rows/columns, the window shape and the 256-cell row tiles
+        with their drifter lists (kalman_window_tiles)."""
         torch = Common._torch()
         pos = np.asarray(positions, dtype=np.float64).reshape(-1, 2)
         key = pos.tobytes()
@@ -232,8 +291,8 @@ class IEWPFOcean(object):
         sc = float(np.abs(self.S_host).max() * np.sqrt(np.abs(np.diag(self.R)).max()))
         sc = sc if np.isfinite(sc) and sc > 0 else 1.0
         with torch.cuda.stream(st):
-            B = torch.zeros((5 * len(pos), 3 * plane), dtype=torch.float32, device=dev)
             tmp = torch.zeros((5, 3 * plane), dtype=torch.float32, device=dev)
+        windows, origins, sizes = [], np.zeros((len(pos), 2), dtype=np.int32), np.zeros((len(pos), 2), dtype=np.int64)
         for d, (px, py) in enumerate(pos):
             cx = int(np.floor(px / float(self.coarse_dx)))
             cy = int(np.floor(py / float(self.coarse_dy)))
@@ -257,11 +316,30 @@ class IEWPFOcean(object):
                 Ly = (t[2] - t[3]) / (2 * sc)
                 Qyy = (t[2] + t[3]) / (2 * sc * sc)
                 Qxy = (t[4] - sc * (Lx + Ly)) / (sc * sc) - Qxx - Qyy
-                B[5 * d:5 * d + 5] = torch.stack([Lx, Ly, Qxx, Qyy, Qxy]).float()
+                Bd = torch.stack([Lx, Ly, Qxx, Qyy, Qxy]).float().view(5, 3, ny + 4, nx + 4)[:, :, 2:-2, 2:-2]
+                nz = Bd != 0
+                rows = nz.any(dim=3).any(dim=1).any(dim=0).to("cpu").numpy()
+                cols = nz.any(dim=2).any(dim=1).any(dim=0).to("cpu").numpy()
+                (r0, wh), (c0, ww) = circular_cover(rows), circular_cover(cols)
+                origins[d], sizes[d] = (r0, c0), (wh, ww)
+                ri = torch.from_numpy((r0 + np.arange(wh)) % ny).to(dev)
+                ci = torch.from_numpy((c0 + np.arange(ww)) % nx).to(dev)
+                windows.append(Bd.index_select(2, ri).index_select(3, ci).contiguous())
+        wh, ww = max(1, int(sizes[:, 0].max())), max(1, int(sizes[:, 1].max()))
+        tiles, list_off, lst = kalman_window_tiles(origins, sizes, nx, ny)
+        with torch.cuda.stream(st):
+            W = torch.zeros((len(pos), 5, 3, wh, ww), dtype=torch.float32, device=dev)
+            for d, w in enumerate(windows):
+                W[d, :, :, :w.shape[2], :w.shape[3]] = w
+            basis = dict(basis=W, wh=wh, ww=ww, n_tiles=len(tiles),
+                         origins=torch.from_numpy(np.ascontiguousarray(origins)).to(dev),
+                         tiles=torch.from_numpy(np.ascontiguousarray(tiles, dtype=np.int32)).to(dev),
+                         list_off=torch.from_numpy(np.ascontiguousarray(list_off, dtype=np.int32)).to(dev),
+                         list=torch.from_numpy(np.ascontiguousarray(lst, dtype=np.int32)).to(dev))
         st.synchronize()
         noise.cleanUp()
-        self._basis = (key, B)
-        return B
+        self._basis = (key, basis)
+        return basis
 
     # ------------------------------------------------------------------
     # the filter
@@ -355,14 +433,18 @@ class IEWPFOcean(object):
                     ens.noise_key, ens.noise_counter.data_ptr(), s), "gocean_noise_batch_perturb")
         else:
             # the interpolated balance reads H + d_eta only: the term is a fixed
-            # quadratic function of e per drifter, one GEMM for all particles
-            B = self._kalman_basis(pos)
-            X = ens.state[0][:N * B.shape[1]].view(N, B.shape[1])
+            # quadratic function of e per drifter, one gather for all particles
+            W = self._kalman_basis(pos)
+            plane = (self.ny + 4) * (self.nx + 4)
+            assert all(int(ens.state_off[k]) == 3 * plane * k for k in range(N)), "members must be contiguous"
             ex, ey = e32[:, :, 0].astype(np.float64), e32[:, :, 1].astype(np.float64)
-            coef = np.stack([ex, ey, ex * ex, ey * ey, ex * ey], axis=2).reshape(N, 5 * Nd)
+            coef = kalman_window_coefficients(np.stack([ex, ey, ex * ex, ey * ey, ex * ey], axis=2))
             with torch.cuda.stream(st):
-                E = torch.from_numpy(coef.astype(np.float32)).to(dev)
-                X.addmm_(E, B)
+                E = torch.from_numpy(coef).to(dev)
+                G.check(self.lib.gocean_iewpf_kalman_window(
+                    ens.state[0].data_ptr(), N, self.nx, self.ny, 3 * plane, W["basis"].data_ptr(), Nd, W["wh"],
+                    W["ww"], W["origins"].data_ptr(), E.data_ptr(), coef.shape[2], W["tiles"].data_ptr(),
+                    W["n_tiles"], W["list_off"].data_ptr(), W["list"].data_ptr(), s), "gocean_iewpf_kalman_window")
         return phi
 
     def iewpf_2stage(self, observations, innovations=None, sample=True):

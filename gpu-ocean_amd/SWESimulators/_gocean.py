@@ -223,6 +223,10 @@ PROTOTYPES.extend([
     ("gocean_iewpf_kalman_patch", c_int,
      [c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p, ctypes.c_int32,
       ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, c_void_p]),
+    ("gocean_iewpf_kalman_window", c_int,
+     [c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64, c_void_p, ctypes.c_int32,
+      ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p, ctypes.c_int32, c_void_p, ctypes.c_int32, c_void_p,
+      c_void_p, c_void_p]),
 ])
 
 # sizeof(gocean_letkf_coef_t): two floats, two doubles

@@ -15,6 +15,9 @@
 //                           window around it <- svd (49 x 49) x window
 //   k_iewpf_kalman_patch    one drifter's Kalman-gain patch e_x P_x + e_y P_y
 //                           of every member written into its zeroed field
+//   k_iewpf_kalman_window   the Kalman-gain term of every drifter added to
+//                           every member's fine-grid state, from windowed
+//                           basis fields (interpolated model error)
 
 #include <algorithm>
 
@@ -140,6 +143,72 @@ __global__ __launch_bounds__(kElemNT) void k_iewpf_kalman_patch(float* xi, const
     xi[offsets[m] + i] = v;
 }
 
+// addKalmanGain with interpolated model error (IEWPFOcean.py:548-595): the
+// term of drifter d is sum_k coef[d][k][m] basis[d][k] with five fields per
+// drifter, non-zero on a window of win_h x win_w interior cells whose first
+// cell is (origins[2d], origins[2d+1]) (row, column; periodic).  A gather:
+// one workgroup per (256-cell segment of an interior row, field, 8
+// members), thread = cell, so each member's row is one contiguous run.  tiles[2t], tiles[2t+1] are the
+// segment's row and index; list[list_off[t] .. list_off[t+1]) the drifters
+// whose window meets it, ascending.  Each cell adds its sum over them once,
+// so the result does not depend on the launch order.
+constexpr int kWinSeg = 256;
+constexpr int kWinNT = 256;
+constexpr int kWinPG = 8;     // consecutive members per workgroup
+
+__global__ __launch_bounds__(kWinNT) void k_iewpf_kalman_window(float* state, int n_members, int nx, int ny,
+                                                                long long member_stride, const float* basis,
+                                                                int n_drifters, int wh, int ww, const int* origins,
+                                                                const float* coef, int coef_stride,
+                                                                const int* tiles, const int* list_off,
+                                                                const int* list) {
+    const int t = blockIdx.x, f = blockIdx.y;
+    const int row = tiles[2 * t], seg = tiles[2 * t + 1];
+    if (row < 0 || row >= ny || seg < 0) return;
+    const int c = seg * kWinSeg + (int)threadIdx.x;
+    const bool in = c < nx;
+    const int pitch = nx + 4;
+    const long long plane = (long long)(ny + 4) * pitch;
+    const long long win = (long long)wh * ww;
+    float* x = state + f * plane + (long long)(row + 2) * pitch + (c + 2);
+    const int q0 = list_off[t], q1 = list_off[t + 1];
+    const int mw = blockIdx.z * kWinPG;     // the workgroup's first member
+    // the state is read first, so that its latency overlaps the sums
+    float acc[kWinPG], xv[kWinPG];
+#pragma unroll
+    for (int g = 0; g < kWinPG; ++g) {
+        const int m = mw + g;
+        acc[g] = 0.0f;
+        xv[g] = (in && m < n_members) ? x[m * member_stride] : 0.0f;
+    }
+    for (int q = q0; q < q1; ++q) {
+        const int d = list[q];
+        if ((unsigned)d >= (unsigned)n_drifters) continue;
+        int i = row - origins[2 * d], j = c - origins[2 * d + 1];
+        i += i < 0 ? ny : 0;
+        j += j < 0 ? nx : 0;
+        float b[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (in && (unsigned)i < (unsigned)wh && (unsigned)j < (unsigned)ww) {
+            const float* bp = basis + ((long long)d * 15 + f) * win + (long long)i * ww + j;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) b[k] = bp[k * 3 * win];
+        }
+        // the workgroup's kWinPG consecutive members: one contiguous run per k
+        const float* e = coef + (long long)d * 5 * coef_stride + mw;
+#pragma unroll
+        for (int g = 0; g < kWinPG; ++g)
+            acc[g] += (((e[g] * b[0] + e[coef_stride + g] * b[1]) + e[2 * coef_stride + g] * b[2])
+                       + e[3 * coef_stride + g] * b[3]) + e[4 * coef_stride + g] * b[4];
+    }
+    if (in) {
+#pragma unroll
+        for (int g = 0; g < kWinPG; ++g) {
+            const int m = mw + g;
+            if (m < n_members) x[m * member_stride] = xv[g] + acc[g];
+        }
+    }
+}
+
 int check_members(const int64_t* offsets, int n_members, int n) {
     if (!offsets || n_members < 1 || n < 1) return set_error(GOCEAN_EINVAL, "offsets NULL or %d members of %d", n_members, n);
     return 0;
@@ -209,6 +278,32 @@ int gocean_iewpf_kalman_patch(float* xi, const int64_t* offsets, int32_t n_membe
     hipLaunchKernelGGL(k_iewpf_kalman_patch, grid, dim3(kElemNT), 0, (hipStream_t)stream, xi, offsets, coarse_nx,
                        coarse_ny, patch, e, n_drifters, drifter, cx, cy);
     return check_launch("k_iewpf_kalman_patch");
+}
+
+int gocean_iewpf_kalman_window(float* state, int32_t n_members, int32_t nx, int32_t ny, int64_t member_stride,
+                               const float* basis, int32_t n_drifters, int32_t win_h, int32_t win_w,
+                               const int32_t* origins, const float* coef, int32_t coef_stride,
+                               const int32_t* tiles, int32_t n_tiles, const int32_t* list_off, const int32_t* list,
+                               void* stream) {
+    clear_error();
+    if (n_members < 1 || nx < 1 || ny < 1) return set_error(GOCEAN_EINVAL, "%d members of %d x %d", n_members, nx, ny);
+    if (member_stride < 3LL * (nx + 4) * (ny + 4))
+        return set_error(GOCEAN_EINVAL, "member stride %lld below three planes", (long long)member_stride);
+    if (n_drifters < 0 || n_tiles < 0) return set_error(GOCEAN_EINVAL, "drifters %d, tiles %d", n_drifters, n_tiles);
+    if (win_h < 1 || win_h > ny || win_w < 1 || win_w > nx)
+        return set_error(GOCEAN_EINVAL, "window %d x %d outside 1..%d x 1..%d", win_h, win_w, ny, nx);
+    if (n_tiles > 0 && n_drifters == 0) return set_error(GOCEAN_EINVAL, "tiles without drifters");
+    if (coef_stride < (n_members + kWinPG - 1) / kWinPG * kWinPG)
+        return set_error(GOCEAN_EINVAL, "coefficient stride %d below %d members rounded up to %d", coef_stride,
+                         n_members, kWinPG);
+    if (n_tiles == 0) return 0;
+    if (!state || !basis || !origins || !coef || !tiles || !list_off || !list)
+        return set_error(GOCEAN_EINVAL, "a buffer is NULL");
+    hipLaunchKernelGGL(k_iewpf_kalman_window, dim3(n_tiles, 3, (n_members + kWinPG - 1) / kWinPG),
+                       dim3(kWinNT), 0, (hipStream_t)stream, state,
+                       n_members, nx, ny, (long long)member_stride, basis, n_drifters, win_h, win_w, origins, coef,
+                       coef_stride, tiles, list_off, list);
+    return check_launch("k_iewpf_kalman_window");
 }
 
 }  // extern "C"

@@ -543,6 +543,23 @@ int gocean_iewpf_local_svd(float* xi, const int64_t* offsets, int32_t n_members,
 int gocean_iewpf_kalman_patch(float* xi, const int64_t* offsets, int32_t n_members, int32_t coarse_nx,
                               int32_t coarse_ny, const float* patch, const float* e, int32_t n_drifters,
                               int32_t drifter, int32_t cx, int32_t cy, void* stream);
+/* Every member's state += the Kalman-gain term of every drifter for
+ * interpolated model error (addKalmanGain, IEWPFOcean.py:548-595, whose term
+ * is the model-error perturbation of the patch above): state is n_members
+ * blocks of three (ny+4) x (nx+4) planes (eta, hu, hv), member_stride floats
+ * apart.  Drifter d's term is sum_k coef[d][k][m] basis[d][k] (coef:
+ * n_drifters x 5 x coef_stride floats, coef_stride at least n_members
+ * rounded up to a multiple of 8, the padding read but not used; basis:
+ * n_drifters x 5 x 3 x win_h x win_w floats), a win_h x win_w window of interior cells starting at row
+ * origins[2d], column origins[2d+1], periodic.  tiles: n_tiles (row,
+ * segment) pairs of 256-cell interior row segments; the drifters meeting tile
+ * t are list[list_off[t] .. list_off[t+1]), ascending.  Ghost cells are not
+ * written. */
+int gocean_iewpf_kalman_window(float* state, int32_t n_members, int32_t nx, int32_t ny, int64_t member_stride,
+                               const float* basis, int32_t n_drifters, int32_t win_h, int32_t win_w,
+                               const int32_t* origins, const float* coef, int32_t coef_stride,
+                               const int32_t* tiles, int32_t n_tiles, const int32_t* list_off, const int32_t* list,
+                               void* stream);
 
 /* ---- Localised ETKF (ETKFOcean.LETKF, gpu_ocean/SWESimulators/ETKFOcean.py:583-744) ----
  * The ensemble state is n_members blocks of three (ny+4) x pitch float planes

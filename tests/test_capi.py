@@ -267,3 +267,23 @@ def test_letkf_shape_and_argument_checks():
     assert apply_call(local_n=21) == -1 and "fit" in lib.gocean_last_error().decode()
     assert apply_call(tiles=None) == -1 and "NULL" in lib.gocean_last_error().decode()
     assert apply_call(n_tiles=-1) == -1
+
+
+def test_iewpf_kalman_window_argument_checks():
+    lib = G.load()
+    plane = 24 * 34
+
+    def call(n=4, nx=30, ny=20, stride=3 * plane, nd=2, wh=5, ww=7, n_tiles=3, state=16, lst=16, cs=32):
+        return lib.gocean_iewpf_kalman_window(state, n, nx, ny, stride, 16, nd, wh, ww, 16, 16, cs, 16, n_tiles,
+                                              16, lst, None)
+
+    assert call(n=0) == -1 and "members" in lib.gocean_last_error().decode()
+    assert call(stride=3 * plane - 1) == -1 and "stride" in lib.gocean_last_error().decode()
+    assert call(wh=21) == -1 and "window" in lib.gocean_last_error().decode()
+    assert call(ww=0) == -1 and "window" in lib.gocean_last_error().decode()
+    assert call(nd=0) == -1 and "without drifters" in lib.gocean_last_error().decode()
+    assert call(n_tiles=-1) == -1
+    assert call(state=None) == -1 and "NULL" in lib.gocean_last_error().decode()
+    assert call(lst=None) == -1 and "NULL" in lib.gocean_last_error().decode()
+    assert call(n=33, cs=39) == -1 and "coefficient stride" in lib.gocean_last_error().decode()
+    assert call(n_tiles=0) == 0                          # nothing to add, no launch

@@ -179,3 +179,58 @@ def test_iewpf_one_stage_matches_oracle(factor, n):
         ratio = err[:, f].max() / np.abs(incr[:, f]).max()
         print(fname, "max error / max increment", ratio, flush=True)
         assert ratio < 2e-5
+
+
+@pytest.mark.parametrize("N", [37, 133])
+def test_kalman_window_gather_equals_dense_sum(N):
+    """gocean_iewpf_kalman_window against the dense sum of every drifter's
+    window (float64 on the host): windows that wrap both ways, overlap each
+    other and cover a whole axis, a partial last segment, member counts not a
+    multiple of the 8 members one workgroup takes, and ghost cells
+    untouched."""
+    import torch
+    from SWESimulators import Common, IEWPFOcean as P, _gocean as Gc
+    print("case kalman window gather", N, flush=True)
+    Common.HIPContext(0)
+    lib = Gc.load()
+    nx, ny = 150, 37
+    plane = (ny + 4) * (nx + 4)
+    rs = np.random.RandomState(12)
+    Nd = 7
+    origins = np.column_stack([rs.randint(0, ny, Nd), rs.randint(0, nx, Nd)]).astype(np.int32)
+    origins[0] = (ny - 3, nx - 10)
+    sizes = np.column_stack([rs.randint(1, 25, Nd), rs.randint(1, 80, Nd)])
+    sizes[0] = (9, 30)
+    sizes[2] = (ny, 12)
+    wh, ww = int(sizes[:, 0].max()), int(sizes[:, 1].max())
+    basis = np.zeros((Nd, 5, 3, wh, ww), np.float32)
+    for d in range(Nd):
+        basis[d, :, :, :sizes[d, 0], :sizes[d, 1]] = rs.randn(5, 3, sizes[d, 0], sizes[d, 1])
+    coef = rs.randn(N, Nd, 5).astype(np.float32)
+    X0 = rs.randn(N, 3, ny + 4, nx + 4).astype(np.float32)
+    want = X0.astype(np.float64)
+    for d in range(Nd):
+        rows = 2 + (origins[d, 0] + np.arange(wh)) % ny
+        cols = 2 + (origins[d, 1] + np.arange(ww)) % nx
+        term = np.einsum("mk,kfij->mfij", coef[:, d].astype(np.float64), basis[d].astype(np.float64))
+        want[:, :, rows[:, None], cols[None, :]] += term
+    tiles, off, lst = P.kalman_window_tiles(origins, sizes, nx, ny)
+    assert tiles[:, 0].max() < ny and tiles[:, 1].max() * P.SEGMENT < nx and lst.max() < Nd
+    dcoef = P.kalman_window_coefficients(coef)
+    assert dcoef.shape == (Nd, 5, -(-N // 8) * 8)
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in
+         dict(X=X0, basis=basis, origins=origins, coef=dcoef, tiles=tiles.astype(np.int32),
+              off=off.astype(np.int32), lst=lst.astype(np.int32)).items()}
+    Gc.check(lib.gocean_iewpf_kalman_window(t["X"].data_ptr(), N, nx, ny, 3 * plane, t["basis"].data_ptr(), Nd,
+                                            wh, ww, t["origins"].data_ptr(), t["coef"].data_ptr(),
+                                            dcoef.shape[2], t["tiles"].data_ptr(), len(tiles), t["off"].data_ptr(),
+                                            t["lst"].data_ptr(), None), "gocean_iewpf_kalman_window")
+    torch.cuda.synchronize()
+    got = t["X"].to("cpu").numpy().astype(np.float64)
+    inner = (slice(None), slice(None), slice(2, ny + 2), slice(2, nx + 2))
+    scale = np.abs(want[inner] - X0[inner]).max()
+    assert np.abs(got[inner] - want[inner]).max() < 1e-6 * scale
+    ghost = np.ones((ny + 4, nx + 4), bool)
+    ghost[2:ny + 2, 2:nx + 2] = False
+    np.testing.assert_array_equal(got[:, :, ghost], X0[:, :, ghost])

@@ -68,3 +68,58 @@ def test_catmull_rom_equals_oracle_bicubic():
     c = rs.randn(7, 7)
     for (i, j, rx, ry) in ((3, 3, 0.0, 0.2), (2, 2, 1.0, 0.8), (3, 2, 0.4, 0.6)):
         assert abs(P.IEWPFOcean._catmull_rom(c, i, j, rx, ry) - I.bicubic_inner(c, i, j, rx, ry)) < 1e-12
+
+
+def test_circular_cover_is_the_shortest_periodic_interval():
+    """Against brute force over every start and length on small masks,
+    including wrapped, full and empty ones."""
+    rs = np.random.RandomState(4)
+    masks = [np.zeros(7, bool), np.ones(7, bool), np.array([1, 0, 0, 0, 0, 0, 1], bool)]
+    masks += [rs.rand(n) < p for n in (1, 5, 9, 16) for p in (0.1, 0.3, 0.7) for _ in range(6)]
+    for m in masks:
+        n = m.size
+        start, length = P.circular_cover(m)
+        if not m.any():
+            assert (start, length) == (0, 0)
+            continue
+        cover = np.zeros(n, bool)
+        cover[(start + np.arange(length)) % n] = True
+        assert np.all(cover[m]), (m, start, length)
+        best = min(L for L in range(1, n + 1) for s in range(n)
+                   if np.all(np.isin(np.flatnonzero(m), (s + np.arange(L)) % n)))
+        assert length == best
+
+
+def test_kalman_window_tiles_list_exactly_the_drifters_meeting_each_segment():
+    """Every 256-cell row segment a window meets is listed with that drifter,
+    nothing else is, and each list is ascending."""
+    nx, ny = 150, 37
+    rs = np.random.RandomState(8)
+    origins = np.column_stack([rs.randint(0, ny, 9), rs.randint(0, nx, 9)])
+    sizes = np.column_stack([rs.randint(1, 20, 9), rs.randint(1, 90, 9)])
+    sizes[3] = (0, 0)                                    # a drifter without a term
+    sizes[5] = (ny, nx)                                  # one covering everything
+    tiles, off, lst = P.kalman_window_tiles(origins, sizes, nx, ny)
+    nseg = -(-nx // P.SEGMENT)
+    want = {}
+    for d in range(9):
+        for i in range(sizes[d, 0]):
+            for j in range(sizes[d, 1]):
+                r, c = (origins[d, 0] + i) % ny, (origins[d, 1] + j) % nx
+                want.setdefault((r, c // P.SEGMENT), set()).add(d)
+    assert len(tiles) == len(want) == ny * nseg
+    assert off[0] == 0 and off[-1] == len(lst) and np.all(np.diff(off) > 0)
+    for t, (r, sg) in enumerate(tiles):
+        got = lst[off[t]:off[t + 1]]
+        assert np.all(np.diff(got) > 0)
+        assert set(got.tolist()) == want[(r, sg)]
+    none = P.kalman_window_tiles(origins[:1], np.zeros((1, 2), int), nx, ny)
+    assert none[0].shape == (0, 2) and none[1].tolist() == [0] and none[2].size == 0
+
+
+def test_kalman_window_coefficients_layout():
+    c = np.random.RandomState(2).randn(37, 4, 5)
+    out = P.kalman_window_coefficients(c)
+    assert out.shape == (4, 5, 40) and out.dtype == np.float32
+    np.testing.assert_array_equal(out[:, :, :37], c.transpose(1, 2, 0).astype(np.float32))
+    assert not out[:, :, 37:].any()
