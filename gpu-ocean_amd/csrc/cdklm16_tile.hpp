@@ -300,7 +300,20 @@ __device__ __forceinline__ Consts make_consts(const SubstepArgs& A) {
 // ---------------------------------------------------------------------------
 // the tile
 // ---------------------------------------------------------------------------
-template <int BW, int BH, int NT, int MODE, bool TAPE = false>
+// A tile whose every access (padded rows by-1 .. by+BH+4, columns bx-1 ..
+// bx+BW+4) lies at least one cell inside the interior rows 2 .. ny+1 and
+// columns 2 .. nx+1: no boundary remap, wall flip, clamp or periodic ghost
+// write applies to it.
+template <int BW, int BH>
+__device__ __forceinline__ bool tile_is_interior(const SubstepArgs& A, int tile_x, int tile_y) {
+    const int bx = tile_x * BW, by = tile_y * BH;
+    return bx >= 4 && by >= 4 && bx + BW + 4 <= A.nx && by + BH + 4 <= A.ny;
+}
+
+// EDGE = false is the body for tiles with tile_is_interior: the boundary
+// flags are constant false and the clamps are dropped, which changes no
+// result there and removes their instructions.
+template <int BW, int BH, int NT, int MODE, bool TAPE = false, bool EDGE = true>
 __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int tile_y,
                                            TileSmem<BW, BH, MODE, TAPE>& S) {
     static_assert(!TAPE || MODE == kModeZero, "the branch record is defined for zero rotation");
@@ -312,14 +325,14 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
     const int by = tile_y * BH;     // padded row of the tile's first halo row
     const Consts K = make_consts(A);
 
-    const bool wN = A.bc_n() == kWall, wS = A.bc_s() == kWall;
-    const bool wE = A.bc_e() == kWall, wW = A.bc_w() == kWall;
-    const bool pNS = A.bc_n() == kPeriodic, pEW = A.bc_e() == kPeriodic;
+    const bool wN = EDGE && A.bc_n() == kWall, wS = EDGE && A.bc_s() == kWall;
+    const bool wE = EDGE && A.bc_e() == kWall, wW = EDGE && A.bc_w() == kWall;
+    const bool pNS = EDGE && A.bc_n() == kPeriodic, pEW = EDGE && A.bc_e() == kPeriodic;
 
     // ---- phase A: corner depths (CDKLM16_kernel.cu:509-525) ----
     for (int it = tid; it < (BH + 5) * (BW + 5); it += NT) {
         const int r = it / (BW + 5), c = it - r * (BW + 5);
-        const int gr = min(by + r, ny + 4), gc = min(bx + c, nx + 4);
+        const int gr = EDGE ? min(by + r, ny + 4) : by + r, gc = EDGE ? min(bx + c, nx + 4) : bx + c;
         float h = A.Hi[(size_t)gr * A.hi_pitch + gc];
         if (fabsf(h - K.land) < kDryEps) h = kDryFlag;
         S.Hi[r][c] = h;
@@ -343,7 +356,7 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
             rr = band < 2 ? band : BH + band;     // 0,1,BH+2,BH+3
             cc = 2 + (k - band * BW);
         }
-        const int R = min(by + rr, ny + 3), C = min(bx + cc, nx + 3);
+        const int R = EDGE ? min(by + rr, ny + 3) : by + rr, C = EDGE ? min(bx + cc, nx + 3) : bx + cc;
         int sr = R, sc = C;
         bool neg_u = false, neg_v = false;
         if (R < 2) {

@@ -88,7 +88,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
     A.tiles_x = M.tiles_x; A.tiles_y = 0;
     const int lt = t - M.tile_begin;
     const int ty = lt / M.tiles_x, tx = lt - ty * M.tiles_x;
-    cdklm_tile<BW, BH, NT, MODE>(A, tx, ty, S);
+    if (tile_is_interior<BW, BH>(A, tx, ty))
+        cdklm_tile<BW, BH, NT, MODE, false, false>(A, tx, ty, S);
+    else
+        cdklm_tile<BW, BH, NT, MODE>(A, tx, ty, S);
 }
 
 // per-member CFL minimum: one workgroup per member; dt[m] = courant * min

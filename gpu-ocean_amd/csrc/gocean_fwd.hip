@@ -102,7 +102,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
     __shared__ TileSmem<BW, BH, MODE> S;
     const int t = xcd_band(blockIdx.x, gridDim.x);
     const int ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
-    cdklm_tile<BW, BH, NT, MODE>(A, tx, ty, S);
+    if (tile_is_interior<BW, BH>(A, tx, ty))
+        cdklm_tile<BW, BH, NT, MODE, false, false>(A, tx, ty, S);
+    else
+        cdklm_tile<BW, BH, NT, MODE>(A, tx, ty, S);
 }
 
 template <int BW, int BH, int NT, int MODE>
