@@ -262,7 +262,10 @@ __global__ __launch_bounds__(kEnsTapeNT) __attribute__((amdgpu_waves_per_eu(8)))
     A.tiles_x = M.ftiles_x; A.tiles_y = 0;
     const int lt = t - M.ftile_begin;
     const int ty = lt / M.ftiles_x, tx = lt - ty * M.ftiles_x;
-    cdklm_tile<kEnsTapeBW, kEnsTapeBH, kEnsTapeNT, kModeZero, true>(A, tx, ty, S);
+    if (tile_is_interior<kEnsTapeBW, kEnsTapeBH>(A, tx, ty))
+        cdklm_tile<kEnsTapeBW, kEnsTapeBH, kEnsTapeNT, kModeZero, true, false>(A, tx, ty, S);
+    else
+        cdklm_tile<kEnsTapeBW, kEnsTapeBH, kEnsTapeNT, kModeZero, true>(A, tx, ty, S);
 }
 
 struct EnsAdjLaunch {

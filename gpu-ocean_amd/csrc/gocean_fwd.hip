@@ -122,7 +122,10 @@ __global__ __launch_bounds__(NT) void k_substep_tape(SubstepArgs A) {
     __shared__ TileSmem<BW, BH, kModeZero, true> S;
     const int t = xcd_band(blockIdx.x, gridDim.x);
     const int ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
-    cdklm_tile<BW, BH, NT, kModeZero, true>(A, tx, ty, S);
+    if (tile_is_interior<BW, BH>(A, tx, ty))
+        cdklm_tile<BW, BH, NT, kModeZero, true, false>(A, tx, ty, S);
+    else
+        cdklm_tile<BW, BH, NT, kModeZero, true>(A, tx, ty, S);
 }
 
 template <int BW, int BH, int NT>
