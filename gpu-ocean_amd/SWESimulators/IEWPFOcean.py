@@ -61,10 +61,10 @@ def circular_cover(mask):
 
 def kalman_window_coefficients(coef):
     """(N, N_d, 5) per-member coefficients -> the (N_d, 5, N rounded up to
-    8) float32 layout gocean_iewpf_kalman_window reads (zero padding)."""
+    16) float32 layout gocean_iewpf_kalman_window reads (zero padding)."""
     coef = np.asarray(coef)
     n = coef.shape[0]
-    out = np.zeros((coef.shape[1], 5, -(-n // 8) * 8), dtype=np.float32)
+    out = np.zeros((coef.shape[1], 5, -(-n // 16) * 16), dtype=np.float32)
     out[:, :, :n] = coef.transpose(1, 2, 0)
     return out
 

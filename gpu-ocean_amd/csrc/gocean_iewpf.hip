@@ -147,14 +147,14 @@ __global__ __launch_bounds__(kElemNT) void k_iewpf_kalman_patch(float* xi, const
 // term of drifter d is sum_k coef[d][k][m] basis[d][k] with five fields per
 // drifter, non-zero on a window of win_h x win_w interior cells whose first
 // cell is (origins[2d], origins[2d+1]) (row, column; periodic).  A gather:
-// one workgroup per (256-cell segment of an interior row, field, 8
+// one workgroup per (256-cell segment of an interior row, field, 16
 // members), thread = cell, so each member's row is one contiguous run.  tiles[2t], tiles[2t+1] are the
 // segment's row and index; list[list_off[t] .. list_off[t+1]) the drifters
 // whose window meets it, ascending.  Each cell adds its sum over them once,
 // so the result does not depend on the launch order.
 constexpr int kWinSeg = 256;
 constexpr int kWinNT = 256;
-constexpr int kWinPG = 8;     // consecutive members per workgroup
+constexpr int kWinPG = 16;    // consecutive members per workgroup
 
 __global__ __launch_bounds__(kWinNT) void k_iewpf_kalman_window(float* state, int n_members, int nx, int ny,
                                                                 long long member_stride, const float* basis,

@@ -549,7 +549,7 @@ int gocean_iewpf_kalman_patch(float* xi, const int64_t* offsets, int32_t n_membe
  * blocks of three (ny+4) x (nx+4) planes (eta, hu, hv), member_stride floats
  * apart.  Drifter d's term is sum_k coef[d][k][m] basis[d][k] (coef:
  * n_drifters x 5 x coef_stride floats, coef_stride at least n_members
- * rounded up to a multiple of 8, the padding read but not used; basis:
+ * rounded up to a multiple of 16, the padding read but not used; basis:
  * n_drifters x 5 x 3 x win_h x win_w floats), a win_h x win_w window of interior cells starting at row
  * origins[2d], column origins[2d+1], periodic.  tiles: n_tiles (row,
  * segment) pairs of 256-cell interior row segments; the drifters meeting tile

@@ -285,5 +285,5 @@ def test_iewpf_kalman_window_argument_checks():
     assert call(n_tiles=-1) == -1
     assert call(state=None) == -1 and "NULL" in lib.gocean_last_error().decode()
     assert call(lst=None) == -1 and "NULL" in lib.gocean_last_error().decode()
-    assert call(n=33, cs=39) == -1 and "coefficient stride" in lib.gocean_last_error().decode()
+    assert call(n=33, cs=47) == -1 and "coefficient stride" in lib.gocean_last_error().decode()
     assert call(n_tiles=0) == 0                          # nothing to add, no launch

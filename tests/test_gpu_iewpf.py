@@ -186,7 +186,7 @@ def test_kalman_window_gather_equals_dense_sum(N):
     """gocean_iewpf_kalman_window against the dense sum of every drifter's
     window (float64 on the host): windows that wrap both ways, overlap each
     other and cover a whole axis, a partial last segment, member counts not a
-    multiple of the 8 members one workgroup takes, and ghost cells
+    multiple of the 16 members one workgroup takes, and ghost cells
     untouched."""
     import torch
     from SWESimulators import Common, IEWPFOcean as P, _gocean as Gc
@@ -217,7 +217,7 @@ def test_kalman_window_gather_equals_dense_sum(N):
     tiles, off, lst = P.kalman_window_tiles(origins, sizes, nx, ny)
     assert tiles[:, 0].max() < ny and tiles[:, 1].max() * P.SEGMENT < nx and lst.max() < Nd
     dcoef = P.kalman_window_coefficients(coef)
-    assert dcoef.shape == (Nd, 5, -(-N // 8) * 8)
+    assert dcoef.shape == (Nd, 5, -(-N // 16) * 16)
     dev = torch.device("cuda:0")
     t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in
          dict(X=X0, basis=basis, origins=origins, coef=dcoef, tiles=tiles.astype(np.int32),

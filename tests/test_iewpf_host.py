@@ -120,6 +120,6 @@ def test_kalman_window_tiles_list_exactly_the_drifters_meeting_each_segment():
 def test_kalman_window_coefficients_layout():
     c = np.random.RandomState(2).randn(37, 4, 5)
     out = P.kalman_window_coefficients(c)
-    assert out.shape == (4, 5, 40) and out.dtype == np.float32
+    assert out.shape == (4, 5, 48) and out.dtype == np.float32
     np.testing.assert_array_equal(out[:, :, :37], c.transpose(1, 2, 0).astype(np.float32))
     assert not out[:, :, 37:].any()
