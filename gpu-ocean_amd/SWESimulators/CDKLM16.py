@@ -14,9 +14,10 @@ Differences from the reference, by design:
   * runs of equal time steps are issued with one library call
     (``gocean_rk_steps``); ``step_n`` and ``capture_graph`` expose fixed-dt
     stepping without any host synchronisation;
-  * open boundaries (3, 4) and gridded wind are not part of this build and
-    raise ``NotImplementedError``; NetCDF output is NetCDF-3 (scipy.io)
-    since netCDF4 is not available.
+  * open boundaries (3, 4) are not part of this build and raise
+    ``NotImplementedError``; gridded wind is read with exact bilinear weights
+    where the reference's texture unit rounds them to 8 fractional bits;
+    NetCDF output is NetCDF-3 (scipy.io) since netCDF4 is not available.
 """
 
 from __future__ import annotations
@@ -71,7 +72,10 @@ class CDKLM16(Simulator.Simulator):
         if max_wind_direction_perturbation > 0:
             raise NotImplementedError("wind direction perturbation needs the legacy wind classes")
         if not wind_stress.is_uniform():
-            raise NotImplementedError("only uniform (1x1) wind stress fields are supported")
+            shapes = {np.asarray(a).shape for a in list(wind_stress.X) + list(wind_stress.Y)}
+            if len(shapes) != 1:
+                raise ValueError("gridded wind stress: every X and Y field must have the same shape, got "
+                                 + str(sorted(shapes)))
 
         ghost_cells_x = 2
         ghost_cells_y = 2
@@ -202,13 +206,33 @@ class CDKLM16(Simulator.Simulator):
         return G.stream_handle(self.gpu_stream)
 
     def _wind(self):
+        """The wind of the current time (Simulator.update_wind_stress,
+        Simulator.py:145-223): uniform values, or for gridded fields the two
+        bracketing samples in one device buffer (X0, X1, Y0, Y1), uploaded
+        when the bracketing interval changes."""
         t0, t1, wt = self.wind_stress_interpolation()
         w = G.Wind()
         w.t = float(np.float32(wt))
-        w.x0 = float(np.asarray(self.wind_stress.X[t0], np.float32).flat[0])
-        w.x1 = float(np.asarray(self.wind_stress.X[t1], np.float32).flat[0])
-        w.y0 = float(np.asarray(self.wind_stress.Y[t0], np.float32).flat[0])
-        w.y1 = float(np.asarray(self.wind_stress.Y[t1], np.float32).flat[0])
+        ws = self.wind_stress
+        if ws.is_uniform():
+            w.x0 = float(np.asarray(ws.X[t0], np.float32).flat[0])
+            w.x1 = float(np.asarray(ws.X[t1], np.float32).flat[0])
+            w.y0 = float(np.asarray(ws.Y[t0], np.float32).flat[0])
+            w.y1 = float(np.asarray(ws.Y[t1], np.float32).flat[0])
+            return w
+        torch = Common._torch()
+        shape = np.asarray(ws.X[0]).shape
+        key = (id(ws), t0, t1)
+        if getattr(self, "_wind_key", None) != key:
+            host = np.stack([np.asarray(a, np.float32) for a in (ws.X[t0], ws.X[t1], ws.Y[t0], ws.Y[t1])])
+            with torch.cuda.stream(self.gpu_stream):
+                if getattr(self, "_wind_grid", None) is None or tuple(self._wind_grid.shape) != host.shape:
+                    self._wind_grid = torch.empty(host.shape, dtype=torch.float32,
+                                                  device=self.gpu_ctx.torch_device)
+                self._wind_grid.copy_(torch.from_numpy(host))
+            self._wind_key = key
+        w.grid = self._wind_grid.data_ptr()
+        w.grid_ny, w.grid_nx = int(shape[0]), int(shape[1])
         return w
 
     def _ensure_h2(self):

@@ -177,6 +177,10 @@ class CDKLM16Ensemble(object):
         ``model_error`` each step is followed by the batched perturbation and
         the ghost-cell exchange (CDKLM16.step with a small-scale
         perturbation, CDKLM16.py:580-586)."""
+        if self.rk_order == 1 and (int(n_steps) % 2 or model_error or self.drifters is not None):
+            raise ValueError("an rk_order 1 ensemble advances in pairs of Euler steps (the two state "
+                             "buffers trade roles every step): use an even n_steps, without per-step "
+                             "model error or drifters")
         if not model_error and self.drifters is None:
             self._rk(n_steps)
         else:
@@ -194,6 +198,13 @@ class CDKLM16Ensemble(object):
     # ------------------------------------------------------------------
     # time-targeted steps (CDKLM16.step / dataAssimilationStep for every member)
     # ------------------------------------------------------------------
+    def _need_single_steps(self, what):
+        # members that finish early sit out a launch with dt 0 and leave the
+        # other buffer untouched, so single Euler steps cannot swap buffers
+        if self.rk_order == 1:
+            raise NotImplementedError(f"{what} needs rk_order 2 or 3: an rk_order 1 ensemble advances "
+                                      "in pairs of Euler steps")
+
     def _dt_sequences(self, dt64, t_span):
         """Per member, the local time steps of CDKLM16.step(t_span) with the
         member's dt (CDKLM16.py:508-593): local_dt = float32(min(dt,
@@ -228,6 +239,7 @@ class CDKLM16Ensemble(object):
         (dt 0 in their column of the launch's dt row).  Drifters, when
         enabled, move with the same local steps."""
         torch = Common._torch()
+        self._need_single_steps("step_to")
         if dt64 is None:
             if getattr(self, "dt64", None) is None:
                 self.update_dt_host()
@@ -265,6 +277,7 @@ class CDKLM16Ensemble(object):
         CDKLM16.dataAssimilationStep (CDKLM16.py:650-704) for every member at
         once, as EnsembleFromFiles.stepToObservation runs it member by member
         (EnsembleFromFiles.py:302-325).  Members share their time."""
+        self._need_single_steps("dataAssimilationStep")
         t0 = float(self.t[0])
         assert np.all(self.t == t0), "members must share their time"
         mts = float(model_time_step)
@@ -435,6 +448,9 @@ class CDKLM16Ensemble(object):
         ``model_error``, each step followed by the batched perturbation; the
         generator state lives on the device, so every replay draws new
         numbers; with drifters enabled, each step also moves them)."""
+        if self.rk_order == 1 and (int(n_steps) % 2 or model_error or self.drifters is not None):
+            raise ValueError("an rk_order 1 ensemble graph needs an even n_steps, without per-step "
+                             "model error or drifters")
         self.gpu_stream.synchronize()
         self.graph_steps = int(n_steps)
         if not model_error and self.drifters is None:

@@ -1,8 +1,9 @@
 """Wind-stress forcing (gpu_ocean/SWESimulators/WindStress.py:34-60).
 
-A time series of wind-stress fields X, Y (tau/rho_water, m^2/s^2).  The HIP
-step supports uniform (1x1) fields, which is how the reference expresses
-uniform wind (its 1x1 textures return the single value everywhere).
+A time series of wind-stress fields X, Y (tau/rho_water, m^2/s^2).  1x1
+fields are uniform wind (the reference's 1x1 textures return the single value
+everywhere); larger fields are gridded wind, which the simulator reads at every
+cell like the reference's bilinear textures.
 """
 
 from __future__ import annotations

@@ -38,7 +38,8 @@ class Domain(ctypes.Structure):
 class Wind(ctypes.Structure):
     """gocean_wind_t"""
     _fields_ = [("t", c_float), ("x0", c_float), ("x1", c_float),
-                ("y0", c_float), ("y1", c_float)]
+                ("y0", c_float), ("y1", c_float),
+                ("grid", c_void_p), ("grid_nx", ctypes.c_int32), ("grid_ny", ctypes.c_int32)]
 
 
 class Fields(ctypes.Structure):

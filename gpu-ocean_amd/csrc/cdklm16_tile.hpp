@@ -46,6 +46,11 @@ struct SubstepArgs {
     // wind stress at the substep's time, interpolated on the host
     // (wind_lerp: the kernel's former expression, so the same bits)
     float wind_x, wind_y;
+    // gridded wind (nullptr: the uniform wind above): X0, X1, Y0, Y1 planes
+    // of wind_wy x wind_wx, time weight wind_t
+    const float* wind_grid;
+    int wind_wx, wind_wy;
+    float wind_t;
     float dt;
     // bits 0-7: north, east, south, west boundary codes (2 bits each); bits
     // 8-9: rk_order; bits 10-11: rk_step.  One word instead of six keeps the
@@ -76,7 +81,47 @@ struct SubstepArgs {
 };
 
 // wind stress at interpolation weight t between two samples
-inline float wind_lerp(float t, float a0, float a1) { return t * a1 + (1.0f - t) * a0; }
+__host__ __device__ inline float wind_lerp(float t, float a0, float a1) { return t * a1 + (1.0f - t) * a0; }
+
+// One axis of the reference's wind texture lookup (common.cu:616-650 from
+// CDKLM16_kernel.cu:827-828): normalised coordinate s = (ti + 0.5) / n of
+// the ghost-inclusive cell index ti, texel coordinate s * w - 0.5 (texel
+// centres at (k + 0.5) / w), clamped neighbours and the linear weight.
+struct WindAxis {
+    int i0, i1;
+    float a;
+};
+
+__device__ __forceinline__ WindAxis wind_axis(int ti, int n, int w) {
+    const float s = ((float)ti + 0.5f) / (float)n;
+    const float x = s * (float)w - 0.5f;
+    const float x0 = floorf(x);
+    WindAxis o;
+    o.a = x - x0;
+    const int k = (int)x0;
+    o.i0 = min(max(k, 0), w - 1);
+    o.i1 = min(max(k + 1, 0), w - 1);
+    return o;
+}
+
+// bilinear value of one wind plane (row length w) at (ax, ay)
+__device__ __forceinline__ float wind_bilinear(const float* F, int w, const WindAxis& ax, const WindAxis& ay) {
+    const float a = ax.a, b = ay.a;
+    const float* r0 = F + (size_t)ay.i0 * w;
+    const float* r1 = F + (size_t)ay.i1 * w;
+    return ((((1.0f - a) * (1.0f - b)) * r0[ax.i0] + (a * (1.0f - b)) * r0[ax.i1])
+            + ((1.0f - a) * b) * r1[ax.i0]) + (a * b) * r1[ax.i1];
+}
+
+// gridded wind stress (X, Y) of one cell, interpolated in time
+__device__ __forceinline__ void wind_grid_at(const float* grid, int wx, int wy, float wt,
+                                             const WindAxis& ax, const WindAxis& ay, float& X, float& Y) {
+    const size_t pl = (size_t)wx * wy;
+    const float x0 = wind_bilinear(grid, wx, ax, ay), x1 = wind_bilinear(grid + pl, wx, ax, ay);
+    const float y0 = wind_bilinear(grid + 2 * pl, wx, ax, ay), y1 = wind_bilinear(grid + 3 * pl, wx, ax, ay);
+    X = wind_lerp(wt, x0, x1);
+    Y = wind_lerp(wt, y0, y1);
+}
 
 inline void substep_derive(SubstepArgs& A) {
     const float two_g = 2.0f * A.g;
@@ -129,10 +174,17 @@ __host__ __device__ __forceinline__ uint32_t minmod_sel(float b, float c, float 
 constexpr int kCodeX = 0, kCodeY = 8, kCodeFx = 16, kCodeFy = 21;
 
 // common.cu:822-824.  The reference's "0.5*eps" is a double literal, so that
-// one addition is evaluated in double there; this does the same.
+// one addition is evaluated in double there and rounded to float.  The
+// float32 sum below gives the same bits.  With c = 0.5*eps (exact in float)
+// of exponent E and 0 <= a = h*h/(2 eps) < c (the branch needs |h| < eps):
+// if a >= 2^(E-28), every bit of a and c is a multiple of 2^(E-51) and the
+// sum is below 2^(E+2), so the double sum is exact and one rounding to float
+// remains; if a < 2^(E-28), a is below half an ulp of c and both forms give
+// c.  (Checked exhaustively over every float32 h in [2^-17, 0.1) for
+// eps = 0.1 by the test suite.)
 __host__ __device__ __forceinline__ float desingularize(float h, float hu, float eps) {
     const float a = (h * h) / (2.0f * eps);
-    const float b = (float)((double)a + 0.5 * (double)eps);
+    const float b = a + 0.5f * eps;
     return hu / fmaxf(fminf(b, eps), fabsf(h));
 }
 
@@ -716,8 +768,6 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
 
     // ---- phase E: sources, RK update, store (:802-1004) ----
     const float dt = A.dt_dev ? *A.dt_dev : A.dt;
-    const float X = A.wind_x;
-    const float Y = A.wind_y;
     const bool friction = A.r > 0.0f;
     for (int it = tid; it < BW * BH; it += NT) {
         const int j = it / BW, i = it - j * BW;
@@ -761,6 +811,10 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
 
         float st1 = 0.0f, st2 = 0.0f;
         if (hreg >= K.cutoff && ec != kDryFlag) {
+            float X = A.wind_x, Y = A.wind_y;
+            if (A.wind_grid)
+                wind_grid_at(A.wind_grid, A.wind_wx, A.wind_wy, A.wind_t, wind_axis(C, nx, A.wind_wx),
+                             wind_axis(R, ny, A.wind_wy), X, Y);
             const float RHxp = 0.5f * (S.Hi[rr][cc + 1] + S.Hi[rr + 1][cc + 1]);
             const float RHxm = 0.5f * (S.Hi[rr][cc] + S.Hi[rr + 1][cc]);
             const float RHyp = 0.5f * (S.Hi[rr + 1][cc] + S.Hi[rr + 1][cc + 1]);

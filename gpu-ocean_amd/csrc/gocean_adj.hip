@@ -74,6 +74,8 @@ static int fwd_tape(const gocean_domain_t* d, const gocean_wind_t* wind, float d
     fill_args(d, A);
     A.dt = dt; A.dt_dev = dt_dev;
     A.set_rk_step(rk_step);
+    if (wind && wind->grid)
+        return set_error(GOCEAN_EINVAL, "the adjoint takes uniform wind only (wind grid given)");
     if (wind) {
         A.wind_x = wind_lerp(wind->t, wind->x0, wind->x1);
         A.wind_y = wind_lerp(wind->t, wind->y0, wind->y1);

@@ -147,6 +147,8 @@ int fill_drift(const gocean_domain_t* d, int num, float sensitivity, const gocea
     if (num < 0) return set_error(GOCEAN_EINVAL, "number of drifters %d < 0", num);
     if (wind_factor != 0.0f && !wind)
         return set_error(GOCEAN_EINVAL, "a wind drift factor needs the wind");
+    if (wind_factor != 0.0f && wind->grid)
+        return set_error(GOCEAN_EINVAL, "drifters take uniform wind only (wind grid given)");
     if (x_zero < 0 || y_zero < 0 || x_zero > 2 || y_zero > 2)
         return set_error(GOCEAN_EINVAL, "reference cells (%d, %d) outside the two ghost cells", x_zero, y_zero);
     M = DriftMember{};

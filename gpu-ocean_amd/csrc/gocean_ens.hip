@@ -15,6 +15,7 @@
 
 #include "../../include/gocean.h"
 #include "adjoint_tile.hpp"
+#include "cdklm16_march.hpp"
 #include "cdklm16_tile.hpp"
 #include "gocean_internal.hpp"
 
@@ -52,18 +53,9 @@ struct EnsLaunch {
 };
 
 // 512-thread tiles: 8 waves per SIMD, as k_substep
-template <int BW, int BH, int NT, int MODE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 8 : 1))) void k_ens_substep(EnsLaunch L) {
-    __shared__ TileSmem<BW, BH, MODE> S;
-    const int t = xcd_band(blockIdx.x, gridDim.x);
-    const int m = L.tile_member[t];
-    // dt 0 never occurs as a step length (the step loop only runs while time
-    // is left): it marks a member that has reached its end time and is left
-    // as it is, so members with different dt can share the launches of a
-    // time-targeted step
-    if (L.dt[m] == 0.0f) return;                       // uniform over the workgroup
-    const EnsMember& M = L.members[m];
-    SubstepArgs A;
+// the member's substep arguments (its constants, offsets and device dt)
+template <int MODE>
+__device__ __forceinline__ void member_args(const EnsLaunch& L, const EnsMember& M, int m, SubstepArgs& A) {
     const long long so = M.state_off, pl = M.plane;
     A.eta_in = L.state_in + so; A.hu_in = L.state_in + so + pl; A.hv_in = L.state_in + so + 2 * pl;
     A.eta_qn = L.state_qn + so; A.hu_qn = L.state_qn + so + pl; A.hv_qn = L.state_qn + so + 2 * pl;
@@ -86,6 +78,37 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
     A.set_flags(M.bc_n, M.bc_e, M.bc_s, M.bc_w, L.rk_order, L.rk_step);
     A.vdx = M.vdx; A.udy = M.udy; A.r_two_g = M.r_two_g; A.r_dx = M.r_dx; A.r_dy = M.r_dy;
     A.tiles_x = M.tiles_x; A.tiles_y = 0;
+}
+
+// column-marching substep over every member's strips: wavefront -> strip
+// through the tile map (one entry per strip), as k_march
+template <int BH>
+__global__ __launch_bounds__(256) void k_ens_march(EnsLaunch L) {
+    const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
+    if (wave >= L.n_tiles) return;
+    const int m = L.tile_member[wave];
+    if (L.dt[m] == 0.0f) return;                       // member done (see k_ens_substep)
+    const EnsMember& M = L.members[m];
+    SubstepArgs A;
+    member_args<kModeZero>(L, M, m, A);
+    const int lt = wave - M.tile_begin;
+    const int sy = lt / M.tiles_x, sx = lt - sy * M.tiles_x;
+    march::march_strip<BH>(A, sx, sy);
+}
+
+template <int BW, int BH, int NT, int MODE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 8 : 1))) void k_ens_substep(EnsLaunch L) {
+    __shared__ TileSmem<BW, BH, MODE> S;
+    const int t = xcd_band(blockIdx.x, gridDim.x);
+    const int m = L.tile_member[t];
+    // dt 0 never occurs as a step length (the step loop only runs while time
+    // is left): it marks a member that has reached its end time and is left
+    // as it is, so members with different dt can share the launches of a
+    // time-targeted step
+    if (L.dt[m] == 0.0f) return;                       // uniform over the workgroup
+    const EnsMember& M = L.members[m];
+    SubstepArgs A;
+    member_args<MODE>(L, M, m, A);
     const int lt = t - M.tile_begin;
     const int ty = lt / M.tiles_x, tx = lt - ty * M.tiles_x;
     if (tile_is_interior<BW, BH>(A, tx, ty))
@@ -106,9 +129,10 @@ __global__ __launch_bounds__(256) void k_ens_dt(const EnsMember* members, const 
     const float* H = Hm + M.hm_off;
     const float qdx = 0.25f * M.dx, qdy = 0.25f * M.dy;
     float m = 1.0e5f;
-    const int n = M.nx * M.ny;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const int j = k / M.nx + 2, i = k % M.nx + 2;
+    // rows of the interior, columns across the workgroup: no per-cell
+    // index division
+    for (int j = 2; j < M.ny + 2; ++j)
+    for (int i = threadIdx.x + 2; i < M.nx + 2; i += blockDim.x) {
         const long long o = (long long)j * M.pitch + i;
         const float Hc = H[o];
         const float h = eta[o] + Hc;
@@ -371,6 +395,11 @@ static void ens_launch_t(const EnsLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL((k_ens_substep<BW, BH, NT, MODE>), dim3(L.n_tiles), dim3(NT), 0, s, L);
 }
 
+template <int BH>
+static void ens_march_t(const EnsLaunch& L, hipStream_t s) {
+    hipLaunchKernelGGL((k_ens_march<BH>), dim3((L.n_tiles + 3) / 4), dim3(256), 0, s, L);
+}
+
 static int ens_launch(const gocean_ensemble* E, EnsLaunch L, hipStream_t s) {
     L.members = E->d_members;
     L.tile_member = E->d_tile_member;
@@ -388,6 +417,9 @@ static int ens_launch(const gocean_ensemble* E, EnsLaunch L, hipStream_t s) {
             case 4: ens_launch_t<64, 4, 256, kModeZero>(L, s); break;
             case 5: ens_launch_t<32, 16, 512, kModeZero>(L, s); break;
             case 6: ens_launch_t<32, 14, 512, kModeZero>(L, s); break;
+            case 7: ens_march_t<8>(L, s); break;
+            case 8: ens_march_t<16>(L, s); break;
+            case 9: ens_march_t<32>(L, s); break;
         }
     }
     return check_launch("ensemble substep launch");
@@ -479,6 +511,10 @@ int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_confi
         M.dx = d.dx; M.dy = d.dy; M.g = d.g; M.r = d.r; M.theta = d.theta;
         M.desing = d.desing_eps; M.flux_eps = d.flux_slope_eps; M.cutoff = d.depth_cutoff;
         M.land = d.land_value; M.f = d.f; M.north_x = d.north_x; M.north_y = d.north_y;
+        if (m.wind.grid) {
+            delete E;
+            return set_error(GOCEAN_EINVAL, "member %d: ensembles take uniform wind only (wind grid given)", k);
+        }
         M.wind_x = wind_lerp(m.wind.t, m.wind.x0, m.wind.x1);
         M.wind_y = wind_lerp(m.wind.t, m.wind.y0, m.wind.y1);
         M.bc_n = d.bc_north; M.bc_e = d.bc_east; M.bc_s = d.bc_south; M.bc_w = d.bc_west;

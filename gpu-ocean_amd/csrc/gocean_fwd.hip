@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../include/gocean.h"
+#include "cdklm16_march.hpp"
 #include "cdklm16_tile.hpp"
 #include "gocean_internal.hpp"
 
@@ -90,6 +91,11 @@ const TileConfig kTileConfigs[] = {
     // 14 rows: the slope and face phases fit two passes of 512 threads
     // (16 rows need a third pass for 96 / 48 threads)
     {32, 14, 512, "32x14/512"},
+    // column-marching strips (zero rotation): 60 output columns per
+    // wavefront, BH rows; four independent wavefronts per workgroup
+    {60, 8, 256, "march60x8/256"},
+    {60, 16, 256, "march60x16/256"},
+    {60, 32, 256, "march60x32/256"},
 };
 const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 const int kDefaultTileConfig = 1;
@@ -106,6 +112,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
         cdklm_tile<BW, BH, NT, MODE, false, false>(A, tx, ty, S);
     else
         cdklm_tile<BW, BH, NT, MODE>(A, tx, ty, S);
+}
+
+// column-marching substep: wavefront w of the grid takes strip (sx, sy),
+// sx fastest, workgroups banded over the XCDs
+template <int BH>
+__global__ __launch_bounds__(256) void k_march(SubstepArgs A) {
+    const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
+    if (wave >= A.tiles_x * A.tiles_y) return;
+    const int sy = wave / A.tiles_x, sx = wave - sy * A.tiles_x;
+    march::march_strip<BH>(A, sx, sy);
+}
+
+template <int BH>
+static void launch_march_t(SubstepArgs A, hipStream_t s) {
+    A.tiles_x = (A.nx + march::kOut - 1) / march::kOut;
+    A.tiles_y = (A.ny + BH - 1) / BH;
+    const int waves = A.tiles_x * A.tiles_y;
+    hipLaunchKernelGGL((k_march<BH>), dim3((waves + 3) / 4), dim3(256), 0, s, A);
 }
 
 template <int BW, int BH, int NT, int MODE>
@@ -176,6 +200,9 @@ int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s) {
             case 4: launch_substep_t<64, 4, 256, kModeZero>(A, s); break;
             case 5: launch_substep_t<32, 16, 512, kModeZero>(A, s); break;
             case 6: launch_substep_t<32, 14, 512, kModeZero>(A, s); break;
+            case 7: launch_march_t<8>(A, s); break;
+            case 8: launch_march_t<16>(A, s); break;
+            case 9: launch_march_t<32>(A, s); break;
         }
     }
     return check_launch("cdklm substep launch");
@@ -291,11 +318,10 @@ __global__ __launch_bounds__(kDtThreads) void k_dt_partial(const float* eta, con
                                                            float g, float land, float* partial) {
     __shared__ float red[kDtThreads / 64];
     float m = 1.0e5f;   // FLT_MAX of max_dt.cu:29
-    const long long n = (long long)nx * ny;
     const float qdx = 0.25f * dx, qdy = 0.25f * dy;
-    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (long long)gridDim.x * blockDim.x) {
-        const int j = (int)(k / nx) + 2, i = (int)(k % nx) + 2;
+    // one interior row per workgroup at a time, columns across its threads
+    for (int j = blockIdx.x + 2; j < ny + 2; j += gridDim.x)
+    for (int i = threadIdx.x + 2; i < nx + 2; i += blockDim.x) {
         const size_t o = (size_t)j * pitch + i;
         const float H = Hm[o];
         const float h = eta[o] + H;
@@ -326,9 +352,7 @@ __global__ __launch_bounds__(kDtThreads) void k_dt_final(const float* partial, i
 
 int max_dt(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
            const float* Hm, float courant, float* out, hipStream_t s) {
-    const long long n = (long long)d->nx * d->ny;
-    int nb = (int)((n + kDtThreads * 8 - 1) / (kDtThreads * 8));
-    nb = nb < 1 ? 1 : (nb > kDtBlocks ? kDtBlocks : nb);
+    const int nb = d->ny < kDtBlocks ? d->ny : kDtBlocks;
     hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kDtThreads), 0, s, eta, hu, hv, Hm, d->nx,
                        d->ny, d->pitch, d->dx, d->dy, d->g, d->land_value, out + 2);
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kDtThreads), 0, s, out + 2, nb, courant, out);
@@ -439,6 +463,15 @@ int gocean_cdklm_substep(const gocean_domain_t* d, float dt, const float* dt_dev
     if (wind) {
         A.wind_x = wind_lerp(wind->t, wind->x0, wind->x1);
         A.wind_y = wind_lerp(wind->t, wind->y0, wind->y1);
+        if (wind->grid) {
+            if (wind->grid_nx < 1 || wind->grid_ny < 1)
+                return set_error(GOCEAN_EINVAL, "wind grid %d x %d: both sides must be >= 1",
+                                 wind->grid_nx, wind->grid_ny);
+            A.wind_grid = wind->grid;
+            A.wind_wx = wind->grid_nx;
+            A.wind_wy = wind->grid_ny;
+            A.wind_t = wind->t;
+        }
     }
     return launch_substep(A, tile_config, field, (hipStream_t)stream);
 }

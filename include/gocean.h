@@ -53,11 +53,23 @@ typedef struct gocean_domain {
     int32_t rk_order;        /* 1, 2 or 3 */
 } gocean_domain_t;
 
-/* Uniform wind stress (tau/rho_water, m^2/s^2) at the two bracketing times
- * and the interpolation weight t in [0,1] (Simulator.py:216-223,
- * gpu_kernels/common.cu:616-650 with 1x1 textures). */
+/* Wind stress (tau/rho_water, m^2/s^2) at the two bracketing times of its
+ * time series and the interpolation weight t in [0,1] (Simulator.py:145-223,
+ * gpu_kernels/common.cu:616-650).
+ *   grid == NULL: uniform wind x0, x1 (current, next) and y0, y1 -- the
+ *                 reference's 1x1 textures;
+ *   grid != NULL: gridded wind, four device planes of grid_ny x grid_nx
+ *                 floats back to back: X current, X next, Y current, Y next,
+ *                 read at each cell like the reference's bilinear textures
+ *                 (normalised coordinates (ti+0.5)/nx with the
+ *                 ghost-inclusive cell index ti, clamped, linear filter;
+ *                 CDKLM16_kernel.cu:827-828).  Only the plain substep and the
+ *                 step loop take a grid; ensembles, drifters and the adjoint
+ *                 reject it. */
 typedef struct gocean_wind {
     float t, x0, x1, y0, y1;
+    const float* grid;
+    int32_t grid_nx, grid_ny;
 } gocean_wind_t;
 
 /* Optional per-cell fields, (ny+4) x pitch floats each; NULL selects the

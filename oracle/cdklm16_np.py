@@ -80,6 +80,10 @@ class Params:
     # uniform wind stress (already divided by rho_water), current and next
     wind_x: Tuple[float, float] = (0.0, 0.0)
     wind_y: Tuple[float, float] = (0.0, 0.0)
+    # gridded wind stress: (X_current, X_next, Y_current, Y_next), each a
+    # (wy, wx) float32 field read as the reference's bilinear textures; None
+    # for the uniform fields above
+    wind_grid: object = None
     land_value: float = LAND_VALUE
 
     @property
@@ -154,6 +158,41 @@ def minmod_slope(left, center, right, C):
     central = (right - left) * C.HALF
     forward = (right - center) * C.THETA
     return minmod_raw(backward, central, forward, C)
+
+
+def wind_texture(field, nx, ny, t=np.float32):
+    """A wind-stress field (wy, wx) read at every interior cell as the
+    reference's texture lookup does (common.cu:616-650 called from
+    CDKLM16_kernel.cu:827-828; texture set up by Simulator.update_wind_stress,
+    Simulator.py:186-195): normalised coordinates s = (ti + 0.5) / nx with the
+    ghost-inclusive cell index ti = i + 2, clamped addressing, bilinear
+    filtering with texel centres at (k + 0.5) / wx.  The weights are exact
+    here (the texture unit quantises them to 8 fractional bits: parity with
+    the reference at that level is not pinned).  Returns (ny, nx)."""
+    F = np.asarray(field, dtype=t)
+    wy, wx = F.shape
+
+    def axis(n, w):
+        ti = np.arange(2, n + 2)
+        s = (ti.astype(t) + t(0.5)) / t(n)
+        x = s * t(w) - t(0.5)
+        x0 = np.floor(x)
+        a = x - x0
+        i0 = np.clip(x0.astype(np.int64), 0, w - 1)
+        i1 = np.clip(x0.astype(np.int64) + 1, 0, w - 1)
+        return i0, i1, a.astype(t)
+
+    i0, i1, a = axis(nx, wx)
+    j0, j1, b = axis(ny, wy)
+    one = t(1.0)
+    a = a[None, :]
+    b = b[:, None]
+    v00 = F[np.ix_(j0, i0)]
+    v01 = F[np.ix_(j0, i1)]
+    v10 = F[np.ix_(j1, i0)]
+    v11 = F[np.ix_(j1, i1)]
+    return ((((one - a) * (one - b)) * v00 + (a * (one - b)) * v01)
+            + ((one - a) * b) * v10) + (a * b) * v11
 
 
 def desingularize(h, hu, C):
@@ -504,8 +543,13 @@ def cdklm_substep(dt, rk_step, eta_in, hu_in, hv_in, eta_out, hu_out, hv_out,
     n_x = nox[J, I]; n_y = noy[J, I]
     e_x = n_y; e_y = -n_x
     wt = t(wind_t)
-    X = wt * t(p.wind_x[1]) + (C.ONE - wt) * t(p.wind_x[0])
-    Y = wt * t(p.wind_y[1]) + (C.ONE - wt) * t(p.wind_y[0])
+    if p.wind_grid is None:
+        X = wt * t(p.wind_x[1]) + (C.ONE - wt) * t(p.wind_x[0])
+        Y = wt * t(p.wind_y[1]) + (C.ONE - wt) * t(p.wind_y[0])
+    else:
+        X0, X1, Y0, Y1 = (wind_texture(a, nx, ny, t) for a in p.wind_grid)
+        X = wt * X1 + (C.ONE - wt) * X0
+        Y = wt * Y1 + (C.ONE - wt) * Y0
     RHxp = C.HALF * (Hif[2:ny + 2, 3:nx + 3] + Hif[3:ny + 3, 3:nx + 3])
     RHxm = C.HALF * (Hif[2:ny + 2, 2:nx + 2] + Hif[3:ny + 3, 2:nx + 2])
     RHyp = C.HALF * (Hif[3:ny + 3, 2:nx + 2] + Hif[3:ny + 3, 3:nx + 3])

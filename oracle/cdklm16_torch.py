@@ -72,12 +72,13 @@ def bc_view(eta, hu, hv, p: Params):
     (periodic) images of interior cells, as the forward kernel reads them;
     differentiable (built by indexing, not in-place writes)."""
     nx, ny = p.nx, p.ny
-    rows = torch.arange(ny + 4)
-    cols = torch.arange(nx + 4)
+    dev = eta.device
+    rows = torch.arange(ny + 4, device=dev)
+    cols = torch.arange(nx + 4, device=dev)
     sr = rows.clone()
     sc = cols.clone()
-    sgn_v = torch.ones(ny + 4, dtype=eta.dtype)
-    sgn_u = torch.ones(nx + 4, dtype=eta.dtype)
+    sgn_v = torch.ones(ny + 4, dtype=eta.dtype, device=dev)
+    sgn_u = torch.ones(nx + 4, dtype=eta.dtype, device=dev)
     if p.bc_south == WALL:
         sr[0], sr[1] = 3, 2
         sgn_v[0] = sgn_v[1] = -1
@@ -141,7 +142,7 @@ def substep(dt, rk_step, eta_in, hu_in, hv_in, eta_qn, hu_qn, hv_qn, Hi, Hm, p: 
     Xc, Xl, Xr = slice(1, nx + 3), slice(0, nx + 2), slice(2, nx + 4)
     ux = minmod_slope(U[J, Xl], U[J, Xc], U[J, Xr], theta)
     vx = minmod_slope(V[J, Xl], V[J, Xc], V[J, Xr], theta)
-    flipc = torch.ones(nx + 4, dtype=dtp)
+    flipc = torch.ones(nx + 4, dtype=dtp, device=eta_in.device)
     if p.bc_west == WALL:
         flipc[0:2] = -1
     if p.bc_east == WALL:
@@ -187,7 +188,7 @@ def substep(dt, rk_step, eta_in, hu_in, hv_in, eta_qn, hu_qn, hv_qn, Hi, Hm, p: 
         dryf = (eta_p == DRY_FLAG) | (eta_m == DRY_FLAG)
         return tuple(torch.where(dryf, torch.zeros_like(x), x) for x in F)
 
-    cols = torch.arange(2, nx + 2)
+    cols = torch.arange(2, nx + 2, device=eta_in.device)
     Fe = fface(cols, cols)
     Fw = fface(cols - 1, cols)
     fdx = [(Fe[k] - Fw[k]) / dx for k in range(3)]
@@ -196,7 +197,7 @@ def substep(dt, rk_step, eta_in, hu_in, hv_in, eta_qn, hu_qn, hv_qn, Hi, Hm, p: 
     Yc, Yl, Yu = slice(1, ny + 3), slice(0, ny + 2), slice(2, ny + 4)
     uy = minmod_slope(U[Yl, I], U[Yc, I], U[Yu, I], theta)
     vy = minmod_slope(V[Yl, I], V[Yc, I], V[Yu, I], theta)
-    flipr = torch.ones(ny + 4, dtype=dtp)
+    flipr = torch.ones(ny + 4, dtype=dtp, device=eta_in.device)
     if p.bc_south == WALL:
         flipr[0:2] = -1
     if p.bc_north == WALL:
@@ -243,7 +244,7 @@ def substep(dt, rk_step, eta_in, hu_in, hv_in, eta_qn, hu_qn, hv_qn, Hi, Hm, p: 
         z = torch.zeros_like(a)
         return torch.where(dryf, z, a), torch.where(dryf, z, c), torch.where(dryf, z, b)
 
-    rows = torch.arange(2, ny + 2)
+    rows = torch.arange(2, ny + 2, device=eta_in.device)
     Gn = gface(rows, rows)
     Gs = gface(rows - 1, rows)
     fd = [fdx[k] + (Gn[k] - Gs[k]) / dy for k in range(3)]

@@ -551,3 +551,97 @@ def test_record_tile_configs_record_the_same_gradient():
             for x, y in zip(ref, got):
                 assert np.array_equal(x, y), (cfg, keep)
     G.check(G.load().gocean_adjoint_set_record_tile_config(-1), "reset")
+
+
+@pytest.mark.parametrize("bcname,bc", [BCS[0], BCS[1], BCS[3]])
+@pytest.mark.parametrize("kind", ["plain", "land"])
+def test_recording_forward_interior_tiles(bcname, bc, kind):
+    """A domain large enough (112 x 70) that most tiles of every recording
+    tile configuration are interior tiles (every access at least one cell
+    inside the interior), so the recording forward's interior body runs:
+    its output equals the plain forward bit for bit, kept records equal
+    recomputed ones, and the gradient equals the float64 reverse pass along
+    the float32 trajectory."""
+    from SWESimulators import autotune
+    _say("recording forward interior tiles", bcname, kind)
+    Common, G, A = _mods()
+    nx, ny, dt, n = 112, 70, 3.0, 5
+    p, Hi, eta, hu, hv = _case(nx, ny, bc, 17, kind)
+    W = _weights(np.random.default_rng(19), nx, ny)
+    d = _domain(p)
+    lib = G.load()
+    # the plain forward (k_substep) for the same steps
+    q = [_dev(a) for a in (eta, hu, hv)]
+    tmp = [x.clone() for x in q]
+    G.check(lib.gocean_rk_steps(d, n, dt, None, *(G.ptr(x) for x in q), *(G.ptr(x) for x in tmp),
+                                None, None, None, G.ptr(_dev(Hi)), G.Fields(), G.Wind(), -1, None), "rk")
+    torch.cuda.synchronize()
+    plain = [x.cpu().numpy() for x in q]
+    ref = None
+    for cfg in (1, 5, 6):
+        for keep in (True, False):
+            adj = A.RK2Adjoint(d, _dev(Hi), checkpoint_every=2, keep_records=keep, record_tile_config=cfg)
+            qd = [_dev(a).requires_grad_(True) for a in (eta, hu, hv)]
+            out = A.CDKLM16RK2.apply(*qd, adj, n, dt)
+            sum((_dev(w) * o).sum() for w, o in zip(W, out)).backward()
+            torch.cuda.synchronize()
+            for a, b in zip(out, plain):
+                assert np.array_equal(a.detach().cpu().numpy(), b), (cfg, keep)
+            got = [x.grad.cpu().numpy().astype(np.float64) for x in qd]
+            if ref is None:
+                ref = got
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b), (cfg, keep)
+    G.check(lib.gocean_adjoint_set_record_tile_config(-1), "reset")
+    traj, _ = _trajectory(p, Hi, (eta, hu, hv), dt, n)
+    want = _chain_grad(p, Hi, traj, W, dt)
+    for name, g, r in zip(("eta", "hu", "hv"), ref, want):
+        e = _rel(g[2:-2, 2:-2], r[2:-2, 2:-2])
+        _say("  ", name, "rel L2", e)
+        assert e <= 1e-5, (name, e)
+
+
+def test_ensemble_adjoint_interior_tiles_match_single_domain():
+    """Ensemble members of 100 x 60 and 80 x 40 (interior tiles in the
+    batched recording forward and reverse kernels) get exactly the
+    single-domain gradient, with the records kept and recomputed."""
+    _say("ensemble adjoint, members with interior tiles")
+    from SWESimulators import Common, CDKLM16Ensemble
+    from test_gpu_ensemble import members
+    Common_, G, A = _mods()
+    ms = members(seed=9, sizes=((100, 60), (80, 40), (70, 70)))
+    ctx = Common.HIPContext(0)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, rk_order=2, courant_number=0.6)
+    n, K = 6, 4
+    state0 = ens.state[0].clone()
+    rng = np.random.default_rng(12)
+    lam = torch.zeros_like(state0)
+    for k in range(ens.n):
+        o, pl = int(ens.state_off[k]), ens.planes[k]
+        for f in range(3):
+            v = lam[o + f * pl:o + (f + 1) * pl].view(ens.ny[k] + 4, ens.pitch[k])
+            v[2:-2, 2:-2] = torch.from_numpy(rng.standard_normal((ens.ny[k], ens.nx[k])).astype(np.float32))
+    dts = ens.dt_host()
+    for keep in (True, False):
+        ea = A.EnsembleAdjoint(ens, checkpoint_every=K, keep_records=keep)
+        out = ea.forward(state0, n)
+        grad = ea.backward(lam)
+        torch.cuda.synchronize()
+        for k, m in enumerate(ms):
+            nx, ny = ens.nx[k], ens.ny[k]
+            d = Common.make_domain(nx, ny, m["dx"], m["dy"], m["g"], 0.0, 1.3, 2, m["bc"], pitch=ens.pitch[k],
+                                   hi_pitch=nx + 5, f=m["f"])
+            Hi_k = ens.Hi[int(ens.hi_off[k]):int(ens.hi_off[k + 1])].view(ny + 5, nx + 5)
+            w = G.Wind()
+            w.x0 = w.x1 = float(np.float32(m["wind_x"]))
+            w.y0 = w.y1 = float(np.float32(m["wind_y"]))
+            adj = A.RK2Adjoint(d, Hi_k, wind=w, checkpoint_every=K, keep_records=keep)
+            o, pl = int(ens.state_off[k]), ens.planes[k]
+            view = lambda t: tuple(t[o + f * pl:o + (f + 1) * pl].view(ny + 4, ens.pitch[k]) for f in range(3))
+            fo = adj.forward(view(state0), n, float(dts[k]))
+            gk = adj.backward(view(lam))
+            torch.cuda.synchronize()
+            for a, b in zip(fo, view(out)):
+                assert torch.equal(a, b), ("forward", k, keep)
+            for a, b in zip(gk, view(grad)):
+                assert torch.equal(a, b), ("gradient", k, keep)

@@ -264,3 +264,29 @@ def test_reference_getters_and_innovations():
     w = ens.getGaussianWeight(inn, np.eye(2))
     assert abs(w.sum() - 1) < 1e-12
     ens.cleanUp()
+
+
+def test_rk_order_1_ensemble_steps_in_pairs():
+    """rk_order 1 advances in pairs of Euler steps; single steps (step_to,
+    dataAssimilationStep, odd step_n, per-step model error) raise a clear
+    error instead of failing inside the library or leaving stale buffers."""
+    from SWESimulators import Common, CDKLM16Ensemble
+    _say("rk_order 1 ensemble")
+    ms = members()[:2]
+    ctx = Common.HIPContext(0)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, rk_order=1, courant_number=0.7)
+    with pytest.raises(ValueError):
+        ens.step_n(3)
+    with pytest.raises(ValueError):
+        ens.step_n(2, model_error=True)
+    with pytest.raises(NotImplementedError):
+        ens.step_to(5.0)
+    with pytest.raises(NotImplementedError):
+        ens.dataAssimilationStep(20.0, 10.0)
+    ens.step_n(2)
+    sim, p = oracle_member(ms[0], 1)
+    dts = ens.dt_host()
+    sim.step_n(2, dts[0])
+    for g, r in zip(ens.download(0), sim.h0):
+        assert np.array_equal(g[2:-2, 2:-2], r[2:-2, 2:-2])
+    ens.cleanUp()

@@ -80,12 +80,17 @@ FEATURES = [
 ]
 
 
+# tile configurations: the default 2-D tile and the column-marching strips
+TILES = [-1, 8]
+
+
 @pytest.mark.parametrize("bcname,bc", BCS)
 @pytest.mark.parametrize("feat", range(len(FEATURES)))
 @pytest.mark.parametrize("rk", [0, 1])
-def test_substep_bitexact(bcname, bc, feat, rk):
+@pytest.mark.parametrize("tile", TILES)
+def test_substep_bitexact(bcname, bc, feat, rk, tile):
     kw = FEATURES[feat]
-    _say("substep", bcname, kw, "rk", rk)
+    _say("substep", bcname, kw, "rk", rk, "tile", tile)
     inp, p, Hif, Hm = small_case(53, 41, bc, 10 + feat, **kw)
     D = H.DeviceDomain(inp, pitch_extra=5 if feat % 2 else 0)
     D.bathymetry_init()
@@ -95,7 +100,7 @@ def test_substep_bitexact(bcname, bc, feat, rk):
     dst = D.state(*qn_host)
     O.periodic_bc(inp.eta, inp.hu, inp.hv, p)
     src = D.state(inp.eta, inp.hu, inp.hv)
-    D.substep(0.7, rk, src, dst)
+    D.substep(0.7, rk, src, dst, tile=tile)
     got = D.host(dst)
     ref = O.cdklm_substep(0.7, rk, inp.eta, inp.hu, inp.hv, *qn_host, Hif, Hm, p)
     for g, r_, name in zip(got, ref, ("eta", "hu", "hv")):
@@ -104,7 +109,7 @@ def test_substep_bitexact(bcname, bc, feat, rk):
         assert nbad == 0, f"{name}: {nbad} cells differ, max |d| = {np.max(np.abs(gi - ri))}"
 
 
-@pytest.mark.parametrize("tile", range(7))
+@pytest.mark.parametrize("tile", range(10))
 @pytest.mark.parametrize("bcname,bc", [BCS[0], BCS[1], BCS[3]])
 def test_tile_configs_bitexact(tile, bcname, bc):
     _say("tile config", tile, bcname)
@@ -147,10 +152,11 @@ def test_field_mode_bitexact(bcname, bc):
 
 @pytest.mark.parametrize("bcname,bc", BCS)
 @pytest.mark.parametrize("rk_order", [1, 2, 3])
-def test_rk_steps_full_arrays(bcname, bc, rk_order):
+@pytest.mark.parametrize("tile", TILES)
+def test_rk_steps_full_arrays(bcname, bc, rk_order, tile):
     """Whole padded arrays (ghost cells included) after several RK steps
     against the oracle host loop."""
-    _say("rk_steps", bcname, "order", rk_order)
+    _say("rk_steps", bcname, "order", rk_order, "tile", tile)
     inp, p, Hif, Hm = small_case(40, 33, bc, 21, wind=(1e-4, 5e-5))
     p.rk_order = rk_order
     sim = O.OracleSim(inp.eta, inp.hu, inp.hv, inp.Hi, p, dt=0.8)
@@ -163,7 +169,7 @@ def test_rk_steps_full_arrays(bcname, bc, rk_order):
     n = 6
     for _ in range(n):
         sim.step(np.float32(0.8))
-    D.rk_steps(n, np.float32(0.8), h0, h1, h2)
+    D.rk_steps(n, np.float32(0.8), h0, h1, h2, tile=tile)
     got = D.host(h0)
     for g, r_, name in zip(got, sim.h0, ("eta", "hu", "hv")):
         if bc == (1, 1, 1, 1) or rk_order != 2:
