@@ -704,18 +704,38 @@ def bench_iewpf(ctx, members=100, nx=500, ny=300, factor=5, n_drifters=64, reps=
     return out
 
 
-def measured_traffic(cells):
-    """HBM bytes per substep launch from the committed rocprofv3 counter
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the calibration copy), when
-    one exists for this domain size; else None."""
-    p = os.path.join(ROOT, "profiles", "cdklm16_substep_2048_traffic.json")
+def _profile_json():
+    p = os.path.join(ROOT, "profiles", "cdklm16_substep_2048_counters.json")
     try:
-        d = json.load(open(p))
+        return json.load(open(p))
     except (OSError, ValueError):
-        return None
-    if d.get("cells") != cells:
+        return {}
+
+
+def measured_traffic(cells, tile):
+    """HBM bytes per substep launch of tile configuration ``tile`` at this
+    domain size, from the committed rocprofv3 counter summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by what the counter reads for a plain copy of
+    known size); None when no summary covers this tile and size."""
+    d = _profile_json().get("tiles", {}).get(tile)
+    if not d or d.get("cells") != cells:
         return None
     return d.get("bytes_per_launch")
+
+
+def measured_valu(cells, tile, launch_s):
+    """VALU work of the substep launch from the committed counter summary
+    (SQ_INSTS_VALU per dispatch) and its issue rate against the non-packed
+    FP32 limit: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave instruction
+    = 614.4 G wave instructions/s (packed FP32 is off in this build)."""
+    d = _profile_json().get("tiles", {}).get(tile)
+    if not d or d.get("cells") != cells or "valu_per_launch" not in d:
+        return None
+    wi = d["valu_per_launch"]
+    return {"valu_wave_instructions_per_launch": wi,
+            "valu_lane_ops_per_cell": round(wi * 64 / cells, 1),
+            "issue_rate_g_per_s": round(wi / launch_s / 1e9, 1),
+            "issue_frac_of_614g": round(wi / launch_s / 614.4e9, 3)}
 
 
 def main():
@@ -774,10 +794,13 @@ def main():
     cells = inp.nx * inp.ny
     value = cells * a.steps / wall / 1e6
     avg_launch = ev / launches
+    from SWESimulators import autotune
+    tile_name = autotune.tile_config_names()[sim.tile_config if sim.tile_config >= 0 else 1]
     achieved = 0.5 * BYTES_PER_CELL_STEP * cells / avg_launch / 1e9
     log(f"[bench] plain: {value:.1f} Mcell-steps/s, event time/launch {avg_launch*1e6:.1f} us")
 
-    extra = {}
+    extra = {"substep_launch_us": round(avg_launch * 1e6, 2),
+             "valu": measured_valu(cells, tile_name, avg_launch)}
     if not a.no_ensemble:
         log("[bench] batched ensemble C3")
         extra["ensemble"] = bench_ensemble(ctx)
@@ -846,10 +869,10 @@ def main():
         "config": {"workload": f"CDKLM16 plain {inp.nx}x{inp.ny} periodic-EW walls-NS, "
                                f"dx=dy=500 m, fixed dt (CFL 0.7), SSP-RK2",
                    "nx": inp.nx, "ny": inp.ny, "cells": cells, "rk_order": 2,
-                   "tile_config": sim.tile_config},
+                   "tile_config": tile_name},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(cells)},
+                     "traffic": measured_traffic(cells, tile_name)},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -8,7 +8,7 @@
 // the two kernels give the same bits; what differs is how work is laid out:
 //
 //   * one wavefront owns a strip of 64 padded columns (lane = column) and
-//     walks up BH output rows; lanes 2..61 write output, lanes 0, 1, 62, 63
+//     walks up bh output rows; lanes 2..61 write output, lanes 0, 1, 62, 63
 //     are the strip's x halo;
 //   * every per-row quantity is formed once per lane and kept in registers
 //     while it is needed: the converted (eta, u, v) of the last three rows,
@@ -27,8 +27,9 @@
 //
 // Work per output cell: the tile kernel converts (BH+4)(BW+4)-4*4 cells for
 // BW*BH outputs and forms every x and y quantity on its halo; here every
-// quantity is formed 64/60 times per output plus (BH+4)/BH for the y halo
-// of the conversion only.
+// quantity is formed 64/60 times per output, plus a prologue of four
+// conversions, two y slopes and a face per strip of bh rows.  The host picks
+// bh so that the strips fill whole rounds of resident waves (march_rows).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,14 +54,20 @@ __device__ __forceinline__ float lane_get(float v, int addr) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
 }
 
-// element at byte offset `off` of a row that starts at the wave-uniform
-// address `row`: the load / store then takes the row from scalar registers
-// and only the lane's 32-bit column offset from a vector register
-__device__ __forceinline__ float ld(const float* row, uint32_t off) {
-    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(row) + off);
+// Each plane is addressed as a wave-uniform row address (scalar registers:
+// the plane's base plus the row's byte offset) plus the lane's column byte
+// offset (a vector register fixed for the strip), so a row's loads and
+// stores take no vector address arithmetic.
+using rsrc_t = const float*;
+
+__device__ __forceinline__ rsrc_t plane(const float* p) { return p; }
+__device__ __forceinline__ float ld(rsrc_t r, uint32_t lane_off, uint32_t row_off) {
+    const char* row = reinterpret_cast<const char*>(r) + row_off;
+    return *reinterpret_cast<const float*>(row + lane_off);
 }
-__device__ __forceinline__ void st(float* row, uint32_t off, float v) {
-    *reinterpret_cast<float*>(reinterpret_cast<char*>(row) + off) = v;
+__device__ __forceinline__ void st(rsrc_t r, uint32_t lane_off, uint32_t row_off, float v) {
+    char* row = const_cast<char*>(reinterpret_cast<const char*>(r)) + row_off;
+    *reinterpret_cast<float*>(row + lane_off) = v;
 }
 
 // corner depth with land flagged (CDKLM16_kernel.cu:509-525)
@@ -83,70 +90,91 @@ __device__ __forceinline__ Cell convert(float eta, float hu, float hv, float ab,
     o.Hm4 = sum * 0.25f;
     float Hm = o.Hm4;
     const bool any_dry = (a == kDryFlag) | (b == kDryFlag) | (c == kDryFlag) | (d == kDryFlag);
+    const float h = eta + Hm;
+    // every lane wet and deep: the plain branch alone
+    if (__all(!any_dry && !(fabsf(Hm - K.land) <= kDryEps) && !(h < K.desing))) {
+        o.E = eta;
+        const float rh = rcp_refined(h);
+        o.U = div_r(hu, h, rh);
+        o.V = div_r(hv, h, rh);
+        return o;
+    }
     if (any_dry) {
         const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag)
                         - (float)(c == kDryFlag) - (float)(d == kDryFlag);
         Hm = (wet == 0.0f) ? K.land : sum / wet;
     }
-    const float h = eta + Hm;
+    const float hh = eta + Hm;
     if (fabsf(Hm - K.land) <= kDryEps) {
         o.E = kDryFlag; o.U = 0.0f; o.V = 0.0f;
-    } else if (h < K.desing) {
-        if (h <= K.cutoff) {
+    } else if (hh < K.desing) {
+        if (hh <= K.cutoff) {
             o.E = -Hm + K.cutoff; o.U = 0.0f; o.V = 0.0f;
         } else {
             o.E = eta;
-            o.U = desingularize(h, hu, K.desing);
-            o.V = desingularize(h, hv, K.desing);
+            o.U = desingularize(hh, hu, K.desing);
+            o.V = desingularize(hh, hv, K.desing);
         }
     } else {
         o.E = eta;
-        const float rh = rcp_refined(h);
-        o.U = div_r(hu, h, rh);
-        o.V = div_r(hv, h, rh);
+        const float rh = rcp_refined(hh);
+        o.U = div_r(hu, hh, rh);
+        o.V = div_r(hv, hh, rh);
     }
     return o;
 }
 
+// The backward candidates of a cell's three minmod slopes equal, bit for
+// bit, the forward candidates of its low-side neighbour: (c - l) * theta of
+// the cell is (r - c) * theta of the neighbour, and the geostrophic candidate
+// theta g ((e_c - e_l) -+ d/(2g) (f_c + f_l)) is the neighbour's forward one
+// up to the order of the commutative sum f_c + f_l.  So each slope centre
+// takes them from its neighbour (the previous row in y, the left lane in x)
+// and hands its own forward ones on.
+struct Carry {
+    float a;     // (a_c - a_l) * theta
+    float b;     // (b_c - b_l) * theta
+    float K;     // geostrophic backward candidate
+    float fb;    // f * b_l (with b_l's wall flip)
+};
+
 // x or y slope centre of one cell (x: :607-666 with adjustSlopes_x :148-191;
-// y: :711-770 with adjustSlopes_y :198-241), zero-rotation form: returns the
-// minmod slopes of the two velocities and T = (K + d f v)/2g for the face
-// reconstruction.  l, c, r: the cell and its two neighbours along the
-// direction; `a*` the velocity along the direction's minmod only, `b*` the
-// velocity entering the geostrophic term (v for x, u for y); Hlo / Hhi the
-// face-averaged depths on the two sides; SGN +1 for y, -1 for x (the sign of
-// the geostrophic term); fl* the wall flips of the three cells; fa the
-// adjustment's own flip.
+// y: :711-770 with adjustSlopes_y :198-241), zero-rotation form.  l, c, r:
+// the cell and its two neighbours along the direction; `a` the velocity
+// along the direction (minmod only), `b` the velocity in the geostrophic
+// term (v for x, u for y); Hlo / Hhi the face-averaged depths on the two
+// sides; SGN -1 for x, +1 for y (the sign of the geostrophic term); flc /
+// flr the wall flips of the cell and its high neighbour; fa the
+// adjustment's own flip.  `in`: the backward candidates (from the low
+// neighbour); `out`: this cell's forward candidates.  Returns the minmod
+// slopes Sa, Sb and T = (K + d f v)/2g (x) or (L - d f u)/2g (y).
 template <int SGN>
-__device__ __forceinline__ void slope_centre(float el, float ec, float er,
-                                             float al, float ac, float ar,
-                                             float bl, float bc, float br,
-                                             float Hlo, float Hhi,
-                                             bool fll, bool flc, bool flr, bool fa,
-                                             float dd, float vd, const Consts& K, float fc,
-                                             float& Sa, float& Sb, float& T) {
-    Sa = minmod_slope(al, ac, ar, K.theta);
-    Sb = minmod_slope(bl, bc, br, K.theta);
-    const float b_raw = bc;
-    if (fll) bl = -bl;
-    if (flc) bc = -bc;
-    if (flr) br = -br;
-    const float fl = bl * fc, fcc = bc * fc, fr = br * fc;
-    float bw, ce, fw;
-    if constexpr (SGN < 0) {
-        bw = K.theta_g * ((ec - el) - vd * (fcc + fl));
-        ce = K.half_g * ((er - el) - vd * ((fr + 2.0f * fcc) + fl));
-        fw = K.theta_g * ((er - ec) - vd * (fcc + fr));
-    } else {
-        bw = K.theta_g * ((ec - el) + vd * (fcc + fl));
-        ce = K.half_g * ((er - el) + vd * ((fr + 2.0f * fcc) + fl));
-        fw = K.theta_g * ((er - ec) + vd * (fcc + fr));
-    }
-    float Kv = minmod_raw(bw, ce, fw);
+__device__ __forceinline__ void slope_fwd(float ec, float er, float ac, float ar, float bc, float br,
+                                          bool flc, bool flr, float vd, const Consts& K, float fc,
+                                          Carry& out, float& fr) {
+    out.a = (ar - ac) * K.theta;
+    out.b = (br - bc) * K.theta;
+    out.fb = (flc ? -bc : bc) * fc;
+    fr = (flr ? -br : br) * fc;
+    if constexpr (SGN < 0) out.K = K.theta_g * ((er - ec) - vd * (out.fb + fr));
+    else out.K = K.theta_g * ((er - ec) + vd * (out.fb + fr));
+}
+
+template <int SGN, bool ONE>
+__device__ __forceinline__ void slope_rest(float el, float ec, float er, float al, float ar, float bl, float bc,
+                                           float br, float Hlo, float Hhi, bool flc, bool fa, float dd, float vd,
+                                           const Consts& K, float fc, const Carry& in, const Carry& out, float fr,
+                                           float& Sa, float& Sb, float& T, float& dlo, float& dhi) {
+    Sa = minmod_raw(in.a, (ar - al) * 0.5f, out.a);
+    Sb = minmod_raw(in.b, (br - bl) * 0.5f, out.b);
+    const float fl = in.fb, fcc = out.fb;
+    float ce;
+    if constexpr (SGN < 0) ce = K.half_g * ((er - el) - vd * ((fr + 2.0f * fcc) + fl));
+    else ce = K.half_g * ((er - el) + vd * ((fr + 2.0f * fcc) + fl));
+    float Kv = minmod_raw(in.K, ce, out.K);
     const float dfc = dd * fc;
-    const float dfb = dfc * (fa ? -b_raw : b_raw);
-    // x: t = (K + d f v)/2g; y: t = (L - d f u)/2g
-    const float t = (SGN < 0) ? div_r(Kv + dfb, K.two_g, K.r_two_g) : div_r(Kv - dfb, K.two_g, K.r_two_g);
+    const float dfb = dfc * (fa ? -bc : bc);
+    const float t = (SGN < 0) ? div_k<ONE>(Kv + dfb, K.two_g, K.r_two_g) : div_k<ONE>(Kv - dfb, K.two_g, K.r_two_g);
     const float hlo = (ec + Hlo) - t;
     const float hhi = (ec + Hhi) + t;
     bool adj = false;
@@ -157,44 +185,89 @@ __device__ __forceinline__ void slope_centre(float el, float ec, float er,
         if (!(hlo > 0.0f)) { Kv = dfb + K.two_g * (ec + Hlo); adj = true; }
         if (!(hhi > 0.0f)) { Kv = dfb - K.two_g * (ec + Hhi); adj = true; }
     }
+    // T and the depths of the cell's two faces as the face fluxes form them,
+    // (e + H_face) -/+ T: with T == t they are hlo / hhi above
     T = t;
+    dlo = hlo;
+    dhi = hhi;
     if (adj || flc != fa) {
-        const float db = dfc * (flc ? -b_raw : b_raw);
-        T = (SGN < 0) ? div_r(Kv + db, K.two_g, K.r_two_g) : div_r(Kv - db, K.two_g, K.r_two_g);
+        const float db = dfc * (flc ? -bc : bc);
+        T = (SGN < 0) ? div_k<ONE>(Kv + db, K.two_g, K.r_two_g) : div_k<ONE>(Kv - db, K.two_g, K.r_two_g);
+        dlo = (ec + Hlo) - T;
+        dhi = (ec + Hhi) + T;
     }
+}
+
+// the backward candidates of a cell computed directly (the first y slope of
+// a strip has no previous row)
+template <int SGN>
+__device__ __forceinline__ Carry slope_bwd(float el, float ec, float al, float ac, float bl, float bc,
+                                           bool fll, bool flc, float vd, const Consts& K, float fc) {
+    Carry o;
+    o.a = (ac - al) * K.theta;
+    o.b = (bc - bl) * K.theta;
+    o.fb = (fll ? -bl : bl) * fc;
+    const float fcc = (flc ? -bc : bc) * fc;
+    if constexpr (SGN < 0) o.K = K.theta_g * ((ec - el) - vd * (fcc + o.fb));
+    else o.K = K.theta_g * ((ec - el) + vd * (fcc + o.fb));
+    return o;
 }
 
 // one face between cell m (low side) and p (high side) (:246-370):
-// reconstructed states, face depths, central-upwind flux.  For x faces
-// (a, b) = (u, v) and the result is (F.x, F.y, F.z); for y faces
-// (a, b) = (v, u) and the flux comes back as (G.x, G.z, G.y) -- the caller
-// swaps, as the reference's computeGFaceFlux does.
-__device__ __forceinline__ void face(float em, float am, float bm, float Sam, float Sbm, float Tm,
-                                     float ep, float ap, float bp, float Sap, float Sbp, float Tp,
-                                     float Hf, const Consts& K, float& F0, float& F1, float& F2) {
-    F0 = 0.0f; F1 = 0.0f; F2 = 0.0f;
-    if (!(ep == kDryFlag || em == kDryFlag)) {
-        const float Rpa = ap - 0.5f * Sap;
-        const float Rpb = bp - 0.5f * Sbp;
-        const float Rma = am + 0.5f * Sam;
-        const float Rmb = bm + 0.5f * Sbm;
-        const float hp = fmaxf(0.0f, (ep + Hf) - Tp);
-        const float hm = fmaxf(0.0f, (em + Hf) + Tm);
-        cu_flux(hm, Rma, Rmb, hp, Rpa, Rpb, K, F0, F1, F2);
+// reconstructed states and central-upwind flux.  dm / dp: the face depths
+// (e + H_face) + T of m and (e + H_face) - T of p before the clamp at zero,
+// as the slope centres formed them.  For x faces (a, b) = (u, v) and the
+// result is (F.x, F.y, F.z); for y faces (a, b) = (v, u) and the flux comes
+// back as (G.x, G.z, G.y) -- the caller swaps, as the reference's
+// computeGFaceFlux does.  When every lane of the wave has a wet face
+// (neither side dry-flagged, both depths above the cutoff, wave-speed spread
+// at least flux_eps), the same operations run without the selects of the
+// dry and zero-flux cases.
+__device__ __forceinline__ void face(float em, float am, float bm, float hSam, float hSbm, float dm,
+                                     float ep, float ap, float bp, float hSap, float hSbp, float dp,
+                                     const Consts& K, float& F0, float& F1, float& F2) {
+    const float hp = fmaxf(0.0f, dp);
+    const float hm = fmaxf(0.0f, dm);
+    if (__all(!(ep == kDryFlag || em == kDryFlag) && hp > K.cutoff && hm > K.cutoff)) {
+        const float up = ap - hSap, vp = bp - hSbp;
+        const float um = am + hSam, vm = bm + hSbm;
+        const float qp = hp * up;
+        const float Fpx = qp, Fpy = qp * up + (K.half_g * hp) * hp, Fpz = qp * vp;
+        const float c_p = sqrt_rn(K.g * hp);
+        const float qm = hm * um;
+        const float Fmx = qm, Fmy = qm * um + (K.half_g * hm) * hm, Fmz = qm * vm;
+        const float c_m = sqrt_rn(K.g * hm);
+        const float amin = fminf(fminf(um - c_m, up - c_p), 0.0f);
+        const float amax = fmaxf(fmaxf(um + c_m, up + c_p), 0.0f);
+        const float den = amax - amin;
+        const float apam = amax * amin;
+        const float rden = rcp_refined(den);
+        F0 = div_r((amax * Fmx - amin * Fpx) + apam * (hp - hm), den, rden);
+        F1 = div_r((amax * Fmy - amin * Fpy) + apam * (Fpx - Fmx), den, rden);
+        F2 = (um + up > 0.0f) ? Fmz : Fpz;
+        if (!__all(!(fabsf(den) < K.flux_eps))) {
+            const bool z = fabsf(den) < K.flux_eps;
+            F0 = z ? 0.0f : F0; F1 = z ? 0.0f : F1; F2 = z ? 0.0f : F2;
+        }
+        return;
     }
+    F0 = 0.0f; F1 = 0.0f; F2 = 0.0f;
+    if (!(ep == kDryFlag || em == kDryFlag))
+        cu_flux(hm, am + hSam, bm + hSbm, hp, ap - hSap, bp - hSbp, K, F0, F1, F2);
 }
 
 // One strip: padded columns sx*60 .. sx*60+63 (lanes), output rows
-// 2 + sy*BH .. min(2 + (sy+1)*BH, ny+2) - 1.
-template <int BH>
-__device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy) {
+// 2 + sy*bh .. min(2 + (sy+1)*bh, ny+2) - 1.
+template <bool ONE>
+__device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy, int bh) {
     const int lane = threadIdx.x & (kLanes - 1);
-    const int nx = A.nx, ny = A.ny, pitch = A.pitch, hp = A.hi_pitch;
+    const int nx = A.nx, ny = A.ny;
+    const uint32_t rowb = (uint32_t)A.pitch * 4u, hrowb = (uint32_t)A.hi_pitch * 4u;   // row bytes
     const Consts K = make_consts(A);
     const int C = sx * kOut + lane;                  // padded column of the lane
     const int Cl = min(C, nx + 3);                   // load column (clamped)
-    const int j0 = 2 + sy * BH;
-    const int j1 = min(j0 + BH, ny + 2);
+    const int j0 = 2 + sy * bh;
+    const int j1 = min(j0 + bh, ny + 2);
     const bool wN = A.bc_n() == kWall, wS = A.bc_s() == kWall;
     const bool wE = A.bc_e() == kWall, wW = A.bc_w() == kWall;
     const bool pNS = A.bc_n() == kPeriodic, pEW = A.bc_e() == kPeriodic;
@@ -208,7 +281,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
         if (wE) { sc = 2 * nx + 3 - Cl; neg_u = true; } else if (pEW) { sc = Cl - nx; }
     }
     auto col_flip = [&](int c) { return (wW && c < 2) || (wE && c > nx + 1); };
-    const bool xfl = col_flip(C - 1), xfc = col_flip(C), xfr = col_flip(C + 1);
+    const bool xfc = col_flip(C), xfr = col_flip(C + 1);
     const bool xfa = wW && C < 2;
     auto row_flip = [&](int r) { return (wS && r < 2) || (wN && r > ny + 1); };
     const int hc = min(C, nx + 4), hn = min(C + 1, nx + 4);   // corner-depth columns
@@ -217,6 +290,8 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
     const uint32_t ocl = (uint32_t)Cl * 4u;
     const int addr_r = ((lane + 1) & (kLanes - 1)) << 2;
     const int addr_l = ((lane - 1) & (kLanes - 1)) << 2;
+    const rsrc_t r_ein = plane(A.eta_in), r_uin = plane(A.hu_in), r_vin = plane(A.hv_in);
+    const rsrc_t r_hi = plane(A.Hi);
 
     const float fc = A.f;
     const float dt = A.dt_dev ? *A.dt_dev : A.dt;
@@ -230,22 +305,22 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
         } else if (R > ny + 1) {
             if (wN) { sr = 2 * ny + 3 - R; neg_v = true; } else if (pNS) { sr = R - ny; }
         }
-        const size_t ro = (size_t)sr * pitch;
-        e = ld(A.eta_in + ro, osc);
-        u = ld(A.hu_in + ro, osc);
-        v = ld(A.hv_in + ro, osc);
+        const uint32_t ro = (uint32_t)sr * rowb;
+        e = ld(r_ein, osc, ro);
+        u = ld(r_uin, osc, ro);
+        v = ld(r_vin, osc, ro);
         if (neg_u) u = -u;
         if (neg_v) v = -v;
     };
     auto load_hi = [&](int R, float& a, float& b) {
-        const size_t ro = (size_t)R * hp;
-        a = flag_land(ld(A.Hi + ro, ohc), K);
-        b = flag_land(ld(A.Hi + ro, ohn), K);
+        const uint32_t ro = (uint32_t)R * hrowb;
+        a = flag_land(ld(r_hi, ohc, ro), K);
+        b = flag_land(ld(r_hi, ohn, ro), K);
     };
 
     // ---- prologue: conversion of rows j0-2 .. j0+1, y slopes of rows
     //      j0-1 and j0, the flux through the face (j0-1 | j0) ----
-    float hcA, hnA, hcB, hnB, hcC, hnC, hcD, hnD;      // corner rows j-2 .. j+1 of the window
+    float hcA, hnA, hcB, hnB, hcC, hnC, hcD, hnD;      // corner rows j0-2 .. j0+1
     load_hi(j0 - 2, hcA, hnA);
     load_hi(j0 - 1, hcB, hnB);
     load_hi(j0, hcC, hnC);
@@ -267,29 +342,41 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
     }
     // corner rows now: A = j0-1, B = j0, C = j0+1, D = j0+2
     // face-averaged depths of the south faces: Hs(R) = 0.5 (Hi[R][C] + Hi[R][C+1])
-    float Hs_m1 = 0.5f * (hcA + hnA);    // row j0-1's south face
-    float Hs_0 = 0.5f * (hcB + hnB);     // row j0's south face (= row j0-1's north)
-    float Hs_1 = 0.5f * (hcC + hnC);     // row j0+1's south face
-    // y slopes of row j0-1 (cells c0, c1, c2) and row j0 (c1, c2, c3)
-    float Uy0, Vy0, S0, Uy1, Vy1, S1;
-    slope_centre<+1>(c0.E, c1.E, c2.E, c0.V, c1.V, c2.V, c0.U, c1.U, c2.U, Hs_m1, Hs_0,
-                     row_flip(j0 - 2), row_flip(j0 - 1), row_flip(j0), wS && j0 - 1 < 2,
-                     K.dy, K.udy, K, fc, Vy0, Uy0, S0);
-    slope_centre<+1>(c1.E, c2.E, c3.E, c1.V, c2.V, c3.V, c1.U, c2.U, c3.U, Hs_0, Hs_1,
-                     row_flip(j0 - 1), row_flip(j0), row_flip(j0 + 1), wS && j0 < 2,
-                     K.dy, K.udy, K, fc, Vy1, Uy1, S1);
-    // flux through (j0-1 | j0): m = row j0-1 (c1), p = row j0 (c2)
+    const float Hs_m1 = 0.5f * (hcA + hnA);    // row j0-1's south face
+    const float Hs_0 = 0.5f * (hcB + hnB);     // row j0's south face (= row j0-1's north)
+    const float Hs_1 = 0.5f * (hcC + hnC);     // row j0+1's south face
+    // y slopes of row j0-1 (cells c0, c1, c2) and row j0 (c1, c2, c3); in y
+    // (a, b) = (v, u).  Faces use half slopes, kept as such.
+    float hUy1, hVy1, S1, dhi1;
+    Carry ycar;
     float gsx, gsy, gsz;
     {
+        float Uy0, Vy0, S0, dlo0, dhi0, Uy1, Vy1, dlo1;
+        const Carry bwd = slope_bwd<+1>(c0.E, c1.E, c0.V, c1.V, c0.U, c1.U, row_flip(j0 - 2), row_flip(j0 - 1),
+                                        K.udy, K, fc);
+        Carry fwd;
+        float fr;
+        slope_fwd<+1>(c1.E, c2.E, c1.V, c2.V, c1.U, c2.U, row_flip(j0 - 1), row_flip(j0), K.udy, K, fc, fwd, fr);
+        slope_rest<+1, ONE>(c0.E, c1.E, c2.E, c0.V, c2.V, c0.U, c1.U, c2.U, Hs_m1, Hs_0, row_flip(j0 - 1),
+                            wS && j0 - 1 < 2, K.dy, K.udy, K, fc, bwd, fwd, fr, Vy0, Uy0, S0, dlo0, dhi0);
+        Carry fwd1;
+        slope_fwd<+1>(c2.E, c3.E, c2.V, c3.V, c2.U, c3.U, row_flip(j0), row_flip(j0 + 1), K.udy, K, fc, fwd1, fr);
+        slope_rest<+1, ONE>(c1.E, c2.E, c3.E, c1.V, c3.V, c1.U, c2.U, c3.U, Hs_0, Hs_1, row_flip(j0),
+                            wS && j0 < 2, K.dy, K.udy, K, fc, fwd, fwd1, fr, Vy1, Uy1, S1, dlo1, dhi1);
+        ycar = fwd1;
+        hUy1 = 0.5f * Uy1;
+        hVy1 = 0.5f * Vy1;
+        // flux through (j0-1 | j0): m = row j0-1 (c1), p = row j0 (c2)
         float a, b, c;
-        face(c1.E, c1.V, c1.U, Vy0, Uy0, S0, c2.E, c2.V, c2.U, Vy1, Uy1, S1, Hs_0, K, a, b, c);
+        face(c1.E, c1.V, c1.U, 0.5f * Vy0, 0.5f * Uy0, dhi0, c2.E, c2.V, c2.U, hVy1, hUy1, dlo1, K, a, b, c);
         gsx = a; gsy = c; gsz = b;
     }
-    // window: (c2, c3) = rows j, j+1; (Uy1, Vy1, S1) = y slopes of row j;
-    // corner rows B = j, C = j+1, D = j+2; Hs_0 = south face of row j,
-    // Hs_1 = its north face
+    // window: (cj, cj1) = rows j, j+1; (hUyj, hVyj, Sj, dhij) = half y slopes,
+    // depth term and north-face depth of row j; ycar = forward candidates of
+    // row j; corner rows j, j+1, j+2; Hsj = south face of row j, Hnj = its
+    // north face
     Cell cj = c2, cj1 = c3;
-    float Uyj = Uy1, Vyj = Vy1, Sj = S1;
+    float hUyj = hUy1, hVyj = hVy1, Sj = S1, dhij = dhi1;
     float hcj = hcB, hnj = hnB, hcj1 = hcC, hnj1 = hnC, hcj2 = hcD, hnj2 = hnD;
     float Hsj = Hs_0, Hnj = Hs_1;
     const bool friction = A.r > 0.0f;
@@ -300,42 +387,62 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
     if (wgrid) wax = wind_axis(C, nx, A.wind_wx);
     const int rk_order = A.rk_order(), rk_step = A.rk_step();
     const bool store_lane = lane >= 2 && lane < 2 + kOut && C <= nx + 1;
+    const rsrc_t r_eqn = plane(A.eta_qn), r_uqn = plane(A.hu_qn), r_vqn = plane(A.hv_qn);
+    const rsrc_t r_eo = plane(A.eta_out), r_uo = plane(A.hu_out), r_vo = plane(A.hv_out);
 
     for (int j = j0; j < j1; ++j) {
         // loads for row j+2 (conversion) and Q^n of row j (second stages)
         float e2, u2, v2, hcj3, hnj3;
         load_state(j + 2, e2, u2, v2);
-        load_hi(j + 3 <= ny + 4 ? j + 3 : ny + 4, hcj3, hnj3);
-        const size_t rj = (size_t)j * pitch;
+        load_hi(j + 3, hcj3, hnj3);
+        const uint32_t rj = (uint32_t)j * rowb;
         float ea = 0.0f, ha = 0.0f, va = 0.0f;
-        if (rk_step > 0) { ea = ld(A.eta_qn + rj, ocl); ha = ld(A.hu_qn + rj, ocl); va = ld(A.hv_qn + rj, ocl); }
+        if (rk_step > 0) { ea = ld(r_eqn, ocl, rj); ha = ld(r_uqn, ocl, rj); va = ld(r_vqn, ocl, rj); }
 
         // ---- x slope centre of row j (neighbours by lane exchange) ----
         const float el = lane_get(cj.E, addr_l), er = lane_get(cj.E, addr_r);
         const float ul = lane_get(cj.U, addr_l), ur = lane_get(cj.U, addr_r);
         const float vl = lane_get(cj.V, addr_l), vr = lane_get(cj.V, addr_r);
-        const float abj = hcj + hcj1;
-        const float Hw = 0.5f * abj;                     // west face depth of the cell
+        const float Hw = 0.5f * (hcj + hcj1);            // west face depth of the cell
         const float He = 0.5f * (hnj + hnj1);            // east face depth
-        float Ux, Vx, T;
-        slope_centre<-1>(el, cj.E, er, ul, cj.U, ur, vl, cj.V, vr, Hw, He, xfl, xfc, xfr, xfa,
-                         K.dx, K.vdx, K, fc, Ux, Vx, T);
+        float hUx, hVx, T, dlox, dhix;
+        {
+            Carry fwd, bwd;
+            float fr, Ux, Vx;
+            slope_fwd<-1>(cj.E, er, cj.U, ur, cj.V, vr, xfc, xfr, K.vdx, K, fc, fwd, fr);
+            bwd.a = lane_get(fwd.a, addr_l);
+            bwd.b = lane_get(fwd.b, addr_l);
+            bwd.K = lane_get(fwd.K, addr_l);
+            bwd.fb = lane_get(fwd.fb, addr_l);
+            slope_rest<-1, ONE>(el, cj.E, er, ul, ur, vl, cj.V, vr, Hw, He, xfc, xfa, K.dx, K.vdx, K, fc, bwd, fwd,
+                                fr, Ux, Vx, T, dlox, dhix);
+            hUx = 0.5f * Ux;
+            hVx = 0.5f * Vx;
+        }
         // ---- east face (C | C+1) ----
-        const float Uxr = lane_get(Ux, addr_r), Vxr = lane_get(Vx, addr_r), Tr = lane_get(T, addr_r);
+        const float hUxr = lane_get(hUx, addr_r), hVxr = lane_get(hVx, addr_r), dpr = lane_get(dlox, addr_r);
         float fex, fey, fez;
-        face(cj.E, cj.U, cj.V, Ux, Vx, T, er, ur, vr, Uxr, Vxr, Tr, He, K, fex, fey, fez);
+        face(cj.E, cj.U, cj.V, hUx, hVx, dhix, er, ur, vr, hUxr, hVxr, dpr, K, fex, fey, fez);
 
         // ---- conversion of row j+2, y slopes of row j+1, north face ----
         const Cell cj2 = convert(e2, u2, v2, hcj2 + hcj3, hcj2, hcj3, hnj2, hnj3, K);
         const float Hnj1 = 0.5f * (hcj2 + hnj2);         // north face of row j+1
-        float Uyn, Vyn, Sn;
-        slope_centre<+1>(cj.E, cj1.E, cj2.E, cj.V, cj1.V, cj2.V, cj.U, cj1.U, cj2.U, Hnj, Hnj1,
-                         row_flip(j), row_flip(j + 1), row_flip(j + 2), wS && j + 1 < 2,
-                         K.dy, K.udy, K, fc, Vyn, Uyn, Sn);
+        float hUyn, hVyn, Sn, dlon, dhin;
+        {
+            Carry fwd;
+            float fr, Uyn, Vyn;
+            slope_fwd<+1>(cj1.E, cj2.E, cj1.V, cj2.V, cj1.U, cj2.U, row_flip(j + 1), row_flip(j + 2), K.udy, K,
+                          fc, fwd, fr);
+            slope_rest<+1, ONE>(cj.E, cj1.E, cj2.E, cj.V, cj2.V, cj.U, cj1.U, cj2.U, Hnj, Hnj1, row_flip(j + 1),
+                                wS && j + 1 < 2, K.dy, K.udy, K, fc, ycar, fwd, fr, Vyn, Uyn, Sn, dlon, dhin);
+            ycar = fwd;
+            hUyn = 0.5f * Uyn;
+            hVyn = 0.5f * Vyn;
+        }
         float gnx, gny, gnz;
         {
             float a, b, c;
-            face(cj.E, cj.V, cj.U, Vyj, Uyj, Sj, cj1.E, cj1.V, cj1.U, Vyn, Uyn, Sn, Hnj, K, a, b, c);
+            face(cj.E, cj.V, cj.U, hVyj, hUyj, dhij, cj1.E, cj1.V, cj1.U, hVyn, hUyn, dlon, K, a, b, c);
             gnx = a; gny = c; gnz = b;
         }
         // ---- west face from the left lane ----
@@ -346,18 +453,13 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
             const float Hm = cj.Hm4;
             const float ec = cj.E, uc = cj.U, vc = cj.V;
             const float hreg = ec + Hm;
-            float hu = 0.0f, hv = 0.0f;
-            if (hreg > K.cutoff) {
-                hu = uc * hreg;
-                hv = vc * hreg;
-            }
-            const float fd0 = div_r(fex - fwx, K.dx, K.r_dx) + div_r(gnx - gsx, K.dy, K.r_dy);
-            const float fd1 = div_r(fey - fwy, K.dx, K.r_dx) + div_r(gny - gsy, K.dy, K.r_dy);
-            const float fd2 = div_r(fez - fwz, K.dx, K.r_dx) + div_r(gnz - gsz, K.dy, K.r_dy);
+            const float fd0 = div_k<ONE>(fex - fwx, K.dx, K.r_dx) + div_k<ONE>(gnx - gsx, K.dy, K.r_dy);
+            const float fd1 = div_k<ONE>(fey - fwy, K.dx, K.r_dx) + div_k<ONE>(gny - gsy, K.dy, K.r_dy);
+            const float fd2 = div_k<ONE>(fez - fwz, K.dx, K.r_dx) + div_k<ONE>(gnz - gsz, K.dy, K.r_dy);
             const float eta_west = ec - T, eta_east = ec + T;
             const float eta_south = ec - Sj, eta_north = ec + Sj;
-            float st1 = 0.0f, st2 = 0.0f;
-            if (hreg >= K.cutoff && ec != kDryFlag) {
+            // wind, Coriolis and bottom slope of a wet cell
+            auto sources = [&](float hu, float hv, float& st1, float& st2) {
                 float X = A.wind_x, Y = A.wind_y;
                 if (wgrid)
                     wind_grid_at(A.wind_grid, A.wind_wx, A.wind_wy, A.wind_t, wax, wind_axis(j, ny, A.wind_wy),
@@ -370,8 +472,22 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
                 const float b2 = (K.g * (eta_sn + Hm)) * H_y;
                 const float hu_cor = fc * hv;
                 const float hv_cor = (-fc) * hu;
-                st1 = (X + hu_cor) + div_r(b1, K.dx, K.r_dx);
-                st2 = (Y + hv_cor) + div_r(b2, K.dy, K.r_dy);
+                st1 = (X + hu_cor) + div_k<ONE>(b1, K.dx, K.r_dx);
+                st2 = (Y + hv_cor) + div_k<ONE>(b2, K.dy, K.r_dy);
+            };
+            float hu, hv, st1, st2;
+            if (__all(hreg > K.cutoff && ec != kDryFlag)) {
+                // every writing lane wet: no selects
+                hu = uc * hreg;
+                hv = vc * hreg;
+                sources(hu, hv, st1, st2);
+            } else {
+                hu = 0.0f; hv = 0.0f; st1 = 0.0f; st2 = 0.0f;
+                if (hreg > K.cutoff) {
+                    hu = uc * hreg;
+                    hv = vc * hreg;
+                }
+                if (hreg >= K.cutoff && ec != kDryFlag) sources(hu, hv, st1, st2);
             }
             const float L1 = -fd0;
             const float L2 = -fd1 + st1;
@@ -428,9 +544,9 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
                 uv = 0.0f;
             }
             ue = fmaxf(-Hm + K.cutoff, ue);
-            st(A.eta_out + rj, ocl, ue);
-            st(A.hu_out + rj, ocl, uu);
-            st(A.hv_out + rj, ocl, uv);
+            st(r_eo, ocl, rj, ue);
+            st(r_uo, ocl, rj, uu);
+            st(r_vo, ocl, rj, uv);
             // periodic ghost images of the output (cells within two rows /
             // columns of a periodic side)
             if ((pNS && (j <= 3 || j >= ny)) || (pEW && (C <= 3 || C >= nx))) {
@@ -447,8 +563,8 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
                 for (int a = 0; a < nr; ++a) {
                     for (int b = 0; b < nc; ++b) {
                         if (a == 0 && b == 0) continue;
-                        const size_t q = (size_t)rows[a] * pitch + cols[b];
-                        A.eta_out[q] = ue; A.hu_out[q] = uu; A.hv_out[q] = uv;
+                        const uint32_t ro = (uint32_t)rows[a] * rowb, co = (uint32_t)cols[b] * 4u;
+                        st(r_eo, co, ro, ue); st(r_uo, co, ro, uu); st(r_vo, co, ro, uv);
                     }
                 }
             }
@@ -457,7 +573,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
         // ---- slide the window one row north ----
         gsx = gnx; gsy = gny; gsz = gnz;
         cj = cj1; cj1 = cj2;
-        Uyj = Uyn; Vyj = Vyn; Sj = Sn;
+        hUyj = hUyn; hVyj = hVyn; Sj = Sn; dhij = dhin;
         hcj = hcj1; hnj = hnj1; hcj1 = hcj2; hnj1 = hnj2; hcj2 = hcj3; hnj2 = hnj3;
         Hsj = Hnj; Hnj = Hnj1;
     }

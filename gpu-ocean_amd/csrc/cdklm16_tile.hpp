@@ -47,22 +47,27 @@ struct SubstepArgs {
     // (wind_lerp: the kernel's former expression, so the same bits)
     float wind_x, wind_y;
     // gridded wind (nullptr: the uniform wind above): X0, X1, Y0, Y1 planes
-    // of wind_wy x wind_wx, time weight wind_t
-    const float* wind_grid;
-    int wind_wx, wind_wy;
-    float wind_t;
+    // of wind_wy x wind_wx, time weight wind_t.  Initialised here so that
+    // every construction of the struct (host or device) starts uniform.
+    const float* wind_grid = nullptr;
+    int wind_wx = 0, wind_wy = 0;
+    float wind_t = 0.0f;
     float dt;
     // bits 0-7: north, east, south, west boundary codes (2 bits each); bits
     // 8-9: rk_order; bits 10-11: rk_step.  One word instead of six keeps the
     // kernel within the scalar registers of 8 waves per SIMD.
     uint32_t flags;
     int tiles_x, tiles_y;
+    int march_bh = 8;          // rows per strip of the column-marching kernel
     __host__ __device__ int bc_n() const { return (int)(flags & 3u); }
     __host__ __device__ int bc_e() const { return (int)((flags >> 2) & 3u); }
     __host__ __device__ int bc_s() const { return (int)((flags >> 4) & 3u); }
     __host__ __device__ int bc_w() const { return (int)((flags >> 6) & 3u); }
     __host__ __device__ int rk_order() const { return (int)((flags >> 8) & 3u); }
     __host__ __device__ int rk_step() const { return (int)((flags >> 10) & 3u); }
+    // bit 12: divisions by dx, dy and 2g may take one residual correction
+    __host__ __device__ bool one_div() const { return (flags >> 12) & 1u; }
+    __host__ __device__ void set_one_div(bool on) { flags = (flags & ~(1u << 12)) | ((on ? 1u : 0u) << 12); }
     __host__ __device__ void set_flags(int n, int e, int s_, int w, int order, int step) {
         flags = (uint32_t)(n & 3) | ((uint32_t)(e & 3) << 2) | ((uint32_t)(s_ & 3) << 4) |
                 ((uint32_t)(w & 3) << 6) | ((uint32_t)(order & 3) << 8) | ((uint32_t)(step & 3) << 10);
@@ -210,6 +215,24 @@ __device__ __forceinline__ float div_r(float n, float d, float r) {
     q = fmaf(e, r, q);
     e = fmaf(-d, q, n);
     return fmaf(e, r, q);
+}
+
+// n/d for a launch-constant divisor d (dx, dy, 2g) with r = RN(1/d) formed on
+// the host: one residual correction.  The host enables it (SubstepArgs
+// one_div()) only for divisors for which it has checked, over every float
+// significand of n, that this gives the correctly rounded quotient (see
+// one_correction_exact in gocean_fwd.hip); otherwise div_r's two
+// corrections are used.
+__device__ __forceinline__ float div_c1(float n, float d, float r) {
+    const float q = n * r;
+    const float e = fmaf(-d, q, n);
+    return fmaf(e, r, q);
+}
+
+template <bool ONE>
+__device__ __forceinline__ float div_k(float n, float d, float r) {
+    if constexpr (ONE) return div_c1(n, d, r);
+    else return div_r(n, d, r);
 }
 
 // correctly rounded sqrt for x == 0 or x >= 2^-96: hardware estimate, then

@@ -50,6 +50,7 @@ struct EnsLaunch {
     const float* dt;           // one per member
     int rk_order, rk_step;
     int n_tiles;
+    int march_bh;              // rows per strip (column-marching configurations)
 };
 
 // 512-thread tiles: 8 waves per SIMD, as k_substep
@@ -74,6 +75,7 @@ __device__ __forceinline__ void member_args(const EnsLaunch& L, const EnsMember&
     A.desing_eps = M.desing; A.flux_eps = M.flux_eps; A.cutoff = M.cutoff; A.land = M.land;
     A.f = M.f; A.north_x = M.north_x; A.north_y = M.north_y;
     A.wind_x = M.wind_x; A.wind_y = M.wind_y;
+    A.wind_grid = nullptr; A.wind_wx = 0; A.wind_wy = 0; A.wind_t = 0.0f;   // members: uniform wind
     A.dt = 0.0f;
     A.set_flags(M.bc_n, M.bc_e, M.bc_s, M.bc_w, L.rk_order, L.rk_step);
     A.vdx = M.vdx; A.udy = M.udy; A.r_two_g = M.r_two_g; A.r_dx = M.r_dx; A.r_dy = M.r_dy;
@@ -82,7 +84,7 @@ __device__ __forceinline__ void member_args(const EnsLaunch& L, const EnsMember&
 
 // column-marching substep over every member's strips: wavefront -> strip
 // through the tile map (one entry per strip), as k_march
-template <int BH>
+template <bool ONE>
 __global__ __launch_bounds__(256) void k_ens_march(EnsLaunch L) {
     const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
     if (wave >= L.n_tiles) return;
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(256) void k_ens_march(EnsLaunch L) {
     member_args<kModeZero>(L, M, m, A);
     const int lt = wave - M.tile_begin;
     const int sy = lt / M.tiles_x, sx = lt - sy * M.tiles_x;
-    march::march_strip<BH>(A, sx, sy);
+    march::march_strip<ONE>(A, sx, sy, L.march_bh);
 }
 
 template <int BW, int BH, int NT, int MODE>
@@ -280,6 +282,7 @@ __global__ __launch_bounds__(kEnsTapeNT) __attribute__((amdgpu_waves_per_eu(8)))
     A.desing_eps = M.desing; A.flux_eps = M.flux_eps; A.cutoff = M.cutoff; A.land = M.land;
     A.f = M.f; A.north_x = M.north_x; A.north_y = M.north_y;
     A.wind_x = M.wind_x; A.wind_y = M.wind_y;
+    A.wind_grid = nullptr; A.wind_wx = 0; A.wind_wy = 0; A.wind_t = 0.0f;   // members: uniform wind
     A.dt = 0.0f;
     A.set_flags(M.bc_n, M.bc_e, M.bc_s, M.bc_w, 2, L.rk_step);
     A.vdx = M.vdx; A.udy = M.udy; A.r_two_g = M.r_two_g; A.r_dx = M.r_dx; A.r_dy = M.r_dy;
@@ -374,6 +377,7 @@ struct gocean_ensemble {
     int bw = 32, bh = 16, nt = 256;
     int n_tiles = 0;
     int device = 0;
+    bool one_div = false;      // every member's dx, dy, 2g divide with one correction
     EnsMember* d_members = nullptr;
     int* d_tile_member = nullptr;
     std::vector<EnsMember> members;
@@ -395,9 +399,24 @@ static void ens_launch_t(const EnsLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL((k_ens_substep<BW, BH, NT, MODE>), dim3(L.n_tiles), dim3(NT), 0, s, L);
 }
 
-template <int BH>
-static void ens_march_t(const EnsLaunch& L, hipStream_t s) {
-    hipLaunchKernelGGL((k_ens_march<BH>), dim3((L.n_tiles + 3) / 4), dim3(256), 0, s, L);
+static void ens_march(EnsLaunch L, int bh, bool one_div, hipStream_t s) {
+    L.march_bh = bh;
+    if (one_div)
+        hipLaunchKernelGGL((k_ens_march<true>), dim3((L.n_tiles + 3) / 4), dim3(256), 0, s, L);
+    else
+        hipLaunchKernelGGL((k_ens_march<false>), dim3((L.n_tiles + 3) / 4), dim3(256), 0, s, L);
+}
+
+// rows per strip of the ensemble's column-marching launch (configuration
+// with bh == 0): whole rounds of resident waves over all members' strips
+static int ens_march_rows(const std::vector<EnsMember>& members, bool one_div) {
+    std::vector<int> strips, ny;
+    for (const EnsMember& M : members) {
+        strips.push_back((M.nx + march::kOut - 1) / march::kOut);
+        ny.push_back(M.ny);
+    }
+    const int resident = resident_waves(one_div ? (const void*)k_ens_march<true> : (const void*)k_ens_march<false>);
+    return march_rows(strips.data(), ny.data(), (int)members.size(), resident);
 }
 
 static int ens_launch(const gocean_ensemble* E, EnsLaunch L, hipStream_t s) {
@@ -417,9 +436,9 @@ static int ens_launch(const gocean_ensemble* E, EnsLaunch L, hipStream_t s) {
             case 4: ens_launch_t<64, 4, 256, kModeZero>(L, s); break;
             case 5: ens_launch_t<32, 16, 512, kModeZero>(L, s); break;
             case 6: ens_launch_t<32, 14, 512, kModeZero>(L, s); break;
-            case 7: ens_march_t<8>(L, s); break;
-            case 8: ens_march_t<16>(L, s); break;
-            case 9: ens_march_t<32>(L, s); break;
+            case 7:
+            case 8:
+            case 9: ens_march(L, E->bh, E->one_div, s); break;
         }
     }
     return check_launch("ensemble substep launch");
@@ -469,7 +488,10 @@ int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_confi
     if (!E) return set_error(GOCEAN_ENOMEM, "host allocation failed");
     E->n = n;
     E->rk_order = members[0].domain.rk_order;
-    E->cfg = tile_config < 0 ? kDefaultTileConfig : tile_config;
+    long long mean_cells = 0;
+    for (int k = 0; k < n; ++k) mean_cells += (long long)members[k].domain.nx * members[k].domain.ny;
+    mean_cells /= n;
+    E->cfg = tile_config < 0 ? default_tile_config(mean_cells) : tile_config;
     bool any_field = false, any_const = false;
     for (int k = 0; k < n; ++k) {
         const gocean_domain_t& d = members[k].domain;
@@ -492,8 +514,19 @@ int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_confi
             }
     }
     E->mode = any_field ? kModeField : (any_const ? kModeConst : kModeZero);
+    E->one_div = true;
+    for (int k = 0; k < n; ++k) {
+        const gocean_domain_t& d = members[k].domain;
+        E->one_div = E->one_div && one_correction_exact(2.0f * d.g) && one_correction_exact(d.dx) &&
+                     one_correction_exact(d.dy);
+    }
     const TileConfig& tc = (E->mode == kModeZero) ? kTileConfigs[E->cfg] : kTileConfigs[0];
     E->bw = tc.bw; E->bh = tc.bh; E->nt = tc.nt;
+    if (E->bh == 0) {
+        std::vector<EnsMember> shape(n);
+        for (int k = 0; k < n; ++k) { shape[k].nx = members[k].domain.nx; shape[k].ny = members[k].domain.ny; }
+        E->bh = ens_march_rows(shape, E->one_div);
+    }
     std::vector<int> tile_member;
     E->members.resize(n);
     for (int k = 0; k < n; ++k) {
@@ -555,7 +588,11 @@ int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_confi
 int gocean_ensemble_set_tile_config(gocean_ensemble_t* E, int tile_config) {
     clear_error();
     if (!E) return set_error(GOCEAN_EINVAL, "ensemble is NULL");
-    if (tile_config < 0) tile_config = kDefaultTileConfig;
+    if (tile_config < 0) {
+        long long mean_cells = 0;
+        for (const EnsMember& M : E->members) mean_cells += (long long)M.nx * M.ny;
+        tile_config = default_tile_config(mean_cells / E->n);
+    }
     if (tile_config >= kNumTileConfigs)
         return set_error(GOCEAN_EINVAL, "tile configuration %d out of range", tile_config);
     // work already queued may still read the old tables
@@ -563,6 +600,7 @@ int gocean_ensemble_set_tile_config(gocean_ensemble_t* E, int tile_config) {
     E->cfg = tile_config;
     const TileConfig& tc = (E->mode == kModeZero) ? kTileConfigs[E->cfg] : kTileConfigs[0];
     E->bw = tc.bw; E->bh = tc.bh; E->nt = tc.nt;
+    if (E->bh == 0) E->bh = ens_march_rows(E->members, E->one_div);
     std::vector<int> tile_member;
     for (int k = 0; k < E->n; ++k) {
         EnsMember& M = E->members[k];

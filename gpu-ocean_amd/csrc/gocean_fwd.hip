@@ -5,9 +5,13 @@
 // include/gocean.h for the contract of each entry point.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "../../include/gocean.h"
 #include "cdklm16_march.hpp"
@@ -67,6 +71,65 @@ int validate_domain(const gocean_domain_t* d) {
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// division by a launch constant with one residual correction
+// ---------------------------------------------------------------------------
+// For q = RN(n r), r = RN(1/d), one correction RN(q + RN(n - d q) r) is the
+// correctly rounded n/d for most d but is not guaranteed for all.  Scaling n
+// by a power of two scales q, the residual and the quotient exactly, so the
+// check over the 2^23 significands of n in [1, 2) decides it for every n of
+// normal magnitude (and -n by symmetry).  Done once per divisor value on the
+// host, with the hardware FMA when the CPU has one.
+namespace {
+
+__attribute__((target("fma"))) bool one_correction_scan_fma(float d) {
+    const float r = 1.0f / d;
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        uint32_t bits = 0x3f800000u | m;
+        float n;
+        std::memcpy(&n, &bits, 4);
+        const float q = n * r;
+        const float e = __builtin_fmaf(-d, q, n);
+        const float q1 = __builtin_fmaf(e, r, q);
+        if (q1 != n / d) return false;
+    }
+    return true;
+}
+
+bool one_correction_scan_libm(float d) {
+    const float r = 1.0f / d;
+    for (uint32_t m = 0; m < (1u << 23); ++m) {
+        uint32_t bits = 0x3f800000u | m;
+        float n;
+        std::memcpy(&n, &bits, 4);
+        const float q = n * r;
+        const float e = std::fmaf(-d, q, n);
+        const float q1 = std::fmaf(e, r, q);
+        if (q1 != n / d) return false;
+    }
+    return true;
+}
+
+std::mutex g_div_mu;
+std::unordered_map<uint32_t, bool> g_div_ok;
+
+}  // namespace
+
+bool one_correction_exact(float d) {
+    if (!(d > 0.0f) || !std::isnormal(d) || !std::isnormal(1.0f / d)) return false;
+    uint32_t key;
+    std::memcpy(&key, &d, 4);
+    {
+        std::lock_guard<std::mutex> lock(g_div_mu);
+        auto it = g_div_ok.find(key);
+        if (it != g_div_ok.end()) return it->second;
+    }
+    const bool ok = __builtin_cpu_supports("fma") ? one_correction_scan_fma(d) : one_correction_scan_libm(d);
+    std::lock_guard<std::mutex> lock(g_div_mu);
+    g_div_ok[key] = ok;
+    return ok;
+}
+
 void fill_args(const gocean_domain_t* d, SubstepArgs& A) {
     std::memset(&A, 0, sizeof(A));
     A.nx = d->nx; A.ny = d->ny; A.pitch = d->pitch; A.hi_pitch = d->hi_pitch;
@@ -76,6 +139,7 @@ void fill_args(const gocean_domain_t* d, SubstepArgs& A) {
     A.f = d->f; A.north_x = d->north_x; A.north_y = d->north_y;
     A.set_flags(d->bc_north, d->bc_east, d->bc_south, d->bc_west, d->rk_order, 0);
     substep_derive(A);
+    A.set_one_div(one_correction_exact(2.0f * A.g) && one_correction_exact(A.dx) && one_correction_exact(A.dy));
 }
 
 // ---------------------------------------------------------------------------
@@ -95,10 +159,11 @@ const TileConfig kTileConfigs[] = {
     // wavefront, BH rows; four independent wavefronts per workgroup
     {60, 8, 256, "march60x8/256"},
     {60, 16, 256, "march60x16/256"},
-    {60, 32, 256, "march60x32/256"},
+    {60, 0, 256, "march60xauto/256"},    // rows per strip chosen to fill whole rounds
 };
 const int kNumTileConfigs = sizeof(kTileConfigs) / sizeof(kTileConfigs[0]);
 const int kDefaultTileConfig = 1;
+const int kMarchAutoConfig = 9;
 
 // 512-thread tiles: 8 waves per SIMD (four 38 KB workgroups per CU) at the
 // price of a few scalar-register spills into vector-register lanes; measured
@@ -116,20 +181,65 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
 
 // column-marching substep: wavefront w of the grid takes strip (sx, sy),
 // sx fastest, workgroups banded over the XCDs
-template <int BH>
+template <bool ONE>
 __global__ __launch_bounds__(256) void k_march(SubstepArgs A) {
     const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
     if (wave >= A.tiles_x * A.tiles_y) return;
     const int sy = wave / A.tiles_x, sx = wave - sy * A.tiles_x;
-    march::march_strip<BH>(A, sx, sy);
+    march::march_strip<ONE>(A, sx, sy, A.march_bh);
 }
 
-template <int BH>
-static void launch_march_t(SubstepArgs A, hipStream_t s) {
+// waves of a kernel that the device holds at once: 4 SIMDs per CU, each
+// holding as many waves as its 512 vector registers per lane allow
+int resident_waves(const void* kernel) {
+    int regs = 128;
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, kernel) == hipSuccess && fa.numRegs > 0) regs = fa.numRegs;
+    else (void)hipGetLastError();
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        cus = 256;
+    }
+    const int alloc = (regs + 7) / 8 * 8;
+    const int per_simd = std::max(1, std::min(8, 512 / alloc));
+    return cus * 4 * per_simd;
+}
+
+// Rows per strip for the column-marching kernel over domains of the given
+// strip counts and heights: each wave's time grows with its rows plus about
+// three rows of prologue, and waves run in rounds of `resident`; pick the
+// row count (8..256) with the least rounds * (rows + 3).
+int march_rows(const int* strips, const int* ny, int n, int resident) {
+    int best = 8;
+    double best_cost = 1e300;
+    for (int bh = 8; bh <= 256; ++bh) {
+        long long waves = 0;
+        for (int k = 0; k < n; ++k) waves += (long long)strips[k] * ((ny[k] + bh - 1) / bh);
+        const long long rounds = (waves + resident - 1) / resident;
+        const double cost = (double)rounds * (bh + 3);
+        if (cost < best_cost - 1e-9) { best_cost = cost; best = bh; }
+    }
+    return best;
+}
+
+// bh > 0: fixed rows per strip; 0: balanced by march_rows
+static void launch_march(SubstepArgs A, int bh, hipStream_t s) {
     A.tiles_x = (A.nx + march::kOut - 1) / march::kOut;
-    A.tiles_y = (A.ny + BH - 1) / BH;
+    if (bh <= 0) {
+        static int resident[2] = {0, 0};
+        const int k = A.one_div() ? 1 : 0;
+        if (!resident[k])
+            resident[k] = resident_waves(k ? (const void*)k_march<true> : (const void*)k_march<false>);
+        bh = march_rows(&A.tiles_x, &A.ny, 1, resident[k]);
+    }
+    A.march_bh = bh;
+    A.tiles_y = (A.ny + bh - 1) / bh;
     const int waves = A.tiles_x * A.tiles_y;
-    hipLaunchKernelGGL((k_march<BH>), dim3((waves + 3) / 4), dim3(256), 0, s, A);
+    if (A.one_div())
+        hipLaunchKernelGGL((k_march<true>), dim3((waves + 3) / 4), dim3(256), 0, s, A);
+    else
+        hipLaunchKernelGGL((k_march<false>), dim3((waves + 3) / 4), dim3(256), 0, s, A);
 }
 
 template <int BW, int BH, int NT, int MODE>
@@ -181,8 +291,12 @@ int substep_mode(const SubstepArgs& A, bool field) {
     return (A.north_x == 0.0f && A.north_y == 1.0f) ? kModeZero : kModeConst;
 }
 
+// default configuration: the column-marching kernel with balanced rows
+// per strip from a million cells on (where it is faster), else the 2-D tile
+int default_tile_config(long long cells) { return cells >= (1LL << 20) ? kMarchAutoConfig : kDefaultTileConfig; }
+
 int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s) {
-    if (cfg < 0) cfg = kDefaultTileConfig;
+    if (cfg < 0) cfg = default_tile_config((long long)A.nx * A.ny);
     if (cfg >= kNumTileConfigs)
         return set_error(GOCEAN_EINVAL, "tile configuration %d out of range [0,%d)", cfg, kNumTileConfigs);
     const int mode = substep_mode(A, field);
@@ -200,9 +314,9 @@ int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s) {
             case 4: launch_substep_t<64, 4, 256, kModeZero>(A, s); break;
             case 5: launch_substep_t<32, 16, 512, kModeZero>(A, s); break;
             case 6: launch_substep_t<32, 14, 512, kModeZero>(A, s); break;
-            case 7: launch_march_t<8>(A, s); break;
-            case 8: launch_march_t<16>(A, s); break;
-            case 9: launch_march_t<32>(A, s); break;
+            case 7: launch_march(A, 8, s); break;
+            case 8: launch_march(A, 16, s); break;
+            case 9: launch_march(A, 0, s); break;
         }
     }
     return check_launch("cdklm substep launch");

@@ -18,6 +18,9 @@ struct TileConfig {
 extern const TileConfig kTileConfigs[];
 extern const int kNumTileConfigs;
 extern const int kDefaultTileConfig;
+extern const int kMarchAutoConfig;
+// configuration used when the caller passes -1
+int default_tile_config(long long cells);
 
 int set_error(int code, const char* fmt, ...);
 void clear_error();
@@ -25,6 +28,13 @@ const char* last_error();
 int check_launch(const char* what);
 int validate_domain(const gocean_domain_t* d);
 void fill_args(const gocean_domain_t* d, SubstepArgs& A);
+// true when one residual correction divides every float by d correctly
+// rounded (checked once per value of d, on the host)
+bool one_correction_exact(float d);
+// column-marching launch shape: waves the device holds at once for a kernel,
+// and the rows per strip that fill whole rounds of them
+int resident_waves(const void* kernel);
+int march_rows(const int* strips, const int* ny, int n, int resident);
 int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s);
 // zero-rotation substep that also writes the branch record (A.tape_*)
 int launch_substep_tape(SubstepArgs A, hipStream_t s);

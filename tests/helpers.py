@@ -110,6 +110,31 @@ def oracle_params(inp, **kw):
                     f=inp.f, wind_x=(inp.wind_x, inp.wind_x), wind_y=(inp.wind_y, inp.wind_y), **kw)
 
 
+TINY32 = np.float32(2.0 ** -126)
+
+
+def normal_equal(a, b):
+    """Bit equality of every value of normal float32 magnitude; values below
+    2^-126 count as zero on both sides.  The reference's CUDA build flushes
+    subnormals to zero (--ftz=true, gpu_ocean/SWESimulators/CDKLM16.py:197),
+    and the HIP kernels' division corrections are exact for normal quotients
+    only, so subnormal results may differ from the IEEE oracle in their last
+    bits."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    fa = np.where(np.abs(a) < TINY32, np.float32(0.0), a)
+    fb = np.where(np.abs(b) < TINY32, np.float32(0.0), b)
+    return np.array_equal(fa, fb)
+
+
+def diff_report(a, b, n=4):
+    """First few cells where a and b differ: (index, a, b)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    idx = np.argwhere(a != b)[:n]
+    return [(tuple(int(v) for v in i), float(a[tuple(i)]), float(b[tuple(i)])) for i in idx]
+
+
 def max_rel(a, b):
     """max |a-b| / max |b| (the normalisation of the reference's tests)."""
     a = np.asarray(a, np.float64)
