@@ -403,10 +403,12 @@ int periodic_bc(const gocean_domain_t* d, float* e, float* u, float* v, hipStrea
 
 // ---------------------------------------------------------------------------
 // CFL time step (max_dt.cu:42-194).  The minimum is exact, so any reduction
-// order gives the reference's value; partial minima go through the caller's
-// buffer, so the two launches are graph-capturable and need no memset.
+// order gives the reference's value; partial minima go through a buffer the
+// caller provides (or the static array of gocean_max_dt_static), so the two
+// launches are graph-capturable and need no memset: every launch writes each
+// partial it reads.
 // ---------------------------------------------------------------------------
-constexpr int kDtBlocks = 1024;
+constexpr int kDtBlocks = kMaxDtPartials;
 constexpr int kDtThreads = 256;
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -464,13 +466,20 @@ __global__ __launch_bounds__(kDtThreads) void k_dt_final(const float* partial, i
     }
 }
 
+int max_dt_partials(const gocean_domain_t* d) { return d->ny < kDtBlocks ? d->ny : kDtBlocks; }
+
+int max_dt_into(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
+                const float* Hm, float courant, float* partial, float* out, hipStream_t s) {
+    const int nb = max_dt_partials(d);
+    hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kDtThreads), 0, s, eta, hu, hv, Hm, d->nx,
+                       d->ny, d->pitch, d->dx, d->dy, d->g, d->land_value, partial);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kDtThreads), 0, s, partial, nb, courant, out);
+    return check_launch("max dt launch");
+}
+
 int max_dt(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
            const float* Hm, float courant, float* out, hipStream_t s) {
-    const int nb = d->ny < kDtBlocks ? d->ny : kDtBlocks;
-    hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(kDtThreads), 0, s, eta, hu, hv, Hm, d->nx,
-                       d->ny, d->pitch, d->dx, d->dy, d->g, d->land_value, out + 2);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(kDtThreads), 0, s, out + 2, nb, courant, out);
-    return check_launch("max dt launch");
+    return max_dt_into(d, eta, hu, hv, Hm, courant, out + 2, out, s);
 }
 
 }  // namespace gocean

@@ -39,6 +39,13 @@ int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s);
 // zero-rotation substep that also writes the branch record (A.tape_*)
 int launch_substep_tape(SubstepArgs A, hipStream_t s);
 int periodic_bc(const gocean_domain_t* d, float* e, float* u, float* v, hipStream_t s);
+// CFL reduction: out[0..1] as gocean_max_dt, partial minima (one per
+// reduction workgroup, max_dt_partials(d) <= kMaxDtPartials of them) in
+// `partial`; max_dt keeps them in out[2..]
+constexpr int kMaxDtPartials = 1024;
+int max_dt_partials(const gocean_domain_t* d);
+int max_dt_into(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
+                const float* Hm, float courant, float* partial, float* out, hipStream_t s);
 int max_dt(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
            const float* Hm, float courant, float* out, hipStream_t s);
 

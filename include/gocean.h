@@ -137,10 +137,24 @@ int gocean_periodic_bc(const gocean_domain_t* d, float* eta, float* hu, float* h
  * gpu_kernels/max_dt.cu:42-194, CDKLM16.updateDt, CDKLM16.py:712-739).
  * Writes out_dev[0] = min over wet interior cells of
  * min(0.25dx/|u+-c|, 0.25dy/|v+-c|) (1e5 if none), and, when courant > 0,
- * out_dev[1] = courant*out_dev[0] rounded to float.  No host sync. */
+ * out_dev[1] = courant*out_dev[0] rounded to float.  out_dev holds
+ * 2 + min(ny, 1024) floats: the partial minima of the reduction follow the
+ * two results.  No host sync. */
 int gocean_max_dt(const gocean_domain_t* d, const float* eta, const float* hu,
                   const float* hv, const float* Hm, float courant, float* out_dev,
                   void* stream);
+
+/* gocean_max_dt without the caller's partial-minima buffer: out_dev holds
+ * just the two results, and the partial minima go to a static device array of
+ * the library (1024 floats, one copy per GPU, kept until the process ends).
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc): it calls no allocator.  Calls that overlap on one GPU
+ * share the array and race; calls on one stream, or ordered by events, are
+ * safe.  Same results as gocean_max_dt, bit for bit. */
+int gocean_max_dt_static(const gocean_domain_t* d, const float* eta, const float* hu,
+                         const float* hv, const float* Hm, float courant, float* out_dev,
+                         void* stream);
 
 /* Tile configuration (index into gocean_tile_configs, -1: the default
  * 32x16/512) of the recording forward substep the adjoint entry points run
