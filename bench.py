@@ -56,6 +56,28 @@ def make_sim(inp, ctx, tile_config=None, dt=None):
     return sim
 
 
+def time_isolated_step(sim, reps=20):
+    """Median time of one RK2 step (two launches) bracketed by its own pair
+    of events and followed by a synchronisation, as a profiler's per-dispatch
+    timestamps see it; time_plain's back-to-back average is the throughput.
+    A short spin kernel ahead of the first event keeps the host's launch
+    latency out of the bracket."""
+    import torch
+    st = sim.gpu_stream
+    out = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        with torch.cuda.stream(st):
+            torch.cuda._sleep(200000)
+        e0.record(st)
+        sim.step_n(1)
+        e1.record(st)
+        e1.synchronize()
+        out.append(e0.elapsed_time(e1) / 1000.0)
+    return float(np.median(out))
+
+
 def time_plain(sim, steps, warmup):
     """Time ``steps`` fixed-dt RK2 steps; returns (seconds, launches)."""
     import torch
@@ -795,11 +817,16 @@ def main():
     value = cells * a.steps / wall / 1e6
     avg_launch = ev / launches
     from SWESimulators import autotune
-    tile_name = autotune.tile_config_names()[sim.tile_config if sim.tile_config >= 0 else 1]
+    from SWESimulators import _gocean as G
+    cfg = sim.tile_config if sim.tile_config >= 0 else G.load().gocean_default_tile_config(inp.nx, inp.ny)
+    tile_name = autotune.tile_config_names()[cfg]
     achieved = 0.5 * BYTES_PER_CELL_STEP * cells / avg_launch / 1e9
     log(f"[bench] plain: {value:.1f} Mcell-steps/s, event time/launch {avg_launch*1e6:.1f} us")
 
+    iso = time_isolated_step(sim)
+    log(f"[bench] one isolated RK2 step: {iso*1e6:.1f} us ({iso*0.5e6:.1f} us per launch)")
     extra = {"substep_launch_us": round(avg_launch * 1e6, 2),
+             "isolated_step_launch_us": round(iso * 0.5e6, 2),
              "valu": measured_valu(cells, tile_name, avg_launch)}
     if not a.no_ensemble:
         log("[bench] batched ensemble C3")

@@ -515,6 +515,8 @@ int gocean_adjoint_set_record_tile_config(int cfg) {
 
 int gocean_adjoint_record_tile_config(void) { return g_record_cfg; }
 
+int gocean_default_tile_config(int nx, int ny) { return default_tile_config((long long)nx * ny); }
+
 int gocean_tile_configs(const char** names, int max) {
     for (int k = 0; k < kNumTileConfigs && k < max; ++k) names[k] = kTileConfigs[k].name;
     return kNumTileConfigs;

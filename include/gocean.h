@@ -167,6 +167,10 @@ int gocean_adjoint_record_tile_config(void);
 /* Names of the tile configurations the library is built with; returns the
  * count and writes up to max entries of "BWxBH/threads" into names. */
 int gocean_tile_configs(const char** names, int max);
+/* The configuration a tile_config of -1 selects for an nx x ny domain (or an
+ * ensemble whose members average nx*ny cells): the column-marching kernel
+ * from a million cells on, the 32x16/512 tile below. */
+int gocean_default_tile_config(int nx, int ny);
 
 /* ------------------------------------------------------------------------
  * Batched ensembles.  The reference steps ensemble members one at a time
