@@ -71,6 +71,8 @@ PROTOTYPES = [
      [ctypes.POINTER(Domain), c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
     ("gocean_default_tile_config", c_int, [c_int, c_int]),
     ("gocean_adjoint_set_record_tile_config", c_int, [c_int]),
+    ("gocean_adjoint_set_reverse_kernel", c_int, [c_int]),
+    ("gocean_adjoint_reverse_kernel", c_int, []),
     ("gocean_adjoint_record_tile_config", c_int, []),
 ]
 

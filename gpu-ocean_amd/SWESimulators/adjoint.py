@@ -109,8 +109,7 @@ class RK2Adjoint(object):
         Ci = np.where(C < 2, C + nx, np.where(C > nx + 1, C - nx, C))
         o = R * pitch + C
         mk = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=self.dev)
-        self.wrap_src = mk(o[written])
-        self.wrap_dst = mk((Ri * pitch + Ci)[written])
+        self.wrap = [(mk(a), mk(b)) for a, b in _unique_dst_groups(o[written], (Ri * pitch + Ci)[written])]
         self.keep = mk(o[ghost & ~written])
 
     def _enter(self):
@@ -225,7 +224,8 @@ class RK2Adjoint(object):
             L = torch.stack([a.detach().to(torch.float32) for a in lam])
             Lf = L.view(3, -1)
             keep = Lf[:, self.keep]
-            Lf.index_add_(1, self.wrap_dst, Lf[:, self.wrap_src])
+            for src, dst in self.wrap:
+                Lf.index_add_(1, dst, Lf[:, src])
             cur = tuple(L.unbind(0))
             nxt = tuple(torch.empty_like(L).unbind(0))
             n_seg = len(self.checkpoints)
@@ -302,6 +302,25 @@ def simulator_adjoint(sim, checkpoint_every=50):
                       tile_config=sim.tile_config)
 
 
+def _unique_dst_groups(src, dst):
+    """Split (src, dst) index pairs, in their order, into groups with distinct
+    destinations.  On a doubly periodic domain an interior cell next to a
+    corner is the image of three ghost cells; one index_add_ over all pairs
+    would sum them in whatever order the device's atomics land, so the
+    folded adjoint would change in its last bits from run to run.  Applied
+    one group after another, every destination takes its adds in a fixed
+    order."""
+    groups = []
+    for a, b in zip(np.asarray(src).tolist(), np.asarray(dst).tolist()):
+        for g in groups:
+            if b not in g[2]:
+                g[0].append(a); g[1].append(b); g[2].add(b)
+                break
+        else:
+            groups.append(([a], [b], {b}))
+    return [(np.asarray(g[0], np.int64), np.asarray(g[1], np.int64)) for g in groups]
+
+
 def _ghost_index(nx, ny, pitch, bc_north, bc_east):
     """Flat indices within one padded plane: output ghost cells the forward
     rewrites (periodic images) with the interior cell each copies, and the
@@ -353,7 +372,9 @@ class EnsembleAdjoint(object):
                 off = int(ens.state_off[k]) + f * ens.planes[k]
                 src.append(a + off); dst.append(b + off); keep.append(c + off)
         mk = lambda parts: torch.as_tensor(np.concatenate(parts), device=dev)
-        self.wrap_src, self.wrap_dst, self.keep = mk(src), mk(dst), mk(keep)
+        self.wrap = [(torch.as_tensor(a, device=dev), torch.as_tensor(b, device=dev))
+                     for a, b in _unique_dst_groups(np.concatenate(src), np.concatenate(dst))]
+        self.keep = mk(keep)
         if keep_records not in (True, False, "auto"):
             raise ValueError("keep_records must be True, False or 'auto'")
         self.keep_records = keep_records
@@ -425,7 +446,8 @@ class EnsembleAdjoint(object):
         with torch.cuda.stream(self.stream):
             a = lam.detach().to(torch.float32).contiguous().clone()
             kept = a[self.keep].clone()
-            a.index_add_(0, self.wrap_dst, a[self.wrap_src])
+            for src, dst in self.wrap:
+                a.index_add_(0, dst, a[src])
             b = torch.empty_like(a)
             n_seg = len(self.checkpoints)
             if self.records is not None:

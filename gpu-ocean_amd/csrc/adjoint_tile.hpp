@@ -105,13 +105,15 @@ __host__ __device__ inline void slope_coef(uint32_t sel, float theta, float& kl,
 //   theta g ((E_r - E_c) + sig D f (s_c t_c + s_r t_r)),
 // or the positivity adjustment sig Dd s_a t_c +- 2g (E_c + H).  Adds
 // d/d(E, t) of cells l, c, r times gT.
+// NOADJ: the caller knows adj == 0 (the same operations without the branch)
+template <bool NOADJ = false>
 __host__ __device__ inline void depth_term_vjp(uint32_t ks, uint32_t adj, float sig, float theta, float g,
                                                float f, float Dd, float D, float r_two_g, float sl, float sc,
                                                float sr, float sa, float gT, float* gE, float* gt) {
     const float two_g = 2.0f * g;
     const float gK = gT * r_two_g;
     float el, ec, er, tl, tc, tr;
-    if (adj != 0u) {
+    if (!NOADJ && adj != 0u) {
         el = 0.0f; er = 0.0f; ec = adj == 1u ? two_g : -two_g;
         tl = 0.0f; tr = 0.0f; tc = sig * Dd * sa;
     } else {
@@ -134,14 +136,17 @@ __host__ __device__ inline void depth_term_vjp(uint32_t ks, uint32_t adj, float 
 // VJP of the central-upwind flux in (depth, normal velocity, tangential
 // velocity) form: mass = F0, normal momentum = F1, tangential momentum = F2
 // (CDKLM16_flux, CDKLM16_kernel.cu:101-140), branch choices from `code`.
+// WET: the caller knows the flux is not zeroed (code bit 4 clear) and both
+// depths exceed the cutoff (the same operations without the branches)
+template <bool WET = false>
 __host__ __device__ inline void flux_vjp(float hm, float nm, float tm, float hp, float np_, float tp,
                                          uint32_t code, float g, float cutoff,
                                          float l0, float l1, float l2,
                                          float& ghm, float& gnm, float& gtm,
                                          float& ghp, float& gnp, float& gtp) {
     ghm = gnm = gtm = ghp = gnp = gtp = 0.0f;
-    if (code & 16u) return;                       // |a+ - a-| < flux_eps: zero flux
-    const bool wp = hp > cutoff, wm = hm > cutoff;
+    if (!WET && (code & 16u)) return;             // |a+ - a-| < flux_eps: zero flux
+    const bool wp = WET || hp > cutoff, wm = WET || hm > cutoff;
     float Fpx = 0.0f, Fpy = 0.0f, c_p = 0.0f, u_p = 0.0f;
     if (wp) { Fpx = hp * np_; Fpy = hp * np_ * np_ + 0.5f * g * hp * hp; u_p = np_; c_p = adj_sqrt(g * hp); }
     float Fmx = 0.0f, Fmy = 0.0f, c_m = 0.0f, u_m = 0.0f;

@@ -11,8 +11,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <vector>
 
 #include "../../include/gocean.h"
+#include "adjoint_march.hpp"
 #include "adjoint_tile.hpp"
 #include "gocean_internal.hpp"
 
@@ -23,6 +25,33 @@ __global__ __launch_bounds__(kAdjNT) void k_adj_tile(AdjTileArgs A) {
     const int t = xcd_band(blockIdx.x, gridDim.x);
     const int ty = t / A.tiles_x, tx = t - ty * A.tiles_x;
     adj_tile_body<kAdjBW, kAdjBH, kAdjNT, false>(A, S, tx, ty, threadIdx.x);
+}
+
+// column-marching form (adjoint_march.hpp): one wavefront per strip of 58
+// output columns and bh rows (padded rows 0 .. ny+3, ghost rows included)
+__global__ __launch_bounds__(256) void k_adj_march(adjm::MArgs A, int strips_x, int bh, int n_strips) {
+    const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
+    if (wave >= n_strips) return;
+    const int sy = wave / strips_x, sx = wave - sy * strips_x;
+    const int r0 = sy * bh;
+    const int r1 = min(r0 + bh, A.ny + 4) - 1;
+    adjm::adj_march_strip(A, sx, r0, r1);
+}
+
+// reverse kernel: 0 tile, 1 column-marching, -1 by size (marching from
+// 4 M cells on, where it is the faster of the two)
+int g_adj_kernel = -1;
+
+// rows per strip of k_adj_march for domains of the given widths and heights
+int adj_march_rows(const int* nx, const int* ny, int n) {
+    static int resident = 0;
+    if (!resident) resident = resident_waves((const void*)k_adj_march);
+    std::vector<int> strips(n), rows(n);
+    for (int k = 0; k < n; ++k) {
+        strips[k] = (nx[k] + adjm::kOut - 1) / adjm::kOut;
+        rows[k] = ny[k] + 4;
+    }
+    return march_rows(strips.data(), rows.data(), n, resident);
 }
 
 __global__ __launch_bounds__(256) void k_adj_fold(FoldArgs F) {
@@ -88,9 +117,54 @@ static int fwd_tape(const gocean_domain_t* d, const gocean_wind_t* wind, float d
     return launch_substep_tape(A, s);
 }
 
+// the three fields of an array as one base and a byte stride, when they are
+// equally spaced (the reverse entry points' buffers always are)
+static bool triple(const float* a, const float* b, const float* c, const char*& base, long long& stride) {
+    base = reinterpret_cast<const char*>(a);
+    if (!a) { stride = 0; return true; }
+    stride = (long long)(reinterpret_cast<const char*>(b) - base);
+    return reinterpret_cast<const char*>(c) - reinterpret_cast<const char*>(b) == stride;
+}
+
+// the marching kernel's arguments, false when an array's planes are not
+// equally spaced (then the tile kernel runs)
+bool adj_march_args(const AdjTileArgs& A, adjm::MArgs& M) {
+    const char* p;
+    long long st;
+    if (!triple(A.w_eta, A.w_hu, A.w_hv, p, st)) return false;
+    M.w = p; M.w_stride = st;
+    if (!triple(A.x_eta, A.x_hu, A.x_hv, p, st)) return false;
+    M.x = p; M.x_stride = st;
+    if (!triple(A.o_eta, A.o_hu, A.o_hv, p, st)) return false;
+    M.o = const_cast<char*>(p); M.o_stride = st;
+    const char* t = reinterpret_cast<const char*>(A.tU);
+    const long long ts = (long long)(reinterpret_cast<const char*>(A.tV) - t);
+    if (reinterpret_cast<const char*>(A.tT) != t + 2 * ts || reinterpret_cast<const char*>(A.tS) != t + 3 * ts ||
+        reinterpret_cast<const char*>(A.tC) != t + 4 * ts)
+        return false;
+    M.tape = t; M.tape_stride = ts;
+    M.eta = A.eta; M.w_clamp = A.w_clamp; M.x_clamp = A.x_clamp; M.Hi = A.Hi; M.dt_dev = A.dt_dev;
+    M.dt = A.dt; M.alpha = A.alpha; M.gamma = A.gamma; M.w_fold = A.w_fold; M.x_fold = A.x_fold;
+    M.nx = A.nx; M.ny = A.ny; M.pitch = A.pitch; M.hi_pitch = A.hi_pitch;
+    M.g = A.g; M.theta = A.theta; M.desing = A.desing; M.cutoff = A.cutoff; M.land = A.land; M.f = A.f;
+    M.bc_n = A.bc_n; M.bc_e = A.bc_e; M.bc_s = A.bc_s; M.bc_w = A.bc_w;
+    M.r_dx = A.r_dx; M.r_dy = A.r_dy; M.r_two_g = A.r_two_g; M.Dx = A.Dx; M.Dy = A.Dy;
+    M.Ddx = A.Ddx; M.Ddy = A.Ddy; M.g_dx = A.g_dx; M.g_dy = A.g_dy;
+    return true;
+}
+
 static int adj_launch(const AdjTileArgs& A, bool fold_out, hipStream_t s) {
-    const int nb = A.tiles_x * A.tiles_y;
-    hipLaunchKernelGGL(k_adj_tile, dim3(nb), dim3(kAdjNT), 0, s, A);
+    adjm::MArgs M;
+    const bool march = g_adj_kernel == 1 || (g_adj_kernel < 0 && (long long)A.nx * A.ny >= (4LL << 20));
+    if (march && adj_march_args(A, M)) {
+        const int sx = (A.nx + adjm::kOut - 1) / adjm::kOut;
+        const int bh = adj_march_rows(&A.nx, &A.ny, 1);
+        const int n = sx * ((A.ny + 4 + bh - 1) / bh);
+        hipLaunchKernelGGL(k_adj_march, dim3((n + 3) / 4), dim3(256), 0, s, M, sx, bh, n);
+    } else {
+        const int nb = A.tiles_x * A.tiles_y;
+        hipLaunchKernelGGL(k_adj_tile, dim3(nb), dim3(kAdjNT), 0, s, A);
+    }
     if (!fold_out) return check_launch("adjoint substep launch");
     FoldArgs F;
     F.e = A.o_eta; F.u = A.o_hu; F.v = A.o_hv;
@@ -129,6 +203,16 @@ static long long plane_of(const gocean_domain_t* d) { return (long long)(d->ny +
 using namespace gocean;
 
 extern "C" {
+
+int gocean_adjoint_set_reverse_kernel(int kind) {
+    clear_error();
+    if (kind < 0) kind = -1;
+    if (kind > 1) return set_error(GOCEAN_EINVAL, "reverse kernel %d: 0 (tile), 1 (column-marching), -1 (by size)", kind);
+    g_adj_kernel = kind;
+    return 0;
+}
+
+int gocean_adjoint_reverse_kernel(void) { return g_adj_kernel; }
 
 long long gocean_adjoint_scratch_floats(const gocean_domain_t* d) {
     if (!d) return -1;

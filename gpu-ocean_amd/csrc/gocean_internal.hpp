@@ -48,5 +48,9 @@ int max_dt_into(const gocean_domain_t* d, const float* eta, const float* hu, con
                 const float* Hm, float courant, float* partial, float* out, hipStream_t s);
 int max_dt(const gocean_domain_t* d, const float* eta, const float* hu, const float* hv,
            const float* Hm, float courant, float* out, hipStream_t s);
+// substep-adjoint kernel of the reverse entry points (0 tile, 1 marching)
+// and the marching kernel's rows per strip for domains of these sizes
+extern int g_adj_kernel;
+int adj_march_rows(const int* nx, const int* ny, int n);
 
 }  // namespace gocean

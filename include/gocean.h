@@ -164,6 +164,14 @@ int gocean_max_dt_static(const gocean_domain_t* d, const float* eta, const float
 int gocean_adjoint_set_record_tile_config(int cfg);
 int gocean_adjoint_record_tile_config(void);
 
+/* Kernel of the substep adjoint in the single-domain reverse entry points:
+ * 0 = one 16x16 tile per workgroup through LDS, 1 = column-marching
+ * wavefronts, -1 (the default) = by size, the marching kernel from 4 M cells
+ * on.  Both give the same bits.  The ensemble entry points use the tile
+ * kernel.  Process-wide; no counterpart in the reference. */
+int gocean_adjoint_set_reverse_kernel(int kind);
+int gocean_adjoint_reverse_kernel(void);
+
 /* Names of the tile configurations the library is built with; returns the
  * count and writes up to max entries of "BWxBH/threads" into names. */
 int gocean_tile_configs(const char** names, int max);

@@ -40,3 +40,26 @@ def test_ghost_index_rules():
                     if r < 2 or r > ny + 1 or c < 2 or c > nx + 1)
         assert len(src) + len(keep) == ghost
         assert len(set(src.tolist()) & set(keep.tolist())) == 0
+
+
+def test_wrap_groups_have_distinct_destinations():
+    """The periodic fold runs as index_add_ groups whose destinations are
+    distinct (so the device's atomic adds cannot reorder a sum); together
+    the groups hold every (ghost, image) pair once, and on a doubly periodic
+    domain a cell next to a corner takes its three images in three groups."""
+    nx, ny, pitch = 7, 5, 13
+    for bc_n, bc_e in [(1, 1), (2, 2), (2, 1), (1, 2)]:
+        src, dst, keep = A._ghost_index(nx, ny, pitch, bc_n, bc_e)
+        groups = A._unique_dst_groups(src, dst)
+        pairs = []
+        for a, b in groups:
+            assert len(set(b.tolist())) == len(b)
+            pairs += list(zip(a.tolist(), b.tolist()))
+        assert sorted(pairs) == sorted(zip(src.tolist(), dst.tolist()))
+        if (bc_n, bc_e) == (2, 2):
+            assert len(groups) == 3
+            assert sum(1 for a, b in groups for d in b.tolist() if d == 2 * pitch + 2) == 3
+        elif (bc_n, bc_e) == (1, 1):
+            assert groups == []
+        else:
+            assert len(groups) == 1

@@ -1,0 +1,96 @@
+"""The column-marching substep adjoint against the tile adjoint: both kernels
+evaluate the same vector-Jacobian products with the same operations in the
+same order, so the gradients of whole checkpointed RK2 runs must agree bit
+for bit (values below 2^-126 taken as zero, as the parity tests do).  The
+domains put strip edges on every boundary case of the marching kernel
+(58 output columns per strip: nx = 56, 58, 59, 116, 117), short and tall row
+strips, land, near-dry cells that the forward clamps, every boundary
+condition, and the ghost-cell fold between substeps (several steps per
+checkpoint segment)."""
+
+import numpy as np
+import pytest
+import torch
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+BCS = [("wall", (1, 1, 1, 1)), ("periodic", (2, 2, 2, 2)),
+       ("periodicNS", (2, 1, 2, 1)), ("periodicEW", (1, 2, 1, 2))]
+SIZES = [(37, 29), (56, 21), (58, 40), (59, 33), (116, 9), (117, 70)]
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _mods():
+    from SWESimulators import Common
+    from SWESimulators import _gocean as G
+    from SWESimulators import adjoint as A
+    return Common, G, A
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def _grad(kernel, p, Hi, q, W, dt, n, K, keep):
+    Common, G, A = _mods()
+    lib = G.load()
+    G.check(lib.gocean_adjoint_set_reverse_kernel(kernel), "set reverse kernel")
+    d = Common.make_domain(p.nx, p.ny, p.dx, p.dy, p.g, 0.0, p.theta, 2, p.bc, f=float(p.f),
+                           north_x=0.0, north_y=1.0)
+    adj = A.RK2Adjoint(d, _dev(Hi), checkpoint_every=K, keep_records=keep)
+    qd = [_dev(a).requires_grad_(True) for a in q]
+    out = A.CDKLM16RK2.apply(*qd, adj, n, dt)
+    sum((w * o).sum() for w, o in zip(W, out)).backward()
+    torch.cuda.synchronize()
+    return [x.grad.cpu().numpy() for x in qd]
+
+
+def _compare(p, Hi, q, dt, n, K, seed, keep=True):
+    rng = np.random.default_rng(seed)
+    W = [_dev(rng.standard_normal(q[0].shape)) for _ in range(3)]
+    try:
+        a = _grad(0, p, Hi, q, W, dt, n, K, keep)
+        b = _grad(1, p, Hi, q, W, dt, n, K, keep)
+    finally:
+        _, G, _ = _mods()
+        G.check(G.load().gocean_adjoint_set_reverse_kernel(-1), "reset")
+        assert G.load().gocean_adjoint_reverse_kernel() == -1
+    for name, x, y in zip(("eta", "hu", "hv"), a, b):
+        assert np.isfinite(x).all() and np.abs(x).max() > 0, name
+        if not H.normal_equal(x, y):
+            bad = np.argwhere(x != y)
+            raise AssertionError(f"{name}: tile and marching adjoints differ at {len(bad)} cells, first "
+                                 f"{H.diff_report(x, y, 8)}")
+
+
+@pytest.mark.parametrize("bcname,bc", BCS)
+@pytest.mark.parametrize("size", SIZES)
+def test_marching_adjoint_equals_tile_adjoint(bcname, bc, size):
+    nx, ny = size
+    _say("marching vs tile adjoint", bcname, nx, ny)
+    p, Hi, Hm, eta, hu, hv = H.random_case(nx, ny, bc, 11, land=(nx + ny) % 2 == 0)
+    _compare(p, Hi, [eta, hu, hv], 3.0, 4, 2, seed=nx * 100 + ny)
+
+
+@pytest.mark.parametrize("bcname,bc", [BCS[0], BCS[3]])
+def test_marching_adjoint_equals_tile_adjoint_near_dry(bcname, bc):
+    """A shallow strip and a step long enough to clamp cells dry: the clamp
+    masks of lambda and of the extra term, and the desingularised and dry
+    conversion branches."""
+    _say("marching vs tile adjoint, near-dry", bcname)
+    p, Hi, Hm, eta, hu, hv = H.random_case(61, 27, bc, 4, shallow=True)
+    hu = hu.copy()
+    hu[:, :12] += 0.05
+    _compare(p, Hi, [eta, hu, hv], 20.0, 3, 3, seed=3)
+
+
+def test_marching_adjoint_equals_tile_adjoint_recomputed_records():
+    """Records recomputed per segment instead of kept."""
+    _say("marching vs tile adjoint, recomputed records")
+    p, Hi, Hm, eta, hu, hv = H.random_case(117, 45, (2, 1, 2, 1), 8)
+    _compare(p, Hi, [eta, hu, hv], 3.0, 5, 2, seed=9, keep=False)
