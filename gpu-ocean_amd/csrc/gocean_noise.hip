@@ -37,6 +37,7 @@ constexpr int kRngNT = 256;
 constexpr int kSoarBW = 32, kSoarBH = 8;   // SOAR tile over the coarse buffer
 constexpr int kBalBW = 64, kBalBH = 4;     // balance tile over fine interior cells
 constexpr int kFuBW = 64, kFuBH = 16;      // fused SOAR + balance tile (factor 1)
+constexpr int kFuNT = 256;                 // its threads: four rows of the tile each
 constexpr int kNormNT = 1024;
 
 struct NoiseMember {
@@ -51,6 +52,11 @@ struct NoiseMember {
     float w[(2 * kCut + 1) * (2 * kCut + 1)];   // (1 + d/L) exp(-d/L), row-major (dy, dx)
     int rng_begin, soar_begin, soar_tiles_x, bal_begin, bal_tiles_x, fu_begin, fu_tiles_x;
     int id;
+    // set on the host from the fields above (fill_member): g / f when the
+    // Coriolis parameter is one constant (beta 0, no f field), else 0; 2 dx,
+    // 2 dy and their reciprocals rounded to float (RN(1/d)), for correctly
+    // rounded divisions without the generic sequence
+    float gc_const, two_dx, two_dy, r_two_dx, r_two_dy;
 };
 
 struct NoiseLaunch {
@@ -414,38 +420,61 @@ __global__ __launch_bounds__(kBalBW * kBalBH) void k_noise_balance(NoiseLaunch L
 
 // SOAR and geostrophic balance in one launch for interpolation factor 1
 // (the SOAR kernel followed by geostrophicBalance, ocean_noise.cu:264-505):
-// the block computes d_eta for its 64x4 tile and a 2-cell ring in LDS from
+// the block computes d_eta for its 64x16 tile and a 2-cell ring in LDS from
 // the random field (2-cell SOAR halo on top), writes the coarse-buffer cells
 // it owns (the tile's interior, plus the ghost rings next to it on the
 // domain's edges: each coarse cell once), and balances its interior cells.
+// Each thread balances four cells of a column of the tile; their depths and
+// state are loaded before the first barrier, so that those loads overlap the
+// random-field and SOAR stages.  The divisions by 2 dx, 2 dy and (constant
+// Coriolis) f are correctly rounded with reciprocals formed on the host, as
+// the generic division rounds them.
 template <bool BATCH>
-__global__ __launch_bounds__(kFuBW * kFuBH) void k_noise_fused1(NoiseLaunch L, float q0_scale,
-                                                                  float scale) {
+__global__ __launch_bounds__(kFuNT) void k_noise_fused1(NoiseLaunch L, float q0_scale, float scale) {
     constexpr int DW = kFuBW + 4, DH = kFuBH + 4;             // d_eta: tile + 2-cell ring
     constexpr int RW = DW + 2 * kCut, RH = DH + 2 * kCut;       // random values: + SOAR halo
     constexpr int NW = (2 * kCut + 1) * (2 * kCut + 1);
+    constexpr int RPT = kFuBH * kFuBW / kFuNT;                  // cells per thread
     __shared__ float xi[RH][RW + 1];
     __shared__ float de[DH][DW + 1];
     const NoiseMember& M = member_of<BATCH>(L);
-    float w[NW];
-#pragma unroll
-    for (int k = 0; k < NW; ++k) w[k] = M.w[k];
     const int lt = blockIdx.x - M.fu_begin;
     const int by = 2 + (lt / M.fu_tiles_x) * kFuBH;
     const int bx = 2 + (lt % M.fu_tiles_x) * kFuBW;
     const int ay0 = by - 2, ax0 = bx - 2;     // coarse (= fine, ghosted) index of de[0][0]
+    const int tx = threadIdx.x % kFuBW, ty0 = threadIdx.x / kFuBW;
+
+    // the balance inputs of this thread's cells (rows ty0, ty0 + 4, ...)
+    const float* Hi = L.Hi + M.hi_off;
+    const int ti = bx + tx;
+    float H00[RPT], H01[RPT], H10[RPT], H11[RPT], e0[RPT], u0[RPT], v0[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int tj = by + ty0 + r * (kFuNT / kFuBW);
+        H00[r] = H01[r] = H10[r] = H11[r] = e0[r] = u0[r] = v0[r] = 0.0f;
+        if (ti < M.nx + 2 && tj < M.ny + 2) {
+            const long long h0 = (long long)tj * M.hi_pitch + ti;
+            H00[r] = Hi[h0]; H01[r] = Hi[h0 + 1]; H10[r] = Hi[h0 + M.hi_pitch]; H11[r] = Hi[h0 + M.hi_pitch + 1];
+            const long long st = (long long)tj * M.pitch + ti;
+            e0[r] = L.eta[M.eta_off + st]; u0[r] = L.hu[M.hu_off + st]; v0[r] = L.hv[M.hv_off + st];
+        }
+    }
+
     const float* rnd = L.random + M.rand_off;
-    for (int k = threadIdx.x; k < RH * RW; k += kFuBW * kFuBH) {
+    for (int k = threadIdx.x; k < RH * RW; k += kFuNT) {
         const int j = k / RW, i = k - j * RW;
         const int gy = M.periodic_ns ? pmod(ay0 + j - 2 * kCut, M.cny) : min(ay0 + j, M.rand_ny - 1);
         const int gx = M.periodic_ew ? pmod(ax0 + i - 2 * kCut, M.cnx) : min(ax0 + i, M.rand_nx - 1);
         xi[j][i] = rnd[(long long)gy * M.rand_nx + gx];
     }
     __syncthreads();
+    float w[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) w[k] = M.w[k];
     const float q = M.q0 * q0_scale;
     const int cp = M.cnx + 4;
     float* C = L.coarse + M.coarse_off;
-    for (int k = threadIdx.x; k < DH * DW; k += kFuBW * kFuBH) {
+    for (int k = threadIdx.x; k < DH * DW; k += kFuNT) {
         const int j = k / DW, i = k - j * DW;
         const int ay = ay0 + j, ax = ax0 + i;
         if (ay > M.cny + 3 || ax > M.cnx + 3) continue;
@@ -462,13 +491,43 @@ __global__ __launch_bounds__(kFuBW * kFuBH) void k_noise_fused1(NoiseLaunch L, f
         if (oy >= by && oy < by + kFuBH && ox >= bx && ox < bx + kFuBW) C[(long long)ay * cp + ax] = v;
     }
     __syncthreads();
-    const int tx = threadIdx.x % kFuBW, ty = threadIdx.x / kFuBW;
-    const int ti = bx + tx, tj = by + ty;
-    if (ti >= M.nx + 2 || tj >= M.ny + 2) return;
-    const float d = de[ty + 2][tx + 2];
-    const float ex = (de[ty + 2][tx + 3] - de[ty + 2][tx + 1]) / (2.0f * M.dx);
-    const float ey = (de[ty + 3][tx + 2] - de[ty + 1][tx + 2]) / (2.0f * M.dy);
-    balance_cell(L, M, ti, tj, d, ex, ey, false);
+    if (ti >= M.nx + 2) return;
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+        const int ty = ty0 + r * (kFuNT / kFuBW), tj = by + ty;
+        if (tj >= M.ny + 2) break;
+        const float d = de[ty + 2][tx + 2];
+        const float ex = div_r(de[ty + 2][tx + 3] - de[ty + 2][tx + 1], M.two_dx, M.r_two_dx);
+        const float ey = div_r(de[ty + 3][tx + 2] - de[ty + 1][tx + 2], M.two_dy, M.r_two_dy);
+        // balance (geostrophicBalance, ocean_noise.cu:443-504)
+        const bool dry = H00[r] == M.land || H01[r] == M.land || H10[r] == M.land || H11[r] == M.land;
+        const float Hmid = 0.25f * (H00[r] + H01[r] + H10[r] + H11[r]);
+        float gc = M.gc_const;
+        if (gc == 0.0f) {
+            const long long st = (long long)tj * M.pitch + ti;
+            float nxv = M.north_x, nyv = M.north_y;
+            if (M.nxf_off >= 0) {
+                nxv = L.fnx[M.nxf_off + st];
+                nyv = L.fny[M.nyf_off + st];
+            }
+            float cor = M.f + M.beta * (((float)ti + 0.5f) * M.dx * nxv + ((float)tj + 0.5f) * M.dy * nyv);
+            if (M.f == 0.0f) cor = L.ff[M.f_off + st];
+            gc = M.g / cor;
+        }
+        const float hm = d + Hmid + e0[r];
+        const float dhu = -gc * hm * ey;
+        const float dhv = gc * hm * ex;
+        const long long st = (long long)tj * M.pitch + ti;
+        if (dry) {
+            L.eta[M.eta_off + st] = 0.0f;
+            L.hu[M.hu_off + st] = 0.0f;
+            L.hv[M.hv_off + st] = 0.0f;
+        } else {
+            L.eta[M.eta_off + st] = e0[r] + d;
+            L.hu[M.hu_off + st] = u0[r] + dhu;
+            L.hv[M.hv_off + st] = v0[r] + dhv;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -596,6 +655,12 @@ void fill_member(const gocean_noise_t* nz, const Sizes& S, NoiseMember& M) {
     M.rand_ny = S.rny;
     M.seed_nx = S.snx;
     M.seed_ny = S.sny;
+    // constant Coriolis parameter: beta 0 makes f + beta*(...) exactly f
+    M.gc_const = (nz->beta == 0.0f && nz->f != 0.0f) ? nz->g / nz->f : 0.0f;
+    M.two_dx = 2.0f * nz->dx;
+    M.two_dy = 2.0f * nz->dy;
+    M.r_two_dx = 1.0f / M.two_dx;
+    M.r_two_dy = 1.0f / M.two_dy;
     const double L = nz->soar_L;
     for (int oy = -kCut; oy <= kCut; ++oy)
         for (int ox = -kCut; ox <= kCut; ++ox) {
@@ -772,7 +837,7 @@ int gocean_noise_soar_balance(const gocean_noise_t* nz, float q0, float scale, c
         L.fny = fields->north_y;
         L.one.nxf_off = L.one.nyf_off = 0;
     }
-    hipLaunchKernelGGL(k_noise_fused1<false>, dim3(tx * ty), dim3(kFuBW * kFuBH), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(k_noise_fused1<false>, dim3(tx * ty), dim3(kFuNT), 0, (hipStream_t)stream,
                        L, 1.0f, scale);
     return check_launch("k_noise_fused1");
 }
@@ -971,7 +1036,7 @@ int gocean_noise_batch_perturb(gocean_noise_batch_t* B, float* state, const floa
     L.ff = L.fnx = L.fny = fields;
     if (B->all_f1) {
         L.map = B->d_fu_map;
-        hipLaunchKernelGGL(k_noise_fused1<true>, dim3(B->n_fu), dim3(kFuBW * kFuBH), 0, s, L, q0_scale,
+        hipLaunchKernelGGL(k_noise_fused1<true>, dim3(B->n_fu), dim3(kFuNT), 0, s, L, q0_scale,
                            perturbation_scale);
         return check_launch("model error (batch, fused)");
     }

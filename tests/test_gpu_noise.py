@@ -517,24 +517,30 @@ def test_graph_replays_with_model_error_match_eager():
         assert not torch.equal(got[0], s0[0])
 
 
-@pytest.mark.parametrize("nx,ny", [(45, 38), (130, 9)])
+@pytest.mark.parametrize("nx,ny", [(45, 38), (130, 9), (131, 40)])
 @pytest.mark.parametrize("pns,pew", BCS)
-def test_fused_soar_balance_equals_two_launches(nx, ny, pns, pew):
+@pytest.mark.parametrize("coriolis", ["fields", "constant"])
+def test_fused_soar_balance_equals_two_launches(nx, ny, pns, pew, coriolis):
     """gocean_noise_soar_balance == gocean_noise_soar then
     gocean_noise_balance, bit for bit: the state and every coarse-buffer cell
-    (interior and both ghost rings), with land and per-cell fields."""
+    (interior and both ghost rings), with land, and with a beta plane and
+    per-cell fields or one constant Coriolis parameter (whose g / f the fused
+    kernel takes from the host)."""
     G, lib = _lib()
     torch = _torch()
-    _say("fused soar+balance", nx, ny, pns, pew)
+    _say("fused soar+balance", nx, ny, pns, pew, coriolis)
     rs = np.random.RandomState(9)
     H, eta, hu, hv = _state(rs, nx, ny, [(5, 6), (ny, nx // 2)])
     cnx, cny, rnx, rny, *_ = O.noise_sizes(nx, ny, 1, pns, pew)
     xi = _dev(rs.randn(rny, rnx).astype(np.float32))
-    d = _desc(nx, ny, 1, pns, pew, beta=2e-6)
-    ang = 0.2 * rs.rand(ny + 4, nx + 4)
-    keep = [_dev(a.astype(np.float32)) for a in (F + 1e-3 * rs.rand(ny + 4, nx + 4), np.sin(ang), np.cos(ang))]
     fields = G.Fields()
-    fields.f, fields.north_x, fields.north_y = (t.data_ptr() for t in keep)
+    if coriolis == "fields":
+        d = _desc(nx, ny, 1, pns, pew, beta=2e-6)
+        ang = 0.2 * rs.rand(ny + 4, nx + 4)
+        keep = [_dev(a.astype(np.float32)) for a in (F + 1e-3 * rs.rand(ny + 4, nx + 4), np.sin(ang), np.cos(ang))]
+        fields.f, fields.north_x, fields.north_y = (t.data_ptr() for t in keep)
+    else:
+        d = _desc(nx, ny, 1, pns, pew)
     q0 = float(np.float32(d.soar_q0 * 0.9))
     outs = []
     for fused in (False, True):
