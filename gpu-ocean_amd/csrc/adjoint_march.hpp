@@ -52,7 +52,7 @@ struct MArgs {
     char* o;
     const float* Hi;
     const float* dt_dev;
-    long long tape_stride, w_stride, x_stride, o_stride;
+    uint32_t tape_stride, w_stride, x_stride, o_stride;   // bytes; every offset stays below 4 GiB
     float dt, alpha, gamma;
     int w_fold, x_fold;
     int nx, ny, pitch, hi_pitch;
@@ -61,13 +61,16 @@ struct MArgs {
     float r_dx, r_dy, r_two_g, Dx, Dy, Ddx, Ddy, g_dx, g_dy;
 };
 
-__device__ __forceinline__ float ldb(const void* base, long long plane_off, uint32_t row_off, uint32_t lane_off) {
-    const char* p = reinterpret_cast<const char*>(base) + plane_off + row_off;
-    return *reinterpret_cast<const float*>(p + lane_off);
+// base + a 32-bit byte offset (plane + row + column): one scalar base per
+// array and the offset in a vector register, the global_load form with a
+// scalar base
+__device__ __forceinline__ float ldb(const void* base, uint32_t plane_off, uint32_t row_off, uint32_t lane_off) {
+    const uint32_t off = lane_off + (row_off + plane_off);
+    return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + off);
 }
-__device__ __forceinline__ void stb(void* base, long long plane_off, uint32_t row_off, uint32_t lane_off, float v) {
-    char* p = reinterpret_cast<char*>(base) + plane_off + row_off;
-    *reinterpret_cast<float*>(p + lane_off) = v;
+__device__ __forceinline__ void stb(void* base, uint32_t plane_off, uint32_t row_off, uint32_t lane_off, float v) {
+    const uint32_t off = lane_off + (row_off + plane_off);
+    *reinterpret_cast<float*>(reinterpret_cast<char*>(base) + off) = v;
 }
 
 // one padded row as a lane holds it
@@ -100,6 +103,13 @@ struct CentreOut {
     V3 lo, mid, hi;
 };
 
+// the loaded words of one row, before the conversion branch and the clamp
+// mask (fetched one iteration ahead of their use)
+struct Raw {
+    float eta, U, V, T, S, W0, W1, W2, wc;
+    uint32_t tc;
+};
+
 // pipeline state carried into iteration k (see adj_march_strip): row k-1's
 // y sum is ((face (k-2|k-1) plus + face (k-1|k) minus) + (centre k-2 hi +
 // centre k-1 mid)) + centre k lo, and the two inner sums are formed in
@@ -108,7 +118,9 @@ struct Carry {
     V3 p1, p2;                              // row k-1's inner sums
     V3 dp, cp, c6;                          // face (k-1|k): plus direct, plus centre terms; centre k-1 hi
     float Suy, Svy;                         // y slopes of row k
-    float phc, phn;                         // corners of row k+2
+    float phc, phn;                         // corners of row k+2 (flagged)
+    Raw raw;                                // row k+2 as loaded
+    float qhc, qhn;                         // corners of row k+3 as loaded
 };
 
 struct Strip {
@@ -145,10 +157,20 @@ struct Strip {
     // corner depths Hi[R][C], Hi[R][C+1] with land flagged (0 outside, as
     // stage_hi leaves them)
     __device__ void corners(int R, float& hc, float& hn) const {
-        hc = 0.0f; hn = 0.0f;
+        float a, b;
+        corners_raw(R, a, b);
+        corners_flag(R, a, b, hc, hn);
+    }
+    __device__ void corners_raw(int R, float& a, float& b) const {
+        a = 0.0f; b = 0.0f;
         if (R < 0 || R > A.ny + 4) return;
         const uint32_t ro = (uint32_t)R * hrowb;
-        const float a = ldb(A.Hi, 0, ro, ohc), b = ldb(A.Hi, 0, ro, ohn);
+        a = ldb(A.Hi, 0, ro, ohc);
+        b = ldb(A.Hi, 0, ro, ohn);
+    }
+    __device__ void corners_flag(int R, float a, float b, float& hc, float& hn) const {
+        hc = 0.0f; hn = 0.0f;
+        if (R < 0 || R > A.ny + 4) return;
         if (C >= 0 && C <= A.nx + 4) hc = hflag(a);
         if (C + 1 >= 0 && C + 1 <= A.nx + 4) hn = hflag(b);
     }
@@ -162,9 +184,9 @@ struct Strip {
 
     // field `field` of a 3-plane array at (R, C) plus the ghost entries whose
     // image it is (AdjTile::fold_read)
-    __device__ float fold_read(const char* base, long long stride, int field, int R) const {
+    __device__ float fold_read(const char* base, uint32_t stride, int field, int R) const {
         const int nx = A.nx, ny = A.ny;
-        const long long po = field * stride;
+        const uint32_t po = (uint32_t)field * stride;
         const uint32_t rr = (uint32_t)R * rowb;
         float s = ldb(base, po, rr, oc);
         if (R <= 3) {
@@ -190,12 +212,10 @@ struct Strip {
         return s;
     }
 
-    // row R of the cell data (stage_cells); a, c = corners of row R (its own
-    // hc, hn), b, d = corners of row R+1
-    __device__ void load(int R, float a, float c, float b, float d, Row& o) const {
-        o.E = 0.0f; o.U = 0.0f; o.V = 0.0f; o.code = 0u; o.T = 0.0f; o.S = 0.0f;
-        o.W0 = 0.0f; o.W1 = 0.0f; o.W2 = 0.0f;
-        o.hc = a; o.hn = c;
+    // the loads of row R of the cell data (stage_cells)
+    __device__ void fetch(int R, Raw& q) const {
+        q.eta = 0.0f; q.U = 0.0f; q.V = 0.0f; q.T = 0.0f; q.S = 0.0f;
+        q.W0 = 0.0f; q.W1 = 0.0f; q.W2 = 0.0f; q.wc = 0.0f; q.tc = 0u;
         const int nx = A.nx, ny = A.ny;
         if (R < 0 || R > ny + 3) return;                 // uniform
         const bool ri = !ghost_r(R);
@@ -204,15 +224,38 @@ struct Strip {
         if (R < 2) sr = (A.bc_s == kWall) ? 3 - R : R + ny;
         else if (R > ny + 1) sr = (A.bc_n == kWall) ? 2 * ny + 3 - R : R - ny;
         const uint32_t ro = (uint32_t)R * rowb;
-        const float eta = ldb(A.eta, 0, (uint32_t)sr * rowb, osc);
-        const long long ts = A.tape_stride;
-        o.U = ldb(A.tape, 0, ro, oc);
-        o.V = ldb(A.tape, ts, ro, oc);
+        q.eta = ldb(A.eta, 0, (uint32_t)sr * rowb, osc);
+        const uint32_t ts = A.tape_stride;
+        q.U = ldb(A.tape, 0, ro, oc);
+        q.V = ldb(A.tape, ts, ro, oc);
         const bool xrec = ri && C >= 1 && C <= nx + 2, yrec = ci && R >= 1 && R <= ny + 2;
-        uint32_t tc = 0u;
-        if (xrec || yrec) tc = __float_as_uint(ldb(A.tape, 4 * ts, ro, oc));
-        if (xrec) o.T = ldb(A.tape, 2 * ts, ro, oc);
-        if (yrec) o.S = ldb(A.tape, 3 * ts, ro, oc);
+        if (xrec || yrec) q.tc = __float_as_uint(ldb(A.tape, 4 * ts, ro, oc));
+        if (xrec) q.T = ldb(A.tape, 2 * ts, ro, oc);
+        if (yrec) q.S = ldb(A.tape, 3 * ts, ro, oc);
+        if (ri && ci) {
+            if (A.w_clamp) q.wc = ldb(A.w_clamp, 0, ro, oc);
+            const uint32_t ws = A.w_stride;
+            q.W0 = ldb(A.w, 0, ro, oc); q.W1 = ldb(A.w, ws, ro, oc); q.W2 = ldb(A.w, 2 * ws, ro, oc);
+        }
+    }
+
+    // row R from its loaded words: conversion branch, code word, clamp mask
+    // and fold of lambda; a, c = corners of row R (its own hc, hn), b, d =
+    // corners of row R+1
+    __device__ void finish(int R, const Raw& q, float a, float c, float b, float d, Row& o) const {
+        o.E = 0.0f; o.U = 0.0f; o.V = 0.0f; o.code = 0u; o.T = 0.0f; o.S = 0.0f;
+        o.W0 = 0.0f; o.W1 = 0.0f; o.W2 = 0.0f;
+        o.hc = a; o.hn = c;
+        const int nx = A.nx, ny = A.ny;
+        if (R < 0 || R > ny + 3) return;                 // uniform
+        const bool ri = !ghost_r(R);
+        if (!col_ok || (!ri && !ci)) return;
+        const float eta = q.eta;
+        o.U = q.U;
+        o.V = q.V;
+        const bool xrec = ri && C >= 1 && C <= nx + 2, yrec = ci && R >= 1 && R <= ny + 2;
+        o.T = q.T;
+        o.S = q.S;
         // conversion branch (cell_hm: the forward's h = eta + Hm)
         const float wet = 4.0f - (float)(a == kDryFlag) - (float)(b == kDryFlag) - (float)(c == kDryFlag)
                           - (float)(d == kDryFlag);
@@ -226,24 +269,30 @@ struct Strip {
             else { o.E = eta; br = kBrDesing; }
         } else { o.E = eta; br = kBrWet; }
         uint32_t code = br << 28;
-        if (xrec) code |= tc & 0x001F00FFu;
-        if (yrec) code |= tc & 0x03E0FF00u;
+        if (xrec) code |= q.tc & 0x001F00FFu;
+        if (yrec) code |= q.tc & 0x03E0FF00u;
         o.code = code;
         if (ri && ci) {
             bool clamped = false;
             if (A.w_clamp) {
                 const float Hreg = 0.25f * (((a + b) + c) + d);
-                clamped = ldb(A.w_clamp, 0, ro, oc) <= -Hreg + A.cutoff;
+                clamped = q.wc <= -Hreg + A.cutoff;
             }
             if (!clamped) {
-                const long long ws = A.w_stride;
+                const uint32_t ws = A.w_stride;
                 if (A.w_fold && near_boundary(R)) {
                     o.W0 = fold_read(A.w, ws, 0, R); o.W1 = fold_read(A.w, ws, 1, R); o.W2 = fold_read(A.w, ws, 2, R);
                 } else {
-                    o.W0 = ldb(A.w, 0, ro, oc); o.W1 = ldb(A.w, ws, ro, oc); o.W2 = ldb(A.w, 2 * ws, ro, oc);
+                    o.W0 = q.W0; o.W1 = q.W1; o.W2 = q.W2;
                 }
             }
         }
+    }
+
+    __device__ void load(int R, float a, float c, float b, float d, Row& o) const {
+        Raw q;
+        fetch(R, q);
+        finish(R, q, a, c, b, d, o);
     }
 
     // ---- x direction of row R (interior rows): the gathered d/d(E, u, v)
@@ -439,7 +488,7 @@ struct Strip {
                 bool clamped = false;
                 if (A.x_clamp) clamped = ldb(A.x_clamp, 0, ro, oc) <= -Hreg + A.cutoff;
                 if (!clamped) {
-                    const long long xs = A.x_stride;
+                    const uint32_t xs = A.x_stride;
                     if (A.x_fold && near_boundary(R)) {
                         xe = fold_read(A.x, xs, 0, R); xu = fold_read(A.x, xs, 1, R); xv = fold_read(A.x, xs, 2, R);
                     } else {
@@ -477,7 +526,7 @@ struct Strip {
             g_hu = gU * rden;
             g_hv = gV * rden;
         }
-        const long long os = A.o_stride;
+        const uint32_t os = A.o_stride;
         stb(A.o, 0, ro, oc, A.alpha * g_eta + A.gamma * xe);
         stb(A.o, os, ro, oc, A.alpha * g_hu + A.gamma * xu);
         stb(A.o, 2 * os, ro, oc, A.alpha * g_hv + A.gamma * xv);
@@ -488,11 +537,15 @@ struct Strip {
     __device__ bool step(int k, int r0, int r1, bool out_lane, Row& ra, Row& rb, Row& rc, Row& rd,
                          Carry& Z) const {
         if (k > r1 + 1) return false;
-        // row k+2 with the corners of rows k+2 (pending) and k+3
+        // row k+2 (loaded in the previous iteration) with the corners of rows
+        // k+2 and k+3, then the loads of row k+3 and of the corners of row k+4
+        // for the next iteration
         float qc, qn;
-        corners(k + 3, qc, qn);
-        load(k + 2, Z.phc, Z.phn, qc, qn, rd);
+        corners_flag(k + 3, Z.qhc, Z.qhn, qc, qn);
+        finish(k + 2, Z.raw, Z.phc, Z.phn, qc, qn, rd);
         Z.phc = qc; Z.phn = qn;
+        fetch(k + 3, Z.raw);
+        corners_raw(k + 4, Z.qhc, Z.qhn);
         // the x direction of row k-1 needs nothing of row k+2: it runs while
         // those loads are in flight
         const int R = k - 1;
@@ -550,6 +603,8 @@ __device__ __forceinline__ void adj_march_strip(const MArgs& A, int sx, int r0, 
     Z.Suy = slope_val((s1.code >> 8) & 3u, s0.U, s1.U, s2.U, A.theta);
     Z.Svy = slope_val((s1.code >> 10) & 3u, s0.V, s1.V, s2.V, A.theta);
     Z.phc = h3c; Z.phn = h3n;
+    P.fetch(k0 + 2, Z.raw);
+    P.corners_raw(k0 + 3, Z.qhc, Z.qhn);
     Z.p1 = v3z(); Z.p2 = v3z(); Z.dp = v3z(); Z.cp = v3z(); Z.c6 = v3z();
     for (int k = k0; P.step(k, r0, r1, out_lane, s0, s1, s2, s3, Z); ++k) {
         s0 = s1; s1 = s2; s2 = s3;

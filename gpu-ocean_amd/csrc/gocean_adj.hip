@@ -127,22 +127,27 @@ static bool triple(const float* a, const float* b, const float* c, const char*& 
 }
 
 // the marching kernel's arguments, false when an array's planes are not
-// equally spaced (then the tile kernel runs)
+// equally spaced in increasing order or its last plane ends beyond 4 GiB of
+// its base (then the tile kernel runs)
 bool adj_march_args(const AdjTileArgs& A, adjm::MArgs& M) {
+    const long long plane = (long long)(A.ny + 4) * A.pitch * 4;     // bytes
+    auto fits = [&](long long st, int planes) {
+        return st >= 0 && st * (planes - 1) + plane <= 0xFFFFFFFFLL;
+    };
     const char* p;
     long long st;
-    if (!triple(A.w_eta, A.w_hu, A.w_hv, p, st)) return false;
-    M.w = p; M.w_stride = st;
-    if (!triple(A.x_eta, A.x_hu, A.x_hv, p, st)) return false;
-    M.x = p; M.x_stride = st;
-    if (!triple(A.o_eta, A.o_hu, A.o_hv, p, st)) return false;
-    M.o = const_cast<char*>(p); M.o_stride = st;
+    if (!triple(A.w_eta, A.w_hu, A.w_hv, p, st) || !fits(st, 3)) return false;
+    M.w = p; M.w_stride = (uint32_t)st;
+    if (!triple(A.x_eta, A.x_hu, A.x_hv, p, st) || !fits(st, 3)) return false;
+    M.x = p; M.x_stride = (uint32_t)st;
+    if (!triple(A.o_eta, A.o_hu, A.o_hv, p, st) || !fits(st, 3)) return false;
+    M.o = const_cast<char*>(p); M.o_stride = (uint32_t)st;
     const char* t = reinterpret_cast<const char*>(A.tU);
     const long long ts = (long long)(reinterpret_cast<const char*>(A.tV) - t);
     if (reinterpret_cast<const char*>(A.tT) != t + 2 * ts || reinterpret_cast<const char*>(A.tS) != t + 3 * ts ||
-        reinterpret_cast<const char*>(A.tC) != t + 4 * ts)
+        reinterpret_cast<const char*>(A.tC) != t + 4 * ts || !fits(ts, 5))
         return false;
-    M.tape = t; M.tape_stride = ts;
+    M.tape = t; M.tape_stride = (uint32_t)ts;
     M.eta = A.eta; M.w_clamp = A.w_clamp; M.x_clamp = A.x_clamp; M.Hi = A.Hi; M.dt_dev = A.dt_dev;
     M.dt = A.dt; M.alpha = A.alpha; M.gamma = A.gamma; M.w_fold = A.w_fold; M.x_fold = A.x_fold;
     M.nx = A.nx; M.ny = A.ny; M.pitch = A.pitch; M.hi_pitch = A.hi_pitch;
