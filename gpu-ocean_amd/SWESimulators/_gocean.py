@@ -70,6 +70,7 @@ PROTOTYPES = [
     ("gocean_max_dt_static", c_int,
      [ctypes.POINTER(Domain), c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p]),
     ("gocean_default_tile_config", c_int, [c_int, c_int]),
+    ("gocean_sqrt_newton_check", c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_ulonglong)]),
     ("gocean_adjoint_set_record_tile_config", c_int, [c_int]),
     ("gocean_adjoint_set_reverse_kernel", c_int, [c_int]),
     ("gocean_adjoint_reverse_kernel", c_int, []),

@@ -160,7 +160,7 @@ __device__ __forceinline__ void slope_fwd(float ec, float er, float ac, float ar
     else out.K = K.theta_g * ((er - ec) + vd * (out.fb + fr));
 }
 
-template <int SGN, bool ONE>
+template <int SGN, bool FAST>
 __device__ __forceinline__ void slope_rest(float el, float ec, float er, float al, float ar, float bl, float bc,
                                            float br, float Hlo, float Hhi, bool flc, bool fa, float dd, float vd,
                                            const Consts& K, float fc, const Carry& in, const Carry& out, float fr,
@@ -174,7 +174,7 @@ __device__ __forceinline__ void slope_rest(float el, float ec, float er, float a
     float Kv = minmod_raw(in.K, ce, out.K);
     const float dfc = dd * fc;
     const float dfb = dfc * (fa ? -bc : bc);
-    const float t = (SGN < 0) ? div_k<ONE>(Kv + dfb, K.two_g, K.r_two_g) : div_k<ONE>(Kv - dfb, K.two_g, K.r_two_g);
+    const float t = (SGN < 0) ? div_k<FAST>(Kv + dfb, K.two_g, K.r_two_g) : div_k<FAST>(Kv - dfb, K.two_g, K.r_two_g);
     const float hlo = (ec + Hlo) - t;
     const float hhi = (ec + Hhi) + t;
     bool adj = false;
@@ -192,7 +192,7 @@ __device__ __forceinline__ void slope_rest(float el, float ec, float er, float a
     dhi = hhi;
     if (adj || flc != fa) {
         const float db = dfc * (flc ? -bc : bc);
-        T = (SGN < 0) ? div_k<ONE>(Kv + db, K.two_g, K.r_two_g) : div_k<ONE>(Kv - db, K.two_g, K.r_two_g);
+        T = (SGN < 0) ? div_k<FAST>(Kv + db, K.two_g, K.r_two_g) : div_k<FAST>(Kv - db, K.two_g, K.r_two_g);
         dlo = (ec + Hlo) - T;
         dhi = (ec + Hhi) + T;
     }
@@ -222,7 +222,9 @@ __device__ __forceinline__ Carry slope_bwd(float el, float ec, float al, float a
 // computeGFaceFlux does.  When every lane of the wave has a wet face
 // (neither side dry-flagged, both depths above the cutoff, wave-speed spread
 // at least flux_eps), the same operations run without the selects of the
-// dry and zero-flux cases.
+// dry and zero-flux cases (and, with FAST, the wave speeds take the Newton
+// square root: both depths are above the cutoff there).
+template <bool FAST>
 __device__ __forceinline__ void face(float em, float am, float bm, float hSam, float hSbm, float dm,
                                      float ep, float ap, float bp, float hSap, float hSbp, float dp,
                                      const Consts& K, float& F0, float& F1, float& F2) {
@@ -233,10 +235,10 @@ __device__ __forceinline__ void face(float em, float am, float bm, float hSam, f
         const float um = am + hSam, vm = bm + hSbm;
         const float qp = hp * up;
         const float Fpx = qp, Fpy = qp * up + (K.half_g * hp) * hp, Fpz = qp * vp;
-        const float c_p = sqrt_rn(K.g * hp);
+        const float c_p = sqrt_k<FAST>(K.g * hp);
         const float qm = hm * um;
         const float Fmx = qm, Fmy = qm * um + (K.half_g * hm) * hm, Fmz = qm * vm;
-        const float c_m = sqrt_rn(K.g * hm);
+        const float c_m = sqrt_k<FAST>(K.g * hm);
         const float amin = fminf(fminf(um - c_m, up - c_p), 0.0f);
         const float amax = fmaxf(fmaxf(um + c_m, up + c_p), 0.0f);
         const float den = amax - amin;
@@ -257,8 +259,11 @@ __device__ __forceinline__ void face(float em, float am, float bm, float hSam, f
 }
 
 // One strip: padded columns sx*60 .. sx*60+63 (lanes), output rows
-// 2 + sy*bh .. min(2 + (sy+1)*bh, ny+2) - 1.
-template <bool ONE>
+// 2 + sy*bh .. min(2 + (sy+1)*bh, ny+2) - 1.  WGRID: the wind stress comes
+// from A.wind_grid (else the uniform A.wind_x, A.wind_y); a separate
+// instantiation, because the per-lane interpolation state and lookups would
+// otherwise hold registers in every launch.
+template <bool FAST, bool WGRID>
 __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy, int bh) {
     const int lane = threadIdx.x & (kLanes - 1);
     const int nx = A.nx, ny = A.ny;
@@ -357,18 +362,18 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
         Carry fwd;
         float fr;
         slope_fwd<+1>(c1.E, c2.E, c1.V, c2.V, c1.U, c2.U, row_flip(j0 - 1), row_flip(j0), K.udy, K, fc, fwd, fr);
-        slope_rest<+1, ONE>(c0.E, c1.E, c2.E, c0.V, c2.V, c0.U, c1.U, c2.U, Hs_m1, Hs_0, row_flip(j0 - 1),
+        slope_rest<+1, FAST>(c0.E, c1.E, c2.E, c0.V, c2.V, c0.U, c1.U, c2.U, Hs_m1, Hs_0, row_flip(j0 - 1),
                             wS && j0 - 1 < 2, K.dy, K.udy, K, fc, bwd, fwd, fr, Vy0, Uy0, S0, dlo0, dhi0);
         Carry fwd1;
         slope_fwd<+1>(c2.E, c3.E, c2.V, c3.V, c2.U, c3.U, row_flip(j0), row_flip(j0 + 1), K.udy, K, fc, fwd1, fr);
-        slope_rest<+1, ONE>(c1.E, c2.E, c3.E, c1.V, c3.V, c1.U, c2.U, c3.U, Hs_0, Hs_1, row_flip(j0),
+        slope_rest<+1, FAST>(c1.E, c2.E, c3.E, c1.V, c3.V, c1.U, c2.U, c3.U, Hs_0, Hs_1, row_flip(j0),
                             wS && j0 < 2, K.dy, K.udy, K, fc, fwd, fwd1, fr, Vy1, Uy1, S1, dlo1, dhi1);
         ycar = fwd1;
         hUy1 = 0.5f * Uy1;
         hVy1 = 0.5f * Vy1;
         // flux through (j0-1 | j0): m = row j0-1 (c1), p = row j0 (c2)
         float a, b, c;
-        face(c1.E, c1.V, c1.U, 0.5f * Vy0, 0.5f * Uy0, dhi0, c2.E, c2.V, c2.U, hVy1, hUy1, dlo1, K, a, b, c);
+        face<FAST>(c1.E, c1.V, c1.U, 0.5f * Vy0, 0.5f * Uy0, dhi0, c2.E, c2.V, c2.U, hVy1, hUy1, dlo1, K, a, b, c);
         gsx = a; gsy = c; gsz = b;
     }
     // window: (cj, cj1) = rows j, j+1; (hUyj, hVyj, Sj, dhij) = half y slopes,
@@ -381,7 +386,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
     float Hsj = Hs_0, Hnj = Hs_1;
     const bool friction = A.r > 0.0f;
     // gridded wind: the lane's column weights, fixed for the strip
-    const bool wgrid = A.wind_grid != nullptr;
+    constexpr bool wgrid = WGRID;
     WindAxis wax;
     wax.i0 = wax.i1 = 0; wax.a = 0.0f;
     if (wgrid) wax = wind_axis(C, nx, A.wind_wx);
@@ -414,7 +419,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
             bwd.b = lane_get(fwd.b, addr_l);
             bwd.K = lane_get(fwd.K, addr_l);
             bwd.fb = lane_get(fwd.fb, addr_l);
-            slope_rest<-1, ONE>(el, cj.E, er, ul, ur, vl, cj.V, vr, Hw, He, xfc, xfa, K.dx, K.vdx, K, fc, bwd, fwd,
+            slope_rest<-1, FAST>(el, cj.E, er, ul, ur, vl, cj.V, vr, Hw, He, xfc, xfa, K.dx, K.vdx, K, fc, bwd, fwd,
                                 fr, Ux, Vx, T, dlox, dhix);
             hUx = 0.5f * Ux;
             hVx = 0.5f * Vx;
@@ -422,7 +427,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
         // ---- east face (C | C+1) ----
         const float hUxr = lane_get(hUx, addr_r), hVxr = lane_get(hVx, addr_r), dpr = lane_get(dlox, addr_r);
         float fex, fey, fez;
-        face(cj.E, cj.U, cj.V, hUx, hVx, dhix, er, ur, vr, hUxr, hVxr, dpr, K, fex, fey, fez);
+        face<FAST>(cj.E, cj.U, cj.V, hUx, hVx, dhix, er, ur, vr, hUxr, hVxr, dpr, K, fex, fey, fez);
 
         // ---- conversion of row j+2, y slopes of row j+1, north face ----
         const Cell cj2 = convert(e2, u2, v2, hcj2 + hcj3, hcj2, hcj3, hnj2, hnj3, K);
@@ -433,7 +438,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
             float fr, Uyn, Vyn;
             slope_fwd<+1>(cj1.E, cj2.E, cj1.V, cj2.V, cj1.U, cj2.U, row_flip(j + 1), row_flip(j + 2), K.udy, K,
                           fc, fwd, fr);
-            slope_rest<+1, ONE>(cj.E, cj1.E, cj2.E, cj.V, cj2.V, cj.U, cj1.U, cj2.U, Hnj, Hnj1, row_flip(j + 1),
+            slope_rest<+1, FAST>(cj.E, cj1.E, cj2.E, cj.V, cj2.V, cj.U, cj1.U, cj2.U, Hnj, Hnj1, row_flip(j + 1),
                                 wS && j + 1 < 2, K.dy, K.udy, K, fc, ycar, fwd, fr, Vyn, Uyn, Sn, dlon, dhin);
             ycar = fwd;
             hUyn = 0.5f * Uyn;
@@ -442,7 +447,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
         float gnx, gny, gnz;
         {
             float a, b, c;
-            face(cj.E, cj.V, cj.U, hVyj, hUyj, dhij, cj1.E, cj1.V, cj1.U, hVyn, hUyn, dlon, K, a, b, c);
+            face<FAST>(cj.E, cj.V, cj.U, hVyj, hUyj, dhij, cj1.E, cj1.V, cj1.U, hVyn, hUyn, dlon, K, a, b, c);
             gnx = a; gny = c; gnz = b;
         }
         // ---- west face from the left lane ----
@@ -453,9 +458,9 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
             const float Hm = cj.Hm4;
             const float ec = cj.E, uc = cj.U, vc = cj.V;
             const float hreg = ec + Hm;
-            const float fd0 = div_k<ONE>(fex - fwx, K.dx, K.r_dx) + div_k<ONE>(gnx - gsx, K.dy, K.r_dy);
-            const float fd1 = div_k<ONE>(fey - fwy, K.dx, K.r_dx) + div_k<ONE>(gny - gsy, K.dy, K.r_dy);
-            const float fd2 = div_k<ONE>(fez - fwz, K.dx, K.r_dx) + div_k<ONE>(gnz - gsz, K.dy, K.r_dy);
+            const float fd0 = div_k<FAST>(fex - fwx, K.dx, K.r_dx) + div_k<FAST>(gnx - gsx, K.dy, K.r_dy);
+            const float fd1 = div_k<FAST>(fey - fwy, K.dx, K.r_dx) + div_k<FAST>(gny - gsy, K.dy, K.r_dy);
+            const float fd2 = div_k<FAST>(fez - fwz, K.dx, K.r_dx) + div_k<FAST>(gnz - gsz, K.dy, K.r_dy);
             const float eta_west = ec - T, eta_east = ec + T;
             const float eta_south = ec - Sj, eta_north = ec + Sj;
             // wind, Coriolis and bottom slope of a wet cell
@@ -472,8 +477,8 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
                 const float b2 = (K.g * (eta_sn + Hm)) * H_y;
                 const float hu_cor = fc * hv;
                 const float hv_cor = (-fc) * hu;
-                st1 = (X + hu_cor) + div_k<ONE>(b1, K.dx, K.r_dx);
-                st2 = (Y + hv_cor) + div_k<ONE>(b2, K.dy, K.r_dy);
+                st1 = (X + hu_cor) + div_k<FAST>(b1, K.dx, K.r_dx);
+                st2 = (Y + hv_cor) + div_k<FAST>(b2, K.dy, K.r_dy);
             };
             float hu, hv, st1, st2;
             if (__all(hreg > K.cutoff && ec != kDryFlag)) {

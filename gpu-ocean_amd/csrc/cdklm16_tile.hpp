@@ -66,8 +66,8 @@ struct SubstepArgs {
     __host__ __device__ int rk_order() const { return (int)((flags >> 8) & 3u); }
     __host__ __device__ int rk_step() const { return (int)((flags >> 10) & 3u); }
     // bit 12: divisions by dx, dy and 2g may take one residual correction
-    __host__ __device__ bool one_div() const { return (flags >> 12) & 1u; }
-    __host__ __device__ void set_one_div(bool on) { flags = (flags & ~(1u << 12)) | ((on ? 1u : 0u) << 12); }
+    __host__ __device__ bool fast_arith() const { return (flags >> 12) & 1u; }
+    __host__ __device__ void set_fast_arith(bool on) { flags = (flags & ~(1u << 12)) | ((on ? 1u : 0u) << 12); }
     __host__ __device__ void set_flags(int n, int e, int s_, int w, int order, int step) {
         flags = (uint32_t)(n & 3) | ((uint32_t)(e & 3) << 2) | ((uint32_t)(s_ & 3) << 4) |
                 ((uint32_t)(w & 3) << 6) | ((uint32_t)(order & 3) << 8) | ((uint32_t)(step & 3) << 10);
@@ -219,19 +219,19 @@ __device__ __forceinline__ float div_r(float n, float d, float r) {
 
 // n/d for a launch-constant divisor d (dx, dy, 2g) with r = RN(1/d) formed on
 // the host: one residual correction.  The host enables it (SubstepArgs
-// one_div()) only for divisors for which it has checked, over every float
+// fast_arith()) only for divisors for which it has checked, over every float
 // significand of n, that this gives the correctly rounded quotient (see
-// one_correction_exact in gocean_fwd.hip); otherwise div_r's two
-// corrections are used.
+// fast_arith_exact in gocean_fwd.hip); otherwise div_r's two corrections are
+// used.
 __device__ __forceinline__ float div_c1(float n, float d, float r) {
     const float q = n * r;
     const float e = fmaf(-d, q, n);
     return fmaf(e, r, q);
 }
 
-template <bool ONE>
+template <bool FAST>
 __device__ __forceinline__ float div_k(float n, float d, float r) {
-    if constexpr (ONE) return div_c1(n, d, r);
+    if constexpr (FAST) return div_c1(n, d, r);
     else return div_r(n, d, r);
 }
 
@@ -246,6 +246,28 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     s = (vm <= 0.0f) ? sm : s;
     s = (vp > 0.0f) ? sp : s;
     return s;
+}
+
+// sqrt(x) from the hardware reciprocal square root y and one Newton step:
+// g = RN(x y), r = x - g^2 (one fma), result RN(g + r * y/2).  Checked
+// exhaustively on the device against sqrt_rn (gocean_sqrt_newton_check): the
+// same bits for every float in [2^-60, FLT_MAX]; near the bottom of the
+// normal range they differ.  Five instructions against sqrt_rn's nine.  The
+// kernels take it (FAST) only where the argument is g h with h above the
+// depth cutoff, and the host enables FAST only when RN(g cutoff) >= 2^-60
+// (fast_arith_exact in gocean_fwd.hip).
+__device__ __forceinline__ float sqrt_newton(float x) {
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float g = x * y;
+    const float h = 0.5f * y;
+    const float r = fmaf(-g, g, x);
+    return fmaf(r, h, g);
+}
+
+template <bool FAST>
+__device__ __forceinline__ float sqrt_k(float x) {
+    if constexpr (FAST) return sqrt_newton(x);
+    else return sqrt_rn(x);
 }
 
 struct Consts {

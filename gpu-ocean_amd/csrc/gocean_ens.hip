@@ -84,7 +84,7 @@ __device__ __forceinline__ void member_args(const EnsLaunch& L, const EnsMember&
 
 // column-marching substep over every member's strips: wavefront -> strip
 // through the tile map (one entry per strip), as k_march
-template <bool ONE>
+template <bool FAST>
 __global__ __launch_bounds__(256) void k_ens_march(EnsLaunch L) {
     const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
     if (wave >= L.n_tiles) return;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_ens_march(EnsLaunch L) {
     member_args<kModeZero>(L, M, m, A);
     const int lt = wave - M.tile_begin;
     const int sy = lt / M.tiles_x, sx = lt - sy * M.tiles_x;
-    march::march_strip<ONE>(A, sx, sy, L.march_bh);
+    march::march_strip<FAST, false>(A, sx, sy, L.march_bh);   // members: uniform wind
 }
 
 template <int BW, int BH, int NT, int MODE>
@@ -377,7 +377,7 @@ struct gocean_ensemble {
     int bw = 32, bh = 16, nt = 256;
     int n_tiles = 0;
     int device = 0;
-    bool one_div = false;      // every member's dx, dy, 2g divide with one correction
+    bool fast_arith = false;   // fast_arith_exact holds for every member
     EnsMember* d_members = nullptr;
     int* d_tile_member = nullptr;
     std::vector<EnsMember> members;
@@ -399,9 +399,9 @@ static void ens_launch_t(const EnsLaunch& L, hipStream_t s) {
     hipLaunchKernelGGL((k_ens_substep<BW, BH, NT, MODE>), dim3(L.n_tiles), dim3(NT), 0, s, L);
 }
 
-static void ens_march(EnsLaunch L, int bh, bool one_div, hipStream_t s) {
+static void ens_march(EnsLaunch L, int bh, bool fast_arith, hipStream_t s) {
     L.march_bh = bh;
-    if (one_div)
+    if (fast_arith)
         hipLaunchKernelGGL((k_ens_march<true>), dim3((L.n_tiles + 3) / 4), dim3(256), 0, s, L);
     else
         hipLaunchKernelGGL((k_ens_march<false>), dim3((L.n_tiles + 3) / 4), dim3(256), 0, s, L);
@@ -409,13 +409,13 @@ static void ens_march(EnsLaunch L, int bh, bool one_div, hipStream_t s) {
 
 // rows per strip of the ensemble's column-marching launch (configuration
 // with bh == 0): whole rounds of resident waves over all members' strips
-static int ens_march_rows(const std::vector<EnsMember>& members, bool one_div) {
+static int ens_march_rows(const std::vector<EnsMember>& members, bool fast_arith) {
     std::vector<int> strips, ny;
     for (const EnsMember& M : members) {
         strips.push_back((M.nx + march::kOut - 1) / march::kOut);
         ny.push_back(M.ny);
     }
-    const int resident = resident_waves(one_div ? (const void*)k_ens_march<true> : (const void*)k_ens_march<false>);
+    const int resident = resident_waves(fast_arith ? (const void*)k_ens_march<true> : (const void*)k_ens_march<false>);
     return march_rows(strips.data(), ny.data(), (int)members.size(), resident);
 }
 
@@ -438,7 +438,7 @@ static int ens_launch(const gocean_ensemble* E, EnsLaunch L, hipStream_t s) {
             case 6: ens_launch_t<32, 14, 512, kModeZero>(L, s); break;
             case 7:
             case 8:
-            case 9: ens_march(L, E->bh, E->one_div, s); break;
+            case 9: ens_march(L, E->bh, E->fast_arith, s); break;
         }
     }
     return check_launch("ensemble substep launch");
@@ -514,18 +514,28 @@ int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_confi
             }
     }
     E->mode = any_field ? kModeField : (any_const ? kModeConst : kModeZero);
-    E->one_div = true;
+    bool fits = true;
+    for (int k = 0; k < n; ++k)
+        fits = fits && march_fits(members[k].domain.ny, members[k].domain.pitch, members[k].domain.hi_pitch);
+    if (E->cfg >= kFirstMarchConfig && !fits) {
+        if (tile_config >= 0) {
+            delete E;
+            return set_error(GOCEAN_EINVAL, "tile configuration %d (column marching) needs member planes below 4 GiB",
+                             tile_config);
+        }
+        E->cfg = kDefaultTileConfig;
+    }
+    E->fast_arith = true;
     for (int k = 0; k < n; ++k) {
         const gocean_domain_t& d = members[k].domain;
-        E->one_div = E->one_div && one_correction_exact(2.0f * d.g) && one_correction_exact(d.dx) &&
-                     one_correction_exact(d.dy);
+        E->fast_arith = E->fast_arith && fast_arith_exact(d.g, d.dx, d.dy, d.depth_cutoff);
     }
     const TileConfig& tc = (E->mode == kModeZero) ? kTileConfigs[E->cfg] : kTileConfigs[0];
     E->bw = tc.bw; E->bh = tc.bh; E->nt = tc.nt;
     if (E->bh == 0) {
         std::vector<EnsMember> shape(n);
         for (int k = 0; k < n; ++k) { shape[k].nx = members[k].domain.nx; shape[k].ny = members[k].domain.ny; }
-        E->bh = ens_march_rows(shape, E->one_div);
+        E->bh = ens_march_rows(shape, E->fast_arith);
     }
     std::vector<int> tile_member;
     E->members.resize(n);
@@ -600,7 +610,7 @@ int gocean_ensemble_set_tile_config(gocean_ensemble_t* E, int tile_config) {
     E->cfg = tile_config;
     const TileConfig& tc = (E->mode == kModeZero) ? kTileConfigs[E->cfg] : kTileConfigs[0];
     E->bw = tc.bw; E->bh = tc.bh; E->nt = tc.nt;
-    if (E->bh == 0) E->bh = ens_march_rows(E->members, E->one_div);
+    if (E->bh == 0) E->bh = ens_march_rows(E->members, E->fast_arith);
     std::vector<int> tile_member;
     for (int k = 0; k < E->n; ++k) {
         EnsMember& M = E->members[k];

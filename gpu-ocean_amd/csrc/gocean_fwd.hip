@@ -130,6 +130,14 @@ bool one_correction_exact(float d) {
     return ok;
 }
 
+bool fast_arith_exact(float g, float dx, float dy, float cutoff) {
+    // sqrt_newton equals sqrt_rn for arguments >= 2^-60; the wave speeds take
+    // it only for depths above the cutoff, whose g h rounds to >= RN(g cutoff)
+    const float lo = g * cutoff;
+    if (!(g > 0.0f) || !std::isfinite(g) || !(lo >= 0x1p-60f)) return false;
+    return one_correction_exact(2.0f * g) && one_correction_exact(dx) && one_correction_exact(dy);
+}
+
 void fill_args(const gocean_domain_t* d, SubstepArgs& A) {
     std::memset(&A, 0, sizeof(A));
     A.nx = d->nx; A.ny = d->ny; A.pitch = d->pitch; A.hi_pitch = d->hi_pitch;
@@ -139,7 +147,7 @@ void fill_args(const gocean_domain_t* d, SubstepArgs& A) {
     A.f = d->f; A.north_x = d->north_x; A.north_y = d->north_y;
     A.set_flags(d->bc_north, d->bc_east, d->bc_south, d->bc_west, d->rk_order, 0);
     substep_derive(A);
-    A.set_one_div(one_correction_exact(2.0f * A.g) && one_correction_exact(A.dx) && one_correction_exact(A.dy));
+    A.set_fast_arith(fast_arith_exact(A.g, A.dx, A.dy, A.cutoff));
 }
 
 // ---------------------------------------------------------------------------
@@ -181,12 +189,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 
 
 // column-marching substep: wavefront w of the grid takes strip (sx, sy),
 // sx fastest, workgroups banded over the XCDs
-template <bool ONE>
+template <bool FAST, bool WGRID>
 __global__ __launch_bounds__(256) void k_march(SubstepArgs A) {
     const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
     if (wave >= A.tiles_x * A.tiles_y) return;
     const int sy = wave / A.tiles_x, sx = wave - sy * A.tiles_x;
-    march::march_strip<ONE>(A, sx, sy, A.march_bh);
+    march::march_strip<FAST, WGRID>(A, sx, sy, A.march_bh);
 }
 
 // waves of a kernel that the device holds at once: 4 SIMDs per CU, each
@@ -223,23 +231,45 @@ int march_rows(const int* strips, const int* ny, int n, int resident) {
     return best;
 }
 
+// the marching kernel for the launch's arithmetic and wind variants
+static const void* march_kernel(bool fast, bool wgrid) {
+    if (fast) return wgrid ? (const void*)k_march<true, true> : (const void*)k_march<true, false>;
+    return wgrid ? (const void*)k_march<false, true> : (const void*)k_march<false, false>;
+}
+
+// sqrt_newton against sqrt_rn over the float bit patterns lo .. hi
+__global__ __launch_bounds__(256) void k_sqrt_check(uint32_t lo, uint32_t hi, unsigned long long* count) {
+    const uint64_t n = (uint64_t)hi - lo + 1;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bad = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float x = __uint_as_float(lo + (uint32_t)i);
+        bad += __float_as_uint(sqrt_newton(x)) != __float_as_uint(sqrt_rn(x));
+    }
+    if (bad) atomicAdd(count, bad);
+}
+
 // bh > 0: fixed rows per strip; 0: balanced by march_rows
 static void launch_march(SubstepArgs A, int bh, hipStream_t s) {
     A.tiles_x = (A.nx + march::kOut - 1) / march::kOut;
+    const bool fast = A.fast_arith(), wgrid = A.wind_grid != nullptr;
     if (bh <= 0) {
-        static int resident[2] = {0, 0};
-        const int k = A.one_div() ? 1 : 0;
-        if (!resident[k])
-            resident[k] = resident_waves(k ? (const void*)k_march<true> : (const void*)k_march<false>);
+        static int resident[4] = {0, 0, 0, 0};
+        const int k = (fast ? 2 : 0) + (wgrid ? 1 : 0);
+        if (!resident[k]) resident[k] = resident_waves(march_kernel(fast, wgrid));
         bh = march_rows(&A.tiles_x, &A.ny, 1, resident[k]);
     }
     A.march_bh = bh;
     A.tiles_y = (A.ny + bh - 1) / bh;
     const int waves = A.tiles_x * A.tiles_y;
-    if (A.one_div())
-        hipLaunchKernelGGL((k_march<true>), dim3((waves + 3) / 4), dim3(256), 0, s, A);
-    else
-        hipLaunchKernelGGL((k_march<false>), dim3((waves + 3) / 4), dim3(256), 0, s, A);
+    const dim3 grid((waves + 3) / 4), block(256);
+    if (fast) {
+        if (wgrid) hipLaunchKernelGGL((k_march<true, true>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((k_march<true, false>), grid, block, 0, s, A);
+    } else {
+        if (wgrid) hipLaunchKernelGGL((k_march<false, true>), grid, block, 0, s, A);
+        else hipLaunchKernelGGL((k_march<false, false>), grid, block, 0, s, A);
+    }
 }
 
 template <int BW, int BH, int NT, int MODE>
@@ -295,10 +325,22 @@ int substep_mode(const SubstepArgs& A, bool field) {
 // per strip from a million cells on (where it is faster), else the 2-D tile
 int default_tile_config(long long cells) { return cells >= (1LL << 20) ? kMarchAutoConfig : kDefaultTileConfig; }
 
+// the column-marching kernel addresses a plane with 32-bit byte offsets
+bool march_fits(int ny, int pitch, int hi_pitch) {
+    return (uint64_t)(ny + 4) * (uint64_t)pitch * 4u <= 0xFFFFFFFFull &&
+           (uint64_t)(ny + 5) * (uint64_t)hi_pitch * 4u <= 0xFFFFFFFFull;
+}
+
 int launch_substep(const SubstepArgs& A, int cfg, bool field, hipStream_t s) {
-    if (cfg < 0) cfg = default_tile_config((long long)A.nx * A.ny);
+    const bool fits = march_fits(A.ny, A.pitch, A.hi_pitch);
+    if (cfg < 0) {
+        cfg = default_tile_config((long long)A.nx * A.ny);
+        if (cfg >= kFirstMarchConfig && !fits) cfg = kDefaultTileConfig;
+    }
     if (cfg >= kNumTileConfigs)
         return set_error(GOCEAN_EINVAL, "tile configuration %d out of range [0,%d)", cfg, kNumTileConfigs);
+    if (cfg >= kFirstMarchConfig && !fits)
+        return set_error(GOCEAN_EINVAL, "tile configuration %d (column marching) needs planes below 4 GiB", cfg);
     const int mode = substep_mode(A, field);
     if (mode == kModeField) {
         // per-cell Coriolis / north fields: one general configuration
@@ -516,6 +558,31 @@ int gocean_adjoint_set_record_tile_config(int cfg) {
 int gocean_adjoint_record_tile_config(void) { return g_record_cfg; }
 
 int gocean_default_tile_config(int nx, int ny) { return default_tile_config((long long)nx * ny); }
+
+int gocean_sqrt_newton_check(uint32_t lo_bits, uint32_t hi_bits, unsigned long long* mismatches) {
+    clear_error();
+    if (!mismatches) return set_error(GOCEAN_EINVAL, "mismatches is null");
+    if (lo_bits > hi_bits || hi_bits > 0x7F7FFFFFu)
+        return set_error(GOCEAN_EINVAL, "range [%08x, %08x] is not within the positive finite floats", lo_bits,
+                         hi_bits);
+    unsigned long long* d = nullptr;
+    if (hipMalloc(&d, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(GOCEAN_ENOMEM, "hipMalloc of the mismatch counter failed");
+    }
+    hipError_t e = hipMemset(d, 0, sizeof(unsigned long long));
+    int rc = (e == hipSuccess) ? GOCEAN_OK : set_error((int)e, "hipMemset: %s", hipGetErrorString(e));
+    if (rc == GOCEAN_OK) {
+        hipLaunchKernelGGL(k_sqrt_check, dim3(4096), dim3(256), 0, 0, lo_bits, hi_bits, d);
+        rc = check_launch("sqrt check launch");
+    }
+    if (rc == GOCEAN_OK) {
+        e = hipMemcpy(mismatches, d, sizeof(unsigned long long), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = set_error((int)e, "hipMemcpy: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(d);
+    return rc;
+}
 
 int gocean_tile_configs(const char** names, int max) {
     for (int k = 0; k < kNumTileConfigs && k < max; ++k) names[k] = kTileConfigs[k].name;

@@ -19,6 +19,8 @@ extern const TileConfig kTileConfigs[];
 extern const int kNumTileConfigs;
 extern const int kDefaultTileConfig;
 extern const int kMarchAutoConfig;
+// configurations kFirstMarchConfig .. kNumTileConfigs-1 are column marching
+constexpr int kFirstMarchConfig = 7;
 // configuration used when the caller passes -1
 int default_tile_config(long long cells);
 
@@ -31,6 +33,13 @@ void fill_args(const gocean_domain_t* d, SubstepArgs& A);
 // true when one residual correction divides every float by d correctly
 // rounded (checked once per value of d, on the host)
 bool one_correction_exact(float d);
+// the substep's FAST arithmetic is exact for these constants: one-correction
+// division by dx, dy and 2g, and the Newton square root of g h for h above
+// the depth cutoff
+bool fast_arith_exact(float g, float dx, float dy, float cutoff);
+// the column-marching kernels' 32-bit plane offsets cover a domain of this
+// height and pitches
+bool march_fits(int ny, int pitch, int hi_pitch);
 // column-marching launch shape: waves the device holds at once for a kernel,
 // and the rows per strip that fill whole rounds of them
 int resident_waves(const void* kernel);

@@ -180,6 +180,15 @@ int gocean_tile_configs(const char** names, int max);
  * from a million cells on, the 32x16/512 tile below. */
 int gocean_default_tile_config(int nx, int ny);
 
+/* Device self-check of the square root the substep kernels take for wave
+ * speeds when the host enables their fast arithmetic: counts the float bit
+ * patterns in [lo_bits, hi_bits] (positive floats, lo_bits <= hi_bits <=
+ * 0x7F7FFFFF) for which the reciprocal-square-root-plus-Newton sequence
+ * differs from the correctly rounded square root, and writes the count to
+ * *mismatches.  Synchronous.  No counterpart in the reference: it backs the
+ * claim that the shortcut is exact for every argument >= 2^-60. */
+int gocean_sqrt_newton_check(uint32_t lo_bits, uint32_t hi_bits, unsigned long long* mismatches);
+
 /* ------------------------------------------------------------------------
  * Batched ensembles.  The reference steps ensemble members one at a time
  * from Python (OceanModelEnsemble.modelStep,

@@ -287,3 +287,30 @@ def test_iewpf_kalman_window_argument_checks():
     assert call(lst=None) == -1 and "NULL" in lib.gocean_last_error().decode()
     assert call(n=33, cs=47) == -1 and "coefficient stride" in lib.gocean_last_error().decode()
     assert call(n_tiles=0) == 0                          # nothing to add, no launch
+
+
+def test_sqrt_newton_check_arguments_rejected():
+    lib = G.load()
+    n = ctypes.c_ulonglong(7)
+    assert lib.gocean_sqrt_newton_check(0x3F800000, 0x3F000000, ctypes.byref(n)) == -1
+    assert b"not within" in lib.gocean_last_error()
+    assert lib.gocean_sqrt_newton_check(0x3F800000, 0x7F800000, ctypes.byref(n)) == -1   # +inf excluded
+    assert lib.gocean_sqrt_newton_check(0x3F800000, 0x3F800001, None) == -1
+    assert n.value == 7
+
+
+def test_marching_config_rejects_planes_of_4gib():
+    """The column-marching kernels address planes with 32-bit byte offsets:
+    an explicit marching configuration on a larger plane is an error (the
+    default configuration falls back to the tile kernel instead)."""
+    lib = G.load()
+    d = _dom(pitch=1 << 26)                    # 24 rows x 256 MiB
+    rc = lib.gocean_cdklm_substep(d, 1.0, None, 0, 16, 16, 16, None, None, None,
+                                  32, 32, 32, 64, None, None, 9, None)
+    assert rc == -1 and "4 GiB" in lib.gocean_last_error().decode()
+    m = (G.Member * 1)()
+    m[0].domain = d
+    m[0].field_offset = -1
+    h = ctypes.c_void_p()
+    assert lib.gocean_ensemble_create(m, 1, 9, ctypes.byref(h)) == -1
+    assert b"4 GiB" in lib.gocean_last_error()
