@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kAdjNT) void k_adj_tile(AdjTileArgs A) {
 
 // column-marching form (adjoint_march.hpp): one wavefront per strip of 58
 // output columns and bh rows (padded rows 0 .. ny+3, ghost rows included)
-__global__ __launch_bounds__(256) void k_adj_march(adjm::MArgs A, int strips_x, int bh, int n_strips) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_adj_march(adjm::MArgs A, int strips_x, int bh, int n_strips) {
     const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
     if (wave >= n_strips) return;
     const int sy = wave / strips_x, sx = wave - sy * strips_x;
@@ -39,8 +39,11 @@ __global__ __launch_bounds__(256) void k_adj_march(adjm::MArgs A, int strips_x, 
 }
 
 // reverse kernel: 0 tile, 1 column-marching, -1 by size (marching from
-// 4 M cells on, where it is the faster of the two)
+// 2 M cells on: at 1024^2 the two take the same time, at 2048^2 marching
+// is 25 % faster)
 int g_adj_kernel = -1;
+
+bool adj_use_march(long long cells) { return g_adj_kernel == 1 || (g_adj_kernel < 0 && cells >= (2LL << 20)); }
 
 // rows per strip of k_adj_march for domains of the given widths and heights
 int adj_march_rows(const int* nx, const int* ny, int n) {
@@ -160,7 +163,7 @@ bool adj_march_args(const AdjTileArgs& A, adjm::MArgs& M) {
 
 static int adj_launch(const AdjTileArgs& A, bool fold_out, hipStream_t s) {
     adjm::MArgs M;
-    const bool march = g_adj_kernel == 1 || (g_adj_kernel < 0 && (long long)A.nx * A.ny >= (4LL << 20));
+    const bool march = adj_use_march((long long)A.nx * A.ny);
     if (march && adj_march_args(A, M)) {
         const int sx = (A.nx + adjm::kOut - 1) / adjm::kOut;
         const int bh = adj_march_rows(&A.nx, &A.ny, 1);

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/gocean.h"
+#include "adjoint_march.hpp"
 #include "adjoint_tile.hpp"
 #include "cdklm16_march.hpp"
 #include "cdklm16_tile.hpp"
@@ -36,6 +37,7 @@ struct EnsMember {
     // adjoint (kAdjBW x kAdjBH) launches
     long long work_off;
     int ftiles_x, ftile_begin, atiles_x, atile_begin;
+    int astrips_x, astrip_begin;   // the member's strips in the column-marching reverse launch
     float r_dx, r_dy, r_two_g, Dx, Dy, Ddx, Ddy, g_dx, g_dy;   // as AdjTileArgs / SubstepArgs
 };
 
@@ -345,6 +347,47 @@ __global__ __launch_bounds__(kAdjNT) __attribute__((amdgpu_waves_per_eu(8))) voi
     adj_tile_body<kAdjBW, kAdjBH, kAdjNT, true>(A, S, tx, ty, threadIdx.x);
 }
 
+// Column-marching reverse substep over every member's strips (wavefront ->
+// strip through the strip map, as k_adj_march for one domain).  Each member's
+// arrays are addressed from their first plane with 32-bit plane strides; the
+// host takes this kernel only when every member's five record planes fit.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_ens_adj_march(EnsAdjLaunch L,
+                                                                                               int bh, int n_strips) {
+    const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
+    if (wave >= n_strips) return;
+    const int m = L.tile_member[wave];
+    const EnsMember& M = L.members[m];
+    const uint32_t pb = (uint32_t)(M.plane * 4);       // plane stride in bytes
+    adjm::MArgs A;
+    A.eta = L.eta.base + ens_off(M, L.eta);
+    A.tape = reinterpret_cast<const char*>(L.tape.base + ens_off(M, L.tape));
+    A.w = reinterpret_cast<const char*>(L.w.base + ens_off(M, L.w));
+    A.w_clamp = L.w_clamp.base + ens_off(M, L.w_clamp);
+    if (L.x.base) {
+        A.x = reinterpret_cast<const char*>(L.x.base + ens_off(M, L.x));
+        A.x_clamp = L.x_clamp.base + ens_off(M, L.x_clamp);
+    } else {
+        A.x = nullptr;
+        A.x_clamp = nullptr;
+    }
+    A.o = reinterpret_cast<char*>(const_cast<float*>(L.o.base) + ens_off(M, L.o));
+    A.Hi = L.Hi + M.hi_off;
+    A.dt_dev = L.dt + m;
+    A.tape_stride = pb; A.w_stride = pb; A.x_stride = pb; A.o_stride = pb;
+    A.dt = 0.0f; A.alpha = L.alpha; A.gamma = L.gamma;
+    A.w_fold = L.w_fold; A.x_fold = L.x_fold;
+    A.nx = M.nx; A.ny = M.ny; A.pitch = M.pitch; A.hi_pitch = M.hi_pitch;
+    A.g = M.g; A.theta = M.theta; A.desing = M.desing; A.cutoff = M.cutoff; A.land = M.land; A.f = M.f;
+    A.bc_n = M.bc_n; A.bc_e = M.bc_e; A.bc_s = M.bc_s; A.bc_w = M.bc_w;
+    A.r_dx = M.r_dx; A.r_dy = M.r_dy; A.r_two_g = M.r_two_g; A.Dx = M.Dx; A.Dy = M.Dy;
+    A.Ddx = M.Ddx; A.Ddy = M.Ddy; A.g_dx = M.g_dx; A.g_dy = M.g_dy;
+    const int lt = wave - M.astrip_begin;
+    const int sy = lt / M.astrips_x, sx = lt - sy * M.astrips_x;
+    const int r0 = sy * bh;
+    const int r1 = min(r0 + bh, M.ny + 4) - 1;
+    adjm::adj_march_strip(A, sx, r0, r1);
+}
+
 // member blockIdx.y's three state planes from src to dst
 __global__ __launch_bounds__(256) void k_ens_copy3(const EnsMember* members, EnsRef src, EnsRef dst) {
     const EnsMember& M = members[blockIdx.y];
@@ -390,6 +433,12 @@ struct gocean_ensemble {
     long long max_plane = 0;
     int* d_ftile_member = nullptr;
     int* d_atile_member = nullptr;
+    // column-marching reverse launch: strip map, rows per strip, and whether
+    // every member's planes fit its 32-bit offsets
+    int* d_astrip_member = nullptr;
+    int n_astrips = 0, adj_bh = 0;
+    long long adj_cells = 0;
+    bool adj_march_fits = false;
 };
 
 namespace gocean {
@@ -647,6 +696,7 @@ int gocean_ensemble_destroy(gocean_ensemble_t* E) {
     hipFree(E->d_tile_member);
     hipFree(E->d_ftile_member);
     hipFree(E->d_atile_member);
+    hipFree(E->d_astrip_member);
     delete E;
     return 0;
 }
@@ -783,17 +833,46 @@ long long gocean_ensemble_adjoint_setup(gocean_ensemble_t* E, int max_steps) {
         max_side = std::max(max_side, std::max(M.nx, M.ny));
         max_plane = std::max(max_plane, M.plane);
     }
+    // column-marching reverse launch: one row count for all members' strips,
+    // whole rounds of the kernel's resident waves
+    std::vector<int> snx(E->n), sny(E->n), astrips;
+    bool fits = true;
+    long long cells = 0;
+    for (int k = 0; k < E->n; ++k) {
+        const EnsMember& M = E->members[k];
+        snx[k] = (M.nx + adjm::kOut - 1) / adjm::kOut;
+        sny[k] = M.ny + 4;
+        fits = fits && 5LL * M.plane * 4 <= 0xFFFFFFFFLL && march_fits(M.ny, M.pitch, M.hi_pitch);
+        cells += (long long)M.nx * M.ny;
+    }
+    static int resident = 0;
+    if (!resident) resident = resident_waves((const void*)k_ens_adj_march);
+    const int abh = march_rows(snx.data(), sny.data(), E->n, resident);
+    for (int k = 0; k < E->n; ++k) {
+        EnsMember& M = E->members[k];
+        M.astrips_x = snx[k];
+        M.astrip_begin = (int)astrips.size();
+        for (int t = 0; t < snx[k] * ((sny[k] + abh - 1) / abh); ++t) astrips.push_back(k);
+    }
     hipFree(E->d_ftile_member);
     hipFree(E->d_atile_member);
+    hipFree(E->d_astrip_member);
     E->d_ftile_member = nullptr;
     E->d_atile_member = nullptr;
+    E->d_astrip_member = nullptr;
     if (hipMalloc(&E->d_ftile_member, sizeof(int) * ftiles.size()) != hipSuccess ||
-        hipMalloc(&E->d_atile_member, sizeof(int) * atiles.size()) != hipSuccess) {
+        hipMalloc(&E->d_atile_member, sizeof(int) * atiles.size()) != hipSuccess ||
+        hipMalloc(&E->d_astrip_member, sizeof(int) * astrips.size()) != hipSuccess) {
         (void)hipGetLastError();
         return set_error(GOCEAN_ENOMEM, "device allocation of the adjoint tile maps failed");
     }
     hipMemcpy(E->d_ftile_member, ftiles.data(), sizeof(int) * ftiles.size(), hipMemcpyHostToDevice);
     hipMemcpy(E->d_atile_member, atiles.data(), sizeof(int) * atiles.size(), hipMemcpyHostToDevice);
+    hipMemcpy(E->d_astrip_member, astrips.data(), sizeof(int) * astrips.size(), hipMemcpyHostToDevice);
+    E->n_astrips = (int)astrips.size();
+    E->adj_bh = abh;
+    E->adj_cells = cells;
+    E->adj_march_fits = fits;
     hipMemcpy(E->d_members, E->members.data(), sizeof(EnsMember) * E->n, hipMemcpyHostToDevice);
     if (int e = check_launch("ensemble adjoint setup")) return e;
     E->adj_kmax = max_steps;
@@ -838,7 +917,13 @@ struct EnsSegment {
         L.members = E->d_members; L.tile_member = E->d_atile_member;
         L.eta = eta; L.tape = tape; L.w = w; L.w_clamp = wc; L.x = x; L.x_clamp = xc; L.o = o;
         L.Hi = Hi; L.dt = dt; L.alpha = alpha; L.gamma = gamma; L.w_fold = w_fold; L.x_fold = x_fold;
-        hipLaunchKernelGGL(k_ens_adj_tile, dim3(E->n_atiles), dim3(kAdjNT), 0, s, L);
+        if (adj_use_march(E->adj_cells) && E->adj_march_fits) {
+            L.tile_member = E->d_astrip_member;
+            hipLaunchKernelGGL(k_ens_adj_march, dim3((E->n_astrips + 3) / 4), dim3(256), 0, s, L, E->adj_bh,
+                               E->n_astrips);
+        } else {
+            hipLaunchKernelGGL(k_ens_adj_tile, dim3(E->n_atiles), dim3(kAdjNT), 0, s, L);
+        }
         if (fold_out) {
             const int gx = (E->max_side + 16 + 255) / 256;
             hipLaunchKernelGGL(k_ens_adj_fold, dim3(gx, 9, E->n), dim3(256), 0, s, E->d_members, o);

@@ -61,5 +61,8 @@ int max_dt(const gocean_domain_t* d, const float* eta, const float* hu, const fl
 // and the marching kernel's rows per strip for domains of these sizes
 extern int g_adj_kernel;
 int adj_march_rows(const int* nx, const int* ny, int n);
+// the reverse kernel for a launch over this many cells: column marching
+// (true) or tile, by g_adj_kernel
+bool adj_use_march(long long cells);
 
 }  // namespace gocean

@@ -164,11 +164,13 @@ int gocean_max_dt_static(const gocean_domain_t* d, const float* eta, const float
 int gocean_adjoint_set_record_tile_config(int cfg);
 int gocean_adjoint_record_tile_config(void);
 
-/* Kernel of the substep adjoint in the single-domain reverse entry points:
- * 0 = one 16x16 tile per workgroup through LDS, 1 = column-marching
- * wavefronts, -1 (the default) = by size, the marching kernel from 4 M cells
- * on.  Both give the same bits.  The ensemble entry points use the tile
- * kernel.  Process-wide; no counterpart in the reference. */
+/* Kernel of the substep adjoint in the reverse entry points: 0 = one 16x16
+ * tile per workgroup through LDS, 1 = column-marching wavefronts, -1 (the
+ * default) = by size, the marching kernel from 2 M cells on (a domain's
+ * cells, or all members' cells together for an ensemble).  Both give the
+ * same bits.  An ensemble whose member planes exceed the marching kernel's
+ * 32-bit offsets takes the tile kernel.  Process-wide; no counterpart in the
+ * reference. */
 int gocean_adjoint_set_reverse_kernel(int kind);
 int gocean_adjoint_reverse_kernel(void);
 

@@ -94,3 +94,46 @@ def test_marching_adjoint_equals_tile_adjoint_recomputed_records():
     _say("marching vs tile adjoint, recomputed records")
     p, Hi, Hm, eta, hu, hv = H.random_case(117, 45, (2, 1, 2, 1), 8)
     _compare(p, Hi, [eta, hu, hv], 3.0, 5, 2, seed=9, keep=False)
+
+
+def _ensemble_grad(kernel, ms, n, K, keep, lam_seed):
+    from SWESimulators import CDKLM16Ensemble
+    Common, G, A = _mods()
+    lib = G.load()
+    ctx = Common.HIPContext(0)
+    ens = CDKLM16Ensemble.CDKLM16Ensemble(ctx, ms, rk_order=2, courant_number=0.6)
+    state0 = ens.state[0].clone()
+    rng = np.random.default_rng(lam_seed)
+    lam = torch.zeros_like(state0)
+    for k in range(ens.n):
+        o, pl = int(ens.state_off[k]), ens.planes[k]
+        for f in range(3):
+            v = lam[o + f * pl:o + (f + 1) * pl].view(ens.ny[k] + 4, ens.pitch[k])
+            v[2:-2, 2:-2] = torch.from_numpy(rng.standard_normal((ens.ny[k], ens.nx[k])).astype(np.float32))
+    G.check(lib.gocean_adjoint_set_reverse_kernel(kernel), "set reverse kernel")
+    ea = A.EnsembleAdjoint(ens, checkpoint_every=K, keep_records=keep)
+    out = ea.forward(state0, n)
+    grad = ea.backward(lam)
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), grad.cpu().numpy()
+
+
+@pytest.mark.parametrize("keep", [True, False])
+def test_ensemble_marching_adjoint_equals_tile_adjoint(keep):
+    """The batched reverse launch over every member's strips (members of
+    different sizes, boundaries, f, dx and dt in one launch) against the
+    batched tile adjoint: the same final state and gradient bits."""
+    from test_gpu_ensemble import members
+    _say("ensemble marching vs tile adjoint, keep_records", keep)
+    ms = members(seed=5, sizes=((117, 70), (58, 40), (59, 33), (37, 29)))
+    try:
+        fa, ga = _ensemble_grad(0, ms, 5, 2, keep, 8)
+        fb, gb = _ensemble_grad(1, ms, 5, 2, keep, 8)
+    finally:
+        _, G, _ = _mods()
+        G.check(G.load().gocean_adjoint_set_reverse_kernel(-1), "reset")
+    assert np.array_equal(fa, fb)
+    assert np.isfinite(ga).all() and np.abs(ga).max() > 0
+    if not H.normal_equal(ga, gb):
+        raise AssertionError(f"ensemble gradients differ at {int(np.sum(ga != gb))} entries, first "
+                             f"{H.diff_report(ga, gb, 8)}")
