@@ -160,11 +160,15 @@ __device__ __forceinline__ void slope_fwd(float ec, float er, float ac, float ar
     else out.K = K.theta_g * ((er - ec) + vd * (out.fb + fr));
 }
 
-template <int SGN, bool FAST>
+// REC: also the slope centre's branch record (tape code byte): the minmod
+// selections of the u and v slopes and of K / L, and which positivity
+// adjustment fired (x: a = u; y: a = v)
+template <int SGN, bool FAST, bool REC = false>
 __device__ __forceinline__ void slope_rest(float el, float ec, float er, float al, float ar, float bl, float bc,
                                            float br, float Hlo, float Hhi, bool flc, bool fa, float dd, float vd,
                                            const Consts& K, float fc, const Carry& in, const Carry& out, float fr,
-                                           float& Sa, float& Sb, float& T, float& dlo, float& dhi) {
+                                           float& Sa, float& Sb, float& T, float& dlo, float& dhi,
+                                           uint32_t* code8 = nullptr) {
     Sa = minmod_raw(in.a, (ar - al) * 0.5f, out.a);
     Sb = minmod_raw(in.b, (br - bl) * 0.5f, out.b);
     const float fl = in.fb, fcc = out.fb;
@@ -177,6 +181,13 @@ __device__ __forceinline__ void slope_rest(float el, float ec, float er, float a
     const float t = (SGN < 0) ? div_k<FAST>(Kv + dfb, K.two_g, K.r_two_g) : div_k<FAST>(Kv - dfb, K.two_g, K.r_two_g);
     const float hlo = (ec + Hlo) - t;
     const float hhi = (ec + Hhi) + t;
+    if constexpr (REC) {
+        const uint32_t sa = minmod_sel(in.a, (ar - al) * 0.5f, out.a);
+        const uint32_t sb = minmod_sel(in.b, (br - bl) * 0.5f, out.b);
+        const uint32_t sk = minmod_sel(in.K, ce, out.K);
+        const uint32_t ad = !(hhi > 0.0f) ? 2u : (!(hlo > 0.0f) ? 1u : 0u);
+        *code8 = (SGN < 0 ? (sa | (sb << 2)) : (sb | (sa << 2))) | (sk << 4) | (ad << 6);
+    }
     bool adj = false;
     if constexpr (SGN < 0) {
         if (!(hlo > 0.0f)) { Kv = (-dfb) + K.two_g * (ec + Hlo); adj = true; }
@@ -224,10 +235,12 @@ __device__ __forceinline__ Carry slope_bwd(float el, float ec, float al, float a
 // at least flux_eps), the same operations run without the selects of the
 // dry and zero-flux cases (and, with FAST, the wave speeds take the Newton
 // square root: both depths are above the cutoff there).
-template <bool FAST>
+// REC: also the face's branch record (which states give a+ and a-, and
+// the zero-flux bit), as cu_flux<true>.
+template <bool FAST, bool REC = false>
 __device__ __forceinline__ void face(float em, float am, float bm, float hSam, float hSbm, float dm,
                                      float ep, float ap, float bp, float hSap, float hSbp, float dp,
-                                     const Consts& K, float& F0, float& F1, float& F2) {
+                                     const Consts& K, float& F0, float& F1, float& F2, uint32_t* code = nullptr) {
     const float hp = fmaxf(0.0f, dp);
     const float hm = fmaxf(0.0f, dm);
     if (__all(!(ep == kDryFlag || em == kDryFlag) && hp > K.cutoff && hm > K.cutoff)) {
@@ -242,6 +255,12 @@ __device__ __forceinline__ void face(float em, float am, float bm, float hSam, f
         const float amin = fminf(fminf(um - c_m, up - c_p), 0.0f);
         const float amax = fmaxf(fmaxf(um + c_m, up + c_p), 0.0f);
         const float den = amax - amin;
+        if constexpr (REC) {
+            const float b1 = um + c_m, b2 = up + c_p, a1 = um - c_m, a2 = up - c_p;
+            const uint32_t cp = (fmaxf(b1, b2) > 0.0f) ? ((b1 >= b2) ? 1u : 2u) : 0u;
+            const uint32_t cm = (fminf(a1, a2) < 0.0f) ? ((a1 <= a2) ? 1u : 2u) : 0u;
+            *code = cp | (cm << 2) | ((fabsf(den) < K.flux_eps) ? 16u : 0u);
+        }
         const float apam = amax * amin;
         const float rden = rcp_refined(den);
         F0 = div_r((amax * Fmx - amin * Fpx) + apam * (hp - hm), den, rden);
@@ -254,16 +273,19 @@ __device__ __forceinline__ void face(float em, float am, float bm, float hSam, f
         return;
     }
     F0 = 0.0f; F1 = 0.0f; F2 = 0.0f;
+    if constexpr (REC) *code = 0u;
     if (!(ep == kDryFlag || em == kDryFlag))
-        cu_flux(hm, am + hSam, bm + hSbm, hp, ap - hSap, bp - hSbp, K, F0, F1, F2);
+        cu_flux<REC>(hm, am + hSam, bm + hSbm, hp, ap - hSap, bp - hSbp, K, F0, F1, F2, code);
 }
 
 // One strip: padded columns sx*60 .. sx*60+63 (lanes), output rows
 // 2 + sy*bh .. min(2 + (sy+1)*bh, ny+2) - 1.  WGRID: the wind stress comes
 // from A.wind_grid (else the uniform A.wind_x, A.wind_y); a separate
 // instantiation, because the per-lane interpolation state and lookups would
-// otherwise hold registers in every launch.
-template <bool FAST, bool WGRID>
+// otherwise hold registers in every launch.  REC: also write the branch
+// record (A.tape_*) of the reverse pass, the same words the tile kernel's
+// recording variant writes.
+template <bool FAST, bool WGRID, bool REC = false>
 __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy, int bh) {
     const int lane = threadIdx.x & (kLanes - 1);
     const int nx = A.nx, ny = A.ny;
@@ -297,6 +319,20 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
     const int addr_l = ((lane - 1) & (kLanes - 1)) << 2;
     const rsrc_t r_ein = plane(A.eta_in), r_uin = plane(A.hu_in), r_vin = plane(A.hv_in);
     const rsrc_t r_hi = plane(A.Hi);
+    // branch record: each cell's words are written by one strip -- interior
+    // columns by lanes 2..61, the ghost columns by the first and last strip
+    // (ghost corners are never written, as in the tile kernel)
+    const bool col_in = C >= 2 && C <= nx + 1;
+    const bool rec_lane = REC && (col_in ? (lane >= 2 && lane <= 61)
+                                         : (C <= 1 ? sx == 0 : (C <= nx + 3 && sx == A.tiles_x - 1)));
+    auto rec_store = [&](int R, float u, float v) {
+        const uint32_t ro = (uint32_t)R * rowb;
+        st(plane(A.tape_u), ocl, ro, u);
+        st(plane(A.tape_v), ocl, ro, v);
+    };
+    auto rec_code = [&](int R, uint32_t code) {
+        st(plane(reinterpret_cast<const float*>(A.tape_code)), ocl, (uint32_t)R * rowb, __uint_as_float(code));
+    };
 
     const float fc = A.f;
     const float dt = A.dt_dev ? *A.dt_dev : A.dt;
@@ -355,26 +391,38 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
     float hUy1, hVy1, S1, dhi1;
     Carry ycar;
     float gsx, gsy, gsz;
+    uint32_t yc1 = 0u;                               // record byte of row j0's y slope centre
     {
         float Uy0, Vy0, S0, dlo0, dhi0, Uy1, Vy1, dlo1;
+        uint32_t yc0 = 0u, gc0 = 0u;
         const Carry bwd = slope_bwd<+1>(c0.E, c1.E, c0.V, c1.V, c0.U, c1.U, row_flip(j0 - 2), row_flip(j0 - 1),
                                         K.udy, K, fc);
         Carry fwd;
         float fr;
         slope_fwd<+1>(c1.E, c2.E, c1.V, c2.V, c1.U, c2.U, row_flip(j0 - 1), row_flip(j0), K.udy, K, fc, fwd, fr);
-        slope_rest<+1, FAST>(c0.E, c1.E, c2.E, c0.V, c2.V, c0.U, c1.U, c2.U, Hs_m1, Hs_0, row_flip(j0 - 1),
-                            wS && j0 - 1 < 2, K.dy, K.udy, K, fc, bwd, fwd, fr, Vy0, Uy0, S0, dlo0, dhi0);
+        slope_rest<+1, FAST, REC>(c0.E, c1.E, c2.E, c0.V, c2.V, c0.U, c1.U, c2.U, Hs_m1, Hs_0, row_flip(j0 - 1),
+                                  wS && j0 - 1 < 2, K.dy, K.udy, K, fc, bwd, fwd, fr, Vy0, Uy0, S0, dlo0, dhi0, &yc0);
         Carry fwd1;
         slope_fwd<+1>(c2.E, c3.E, c2.V, c3.V, c2.U, c3.U, row_flip(j0), row_flip(j0 + 1), K.udy, K, fc, fwd1, fr);
-        slope_rest<+1, FAST>(c1.E, c2.E, c3.E, c1.V, c3.V, c1.U, c2.U, c3.U, Hs_0, Hs_1, row_flip(j0),
-                            wS && j0 < 2, K.dy, K.udy, K, fc, fwd, fwd1, fr, Vy1, Uy1, S1, dlo1, dhi1);
+        slope_rest<+1, FAST, REC>(c1.E, c2.E, c3.E, c1.V, c3.V, c1.U, c2.U, c3.U, Hs_0, Hs_1, row_flip(j0),
+                                  wS && j0 < 2, K.dy, K.udy, K, fc, fwd, fwd1, fr, Vy1, Uy1, S1, dlo1, dhi1, &yc1);
         ycar = fwd1;
         hUy1 = 0.5f * Uy1;
         hVy1 = 0.5f * Vy1;
         // flux through (j0-1 | j0): m = row j0-1 (c1), p = row j0 (c2)
         float a, b, c;
-        face<FAST>(c1.E, c1.V, c1.U, 0.5f * Vy0, 0.5f * Uy0, dhi0, c2.E, c2.V, c2.U, hVy1, hUy1, dlo1, K, a, b, c);
+        face<FAST, REC>(c1.E, c1.V, c1.U, 0.5f * Vy0, 0.5f * Uy0, dhi0, c2.E, c2.V, c2.U, hVy1, hUy1, dlo1, K, a, b,
+                        c, &gc0);
         gsx = a; gsy = c; gsz = b;
+        // the first strip of a column records ghost rows 0 and 1 (interior
+        // columns; row 1's y slope centre and its north face (1 | 2))
+        if (REC && sy == 0 && rec_lane && col_in) {
+            rec_store(0, c0.U, c0.V);
+            rec_code(0, 0u);
+            rec_store(1, c1.U, c1.V);
+            st(plane(A.tape_s), ocl, rowb, S0);
+            rec_code(1, (yc0 << kCodeY) | (gc0 << kCodeFy));
+        }
     }
     // window: (cj, cj1) = rows j, j+1; (hUyj, hVyj, Sj, dhij) = half y slopes,
     // depth term and north-face depth of row j; ycar = forward candidates of
@@ -395,6 +443,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
     const rsrc_t r_eqn = plane(A.eta_qn), r_uqn = plane(A.hu_qn), r_vqn = plane(A.hv_qn);
     const rsrc_t r_eo = plane(A.eta_out), r_uo = plane(A.hu_out), r_vo = plane(A.hv_out);
 
+    uint32_t ycj = yc1;                               // record byte of row j's y slope centre
     for (int j = j0; j < j1; ++j) {
         // loads for row j+2 (conversion) and Q^n of row j (second stages)
         float e2, u2, v2, hcj3, hnj3;
@@ -411,6 +460,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
         const float Hw = 0.5f * (hcj + hcj1);            // west face depth of the cell
         const float He = 0.5f * (hnj + hnj1);            // east face depth
         float hUx, hVx, T, dlox, dhix;
+        uint32_t xc = 0u, fce = 0u, ycn = 0u, gcn = 0u;   // record bytes of row j (x, east face, north face), row j+1 (y)
         {
             Carry fwd, bwd;
             float fr, Ux, Vx;
@@ -419,15 +469,15 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
             bwd.b = lane_get(fwd.b, addr_l);
             bwd.K = lane_get(fwd.K, addr_l);
             bwd.fb = lane_get(fwd.fb, addr_l);
-            slope_rest<-1, FAST>(el, cj.E, er, ul, ur, vl, cj.V, vr, Hw, He, xfc, xfa, K.dx, K.vdx, K, fc, bwd, fwd,
-                                fr, Ux, Vx, T, dlox, dhix);
+            slope_rest<-1, FAST, REC>(el, cj.E, er, ul, ur, vl, cj.V, vr, Hw, He, xfc, xfa, K.dx, K.vdx, K, fc, bwd,
+                                      fwd, fr, Ux, Vx, T, dlox, dhix, &xc);
             hUx = 0.5f * Ux;
             hVx = 0.5f * Vx;
         }
         // ---- east face (C | C+1) ----
         const float hUxr = lane_get(hUx, addr_r), hVxr = lane_get(hVx, addr_r), dpr = lane_get(dlox, addr_r);
         float fex, fey, fez;
-        face<FAST>(cj.E, cj.U, cj.V, hUx, hVx, dhix, er, ur, vr, hUxr, hVxr, dpr, K, fex, fey, fez);
+        face<FAST, REC>(cj.E, cj.U, cj.V, hUx, hVx, dhix, er, ur, vr, hUxr, hVxr, dpr, K, fex, fey, fez, &fce);
 
         // ---- conversion of row j+2, y slopes of row j+1, north face ----
         const Cell cj2 = convert(e2, u2, v2, hcj2 + hcj3, hcj2, hcj3, hnj2, hnj3, K);
@@ -438,8 +488,9 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
             float fr, Uyn, Vyn;
             slope_fwd<+1>(cj1.E, cj2.E, cj1.V, cj2.V, cj1.U, cj2.U, row_flip(j + 1), row_flip(j + 2), K.udy, K,
                           fc, fwd, fr);
-            slope_rest<+1, FAST>(cj.E, cj1.E, cj2.E, cj.V, cj2.V, cj.U, cj1.U, cj2.U, Hnj, Hnj1, row_flip(j + 1),
-                                wS && j + 1 < 2, K.dy, K.udy, K, fc, ycar, fwd, fr, Vyn, Uyn, Sn, dlon, dhin);
+            slope_rest<+1, FAST, REC>(cj.E, cj1.E, cj2.E, cj.V, cj2.V, cj.U, cj1.U, cj2.U, Hnj, Hnj1, row_flip(j + 1),
+                                      wS && j + 1 < 2, K.dy, K.udy, K, fc, ycar, fwd, fr, Vyn, Uyn, Sn, dlon, dhin,
+                                      &ycn);
             ycar = fwd;
             hUyn = 0.5f * Uyn;
             hVyn = 0.5f * Vyn;
@@ -447,8 +498,24 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
         float gnx, gny, gnz;
         {
             float a, b, c;
-            face<FAST>(cj.E, cj.V, cj.U, hVyj, hUyj, dhij, cj1.E, cj1.V, cj1.U, hVyn, hUyn, dlon, K, a, b, c);
+            face<FAST, REC>(cj.E, cj.V, cj.U, hVyj, hUyj, dhij, cj1.E, cj1.V, cj1.U, hVyn, hUyn, dlon, K, a, b, c,
+                            &gcn);
             gnx = a; gny = c; gnz = b;
+        }
+        // ---- branch record of row j (an interior row) ----
+        if (REC && rec_lane) {
+            rec_store(j, cj.U, cj.V);
+            uint32_t code = 0u;
+            if (C >= 1 && C <= nx + 2) {
+                st(plane(A.tape_t), ocl, rj, T);
+                code |= xc << kCodeX;
+                if (C <= nx + 1) code |= fce << kCodeFx;
+            }
+            if (col_in) {
+                st(plane(A.tape_s), ocl, rj, Sj);
+                code |= (ycj << kCodeY) | (gcn << kCodeFy);
+            }
+            rec_code(j, code);
         }
         // ---- west face from the left lane ----
         const float fwx = lane_get(fex, addr_l), fwy = lane_get(fey, addr_l), fwz = lane_get(fez, addr_l);
@@ -576,11 +643,21 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
         }
 
         // ---- slide the window one row north ----
+        ycj = ycn;
         gsx = gnx; gsy = gny; gsz = gnz;
         cj = cj1; cj1 = cj2;
         hUyj = hUyn; hVyj = hVyn; Sj = Sn; dhij = dhin;
         hcj = hcj1; hnj = hnj1; hcj1 = hcj2; hnj1 = hnj2; hcj2 = hcj3; hnj2 = hnj3;
         Hsj = Hnj; Hnj = Hnj1;
+    }
+    // the last strip of a column records ghost rows ny+2 (its y slope
+    // centre) and ny+3 (interior columns)
+    if (REC && j1 == ny + 2 && rec_lane && col_in) {
+        rec_store(ny + 2, cj.U, cj.V);
+        st(plane(A.tape_s), ocl, (uint32_t)(ny + 2) * rowb, Sj);
+        rec_code(ny + 2, ycj << kCodeY);
+        rec_store(ny + 3, cj1.U, cj1.V);
+        rec_code(ny + 3, 0u);
     }
 }
 

@@ -297,6 +297,43 @@ __global__ __launch_bounds__(kEnsTapeNT) __attribute__((amdgpu_waves_per_eu(8)))
         cdklm_tile<kEnsTapeBW, kEnsTapeBH, kEnsTapeNT, kModeZero, true>(A, tx, ty, S);
 }
 
+// column-marching recording substep over every member's strips (the strip
+// map of the ensemble's marching configuration)
+template <bool FAST>
+__global__ __launch_bounds__(256) void k_ens_march_tape(EnsTapeLaunch L, int bh, int n_strips) {
+    const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
+    if (wave >= n_strips) return;
+    const int m = L.tile_member[wave];
+    const EnsMember& M = L.members[m];
+    SubstepArgs A;
+    const long long pl = M.plane;
+    const float* in = L.in.base + ens_off(M, L.in);
+    const float* qn = L.qn.base + ens_off(M, L.qn);
+    float* out = const_cast<float*>(L.out.base) + ens_off(M, L.out);
+    float* tp = const_cast<float*>(L.tape.base) + ens_off(M, L.tape);
+    A.eta_in = in; A.hu_in = in + pl; A.hv_in = in + 2 * pl;
+    A.eta_qn = qn; A.hu_qn = qn + pl; A.hv_qn = qn + 2 * pl;
+    A.eta_out = out; A.hu_out = out + pl; A.hv_out = out + 2 * pl;
+    A.tape_u = tp; A.tape_v = tp + pl; A.tape_t = tp + 2 * pl; A.tape_s = tp + 3 * pl;
+    A.tape_code = reinterpret_cast<uint32_t*>(tp + 4 * pl);
+    A.Hi = L.Hi + M.hi_off;
+    A.f_field = nullptr; A.nx_field = nullptr; A.ny_field = nullptr;
+    A.dt_dev = L.dt + m;
+    A.nx = M.nx; A.ny = M.ny; A.pitch = M.pitch; A.hi_pitch = M.hi_pitch;
+    A.dx = M.dx; A.dy = M.dy; A.g = M.g; A.r = M.r; A.theta = M.theta;
+    A.desing_eps = M.desing; A.flux_eps = M.flux_eps; A.cutoff = M.cutoff; A.land = M.land;
+    A.f = M.f; A.north_x = M.north_x; A.north_y = M.north_y;
+    A.wind_x = M.wind_x; A.wind_y = M.wind_y;
+    A.wind_grid = nullptr; A.wind_wx = 0; A.wind_wy = 0; A.wind_t = 0.0f;   // members: uniform wind
+    A.dt = 0.0f;
+    A.set_flags(M.bc_n, M.bc_e, M.bc_s, M.bc_w, 2, L.rk_step);
+    A.vdx = M.vdx; A.udy = M.udy; A.r_two_g = M.r_two_g; A.r_dx = M.r_dx; A.r_dy = M.r_dy;
+    A.tiles_x = M.tiles_x; A.tiles_y = 0;
+    const int lt = wave - M.tile_begin;
+    const int sy = lt / M.tiles_x, sx = lt - sy * M.tiles_x;
+    march::march_strip<FAST, false, true>(A, sx, sy, bh);
+}
+
 struct EnsAdjLaunch {
     const EnsMember* members;
     const int* tile_member;
@@ -906,7 +943,15 @@ struct EnsSegment {
         EnsTapeLaunch L;
         L.members = E->d_members; L.tile_member = E->d_ftile_member;
         L.in = in; L.qn = qn; L.out = out; L.tape = tape; L.Hi = Hi; L.dt = dt; L.rk_step = rk_step;
-        hipLaunchKernelGGL(k_ens_substep_tape, dim3(E->n_ftiles), dim3(kEnsTapeNT), 0, s, L);
+        if (E->cfg >= kFirstMarchConfig && E->mode == kModeZero) {
+            // the ensemble's marching configuration records through its strips
+            L.tile_member = E->d_tile_member;
+            const dim3 grid((E->n_tiles + 3) / 4), block(256);
+            if (E->fast_arith) hipLaunchKernelGGL((k_ens_march_tape<true>), grid, block, 0, s, L, E->bh, E->n_tiles);
+            else hipLaunchKernelGGL((k_ens_march_tape<false>), grid, block, 0, s, L, E->bh, E->n_tiles);
+        } else {
+            hipLaunchKernelGGL(k_ens_substep_tape, dim3(E->n_ftiles), dim3(kEnsTapeNT), 0, s, L);
+        }
         return check_launch("ensemble recording substep launch");
     }
     // w_fold / x_fold: w / x is an unfolded output of another substep

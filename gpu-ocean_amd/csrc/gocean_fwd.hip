@@ -299,18 +299,51 @@ static void launch_tape_t(SubstepArgs A, hipStream_t s) {
     hipLaunchKernelGGL((k_substep_tape<BW, BH, NT>), dim3(A.tiles_x * A.tiles_y), dim3(NT), 0, s, A);
 }
 
+// column-marching recording variant (uniform wind only: the adjoint's scope)
+template <bool FAST>
+__global__ __launch_bounds__(256) void k_march_tape(SubstepArgs A) {
+    const int wave = xcd_band(blockIdx.x, gridDim.x) * 4 + march::wave_in_group();
+    if (wave >= A.tiles_x * A.tiles_y) return;
+    const int sy = wave / A.tiles_x, sx = wave - sy * A.tiles_x;
+    march::march_strip<FAST, false, true>(A, sx, sy, A.march_bh);
+}
+
+// bh > 0: fixed rows per strip; 0: balanced by march_rows
+static void launch_march_tape(SubstepArgs A, int bh, hipStream_t s) {
+    A.tiles_x = (A.nx + march::kOut - 1) / march::kOut;
+    const bool fast = A.fast_arith();
+    if (bh <= 0) {
+        static int resident[2] = {0, 0};
+        const void* k = fast ? (const void*)k_march_tape<true> : (const void*)k_march_tape<false>;
+        if (!resident[fast]) resident[fast] = resident_waves(k);
+        bh = march_rows(&A.tiles_x, &A.ny, 1, resident[fast]);
+    }
+    A.march_bh = bh;
+    A.tiles_y = (A.ny + bh - 1) / bh;
+    const int waves = A.tiles_x * A.tiles_y;
+    if (fast) hipLaunchKernelGGL((k_march_tape<true>), dim3((waves + 3) / 4), dim3(256), 0, s, A);
+    else hipLaunchKernelGGL((k_march_tape<false>), dim3((waves + 3) / 4), dim3(256), 0, s, A);
+}
+
 // tile configuration of the recording forward (process-wide; the record is
-// laid out per cell, so every configuration writes the same record)
-static int g_record_cfg = 5;
+// laid out per cell, so every configuration writes the same record); -1:
+// the column-marching kernel from a million cells on, 32x16/512 below
+static int g_record_cfg = -1;
 
 int launch_substep_tape(SubstepArgs A, hipStream_t s) {
-    switch (g_record_cfg) {
+    int cfg = g_record_cfg;
+    if (cfg < 0) cfg = default_tile_config((long long)A.nx * A.ny) >= kFirstMarchConfig ? kMarchAutoConfig : 5;
+    if (cfg >= kFirstMarchConfig && !march_fits(A.ny, A.pitch, A.hi_pitch)) cfg = 5;
+    switch (cfg) {
         case 0: launch_tape_t<32, 8, 256>(A, s); break;
         case 1: launch_tape_t<32, 16, 256>(A, s); break;
         case 2: launch_tape_t<64, 8, 256>(A, s); break;
         case 3: launch_tape_t<16, 16, 256>(A, s); break;
         case 4: launch_tape_t<64, 4, 256>(A, s); break;
         case 6: launch_tape_t<32, 14, 512>(A, s); break;
+        case 7: launch_march_tape(A, 8, s); break;
+        case 8: launch_march_tape(A, 16, s); break;
+        case 9: launch_march_tape(A, 0, s); break;
         default: launch_tape_t<32, 16, 512>(A, s); break;
     }
     return check_launch("recording substep launch");
@@ -548,7 +581,7 @@ int gocean_device_count(void) {
 
 int gocean_adjoint_set_record_tile_config(int cfg) {
     clear_error();
-    if (cfg < 0) cfg = 5;
+    if (cfg < 0) cfg = -1;
     if (cfg >= kNumTileConfigs)
         return set_error(GOCEAN_EINVAL, "tile configuration %d out of range [0,%d)", cfg, kNumTileConfigs);
     g_record_cfg = cfg;

@@ -156,11 +156,15 @@ int gocean_max_dt_static(const gocean_domain_t* d, const float* eta, const float
                          const float* hv, const float* Hm, float courant, float* out_dev,
                          void* stream);
 
-/* Tile configuration (index into gocean_tile_configs, -1: the default
- * 32x16/512) of the recording forward substep the adjoint entry points run
+/* Tile configuration (index into gocean_tile_configs; -1, the default: the
+ * column-marching kernel from a million cells on, 32x16/512 below) of the
+ * recording forward substep the adjoint entry points run
  * (gocean_adjoint_record / _segment).  Process-wide; every configuration
- * writes the same record.  No counterpart in the reference (the adjoint is
- * new); it lets a tuner pick the fastest recording tile, as for the forward. */
+ * writes the same record.  An ensemble records with its own configuration
+ * (gocean_ensemble_set_tile_config): through its strips when that is a
+ * column-marching one, else with 32x16/512 tiles.  No counterpart in the
+ * reference (the adjoint is new); it lets a tuner pick the fastest recording
+ * tile, as for the forward. */
 int gocean_adjoint_set_record_tile_config(int cfg);
 int gocean_adjoint_record_tile_config(void);
 
