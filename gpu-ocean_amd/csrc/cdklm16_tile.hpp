@@ -857,7 +857,8 @@ __device__ __forceinline__ void cdklm_tile(const SubstepArgs& A, int tile_x, int
         float st1 = 0.0f, st2 = 0.0f;
         if (hreg >= K.cutoff && ec != kDryFlag) {
             float X = A.wind_x, Y = A.wind_y;
-            if (A.wind_grid)
+            // the recording variant serves the adjoint, which takes uniform wind
+            if (!TAPE && A.wind_grid)
                 wind_grid_at(A.wind_grid, A.wind_wx, A.wind_wy, A.wind_t, wind_axis(C, nx, A.wind_wx),
                              wind_axis(R, ny, A.wind_wy), X, Y);
             const float RHxp = 0.5f * (S.Hi[rr][cc + 1] + S.Hi[rr + 1][cc + 1]);
