@@ -747,9 +747,11 @@ def measured_traffic(cells, tile):
 
 def measured_valu(cells, tile, launch_s):
     """VALU work of the substep launch from the committed counter summary
-    (SQ_INSTS_VALU per dispatch) and its issue rate against the non-packed
-    FP32 limit: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave instruction
-    = 614.4 G wave instructions/s (packed FP32 is off in this build)."""
+    (SQ_INSTS_VALU per dispatch) and its issue rate against the nominal
+    non-packed FP32 rate: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave
+    instruction = 614.4 G wave instructions/s (packed FP32 is off in this
+    build).  Moves, selects and bit operations of the kernel's mix do not
+    all take an FP32 issue slot, so the fraction can exceed 1."""
     d = _profile_json().get("tiles", {}).get(tile)
     if not d or d.get("cells") != cells or "valu_per_launch" not in d:
         return None
