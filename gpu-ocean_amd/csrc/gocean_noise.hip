@@ -73,6 +73,9 @@ struct NoiseLaunch {
     unsigned long long* seed;
     float* random;
     float* coarse;
+    // Philox counter advanced by the first kernel after the generator (the
+    // batched perturbation folds the advance into its SOAR launch), or NULL
+    unsigned long long* bump;
 };
 
 // The member a block works on.  Single domain: the table of one in the
@@ -185,6 +188,12 @@ __global__ __launch_bounds__(kRngNT) void k_noise_rng(NoiseLaunch L, int kind,
 
 __global__ void k_counter_bump(unsigned long long* counter) { *counter += 1ull; }
 
+// the counter advance of a launch that runs after the generator read it; at
+// the end of the kernel, so that the atomic orders none of its loads
+__device__ __forceinline__ void bump_counter(const NoiseLaunch& L) {
+    if (L.bump && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(L.bump, 1ull);
+}
+
 // ---------------------------------------------------------------------------
 // SOAR on the coarse grid: coarse(a) = q0 scale sum_{|o| <= 2} w(o) xi(a+o).
 // Coarse index a reads random index a+2 (+o) when the random field carries
@@ -223,6 +232,7 @@ __global__ __launch_bounds__(kSoarBW * kSoarBH) void k_noise_soar(NoiseLaunch L,
     const float v = ((M.q0 * q0_scale) * acc) * scale;
     float* c = L.coarse + M.coarse_off + (long long)ay * (M.cnx + 4) + ax;
     *c = additive ? *c + v : v;
+    bump_counter(L);   // thread 0 of block 0 (coarse cell (0, 0)) gets here
 }
 
 // ---------------------------------------------------------------------------
@@ -528,6 +538,7 @@ __global__ __launch_bounds__(kFuNT) void k_noise_fused1(NoiseLaunch L, float q0_
             L.hv[M.hv_off + st] = v0[r] + dhv;
         }
     }
+    bump_counter(L);   // thread 0 of block 0 (column 2) gets here
 }
 
 // ---------------------------------------------------------------------------
@@ -1000,9 +1011,10 @@ static NoiseLaunch batch_launch(const gocean_noise_batch_t* B) {
     return L;
 }
 
-int gocean_noise_batch_generate(gocean_noise_batch_t* B, uint64_t* seed, float* random, uint64_t key,
-                                uint64_t* counter, void* stream) {
-    clear_error();
+// bump: advance the Philox counter in its own launch (else the caller's next
+// launch does)
+static int batch_generate(gocean_noise_batch_t* B, uint64_t* seed, float* random, uint64_t key,
+                          uint64_t* counter, void* stream, bool bump) {
     if (!B || !random) return set_error(GOCEAN_EINVAL, "batch/random is NULL");
     const bool lcg = B->kind <= GOCEAN_RNG_LCG_UNIFORM;
     if (lcg && !seed) return set_error(GOCEAN_EINVAL, "LCG generator needs the seed buffer");
@@ -1014,8 +1026,14 @@ int gocean_noise_batch_generate(gocean_noise_batch_t* B, uint64_t* seed, float* 
     const hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_noise_rng<true>, dim3(B->n_rng), dim3(kRngNT), 0, s, L, B->kind,
                        (const unsigned long long*)counter, (unsigned long long)key);
-    if (!lcg) hipLaunchKernelGGL(k_counter_bump, dim3(1), dim3(1), 0, s, (unsigned long long*)counter);
+    if (!lcg && bump) hipLaunchKernelGGL(k_counter_bump, dim3(1), dim3(1), 0, s, (unsigned long long*)counter);
     return check_launch("k_noise_rng (batch)");
+}
+
+int gocean_noise_batch_generate(gocean_noise_batch_t* B, uint64_t* seed, float* random, uint64_t key,
+                                uint64_t* counter, void* stream) {
+    clear_error();
+    return batch_generate(B, seed, random, key, counter, stream, true);
 }
 
 int gocean_noise_batch_perturb(gocean_noise_batch_t* B, float* state, const float* Hi,
@@ -1025,10 +1043,12 @@ int gocean_noise_batch_perturb(gocean_noise_batch_t* B, float* state, const floa
     clear_error();
     if (!B || !state || !Hi || !random || !coarse) return set_error(GOCEAN_EINVAL, "a buffer is NULL");
     if (B->any_field && !fields) return set_error(GOCEAN_EINVAL, "members use fields but fields is NULL");
+    const bool philox = B->kind > GOCEAN_RNG_LCG_UNIFORM;
     if (update_random)
-        if (int e = gocean_noise_batch_generate(B, seed, random, key, counter, stream)) return e;
+        if (int e = batch_generate(B, seed, random, key, counter, stream, false)) return e;
     const hipStream_t s = (hipStream_t)stream;
     NoiseLaunch L = batch_launch(B);
+    if (update_random && philox) L.bump = (unsigned long long*)counter;   // advanced by the next launch
     L.random = random;
     L.coarse = coarse;
     L.eta = L.hu = L.hv = state;
