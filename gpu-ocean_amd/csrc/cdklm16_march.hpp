@@ -61,13 +61,36 @@ __device__ __forceinline__ float lane_get(float v, int addr) {
 using rsrc_t = const float*;
 
 __device__ __forceinline__ rsrc_t plane(const float* p) { return p; }
+// The row address, held in scalar registers and opaque to the optimizer:
+// without this, loop-invariant code motion regroups (plane + row) + lane
+// into (plane + lane) + row, which costs a 64-bit vector add per access
+// and two vector registers per plane.  The empty asm emits no instruction.
+using gchar = __attribute__((address_space(1))) char;     // global memory
+__device__ __forceinline__ gchar* srow(rsrc_t r, uint32_t row_off) {
+    gchar* row = (gchar*)(const_cast<char*>(reinterpret_cast<const char*>(r))) + row_off;
+    asm("" : "+s"(row));
+    return row;
+}
+// a wave-uniform flag as an opaque 32-bit scalar: held across the row loop
+// in one scalar register and tested with a scalar compare, where a bool
+// would be kept as a 64-bit lane mask (and spilled under register pressure)
+__device__ __forceinline__ int sflag(bool b) {
+    int v = __builtin_amdgcn_readfirstlane(b ? 1 : 0);   // float compares give a vector result
+    asm("" : "+s"(v));
+    return v;
+}
+
+// the lane's 32-bit offset, zero-extended at each access (scalar base plus
+// 32-bit vector offset is the load's addressing form)
+__device__ __forceinline__ uint32_t voff(uint32_t lane_off) {
+    asm("" : "+v"(lane_off));
+    return lane_off;
+}
 __device__ __forceinline__ float ld(rsrc_t r, uint32_t lane_off, uint32_t row_off) {
-    const char* row = reinterpret_cast<const char*>(r) + row_off;
-    return *reinterpret_cast<const float*>(row + lane_off);
+    return *(const __attribute__((address_space(1))) float*)(srow(r, row_off) + voff(lane_off));
 }
 __device__ __forceinline__ void st(rsrc_t r, uint32_t lane_off, uint32_t row_off, float v) {
-    char* row = const_cast<char*>(reinterpret_cast<const char*>(r)) + row_off;
-    *reinterpret_cast<float*>(row + lane_off) = v;
+    *(__attribute__((address_space(1))) float*)(srow(r, row_off) + voff(lane_off)) = v;
 }
 
 // corner depth with land flagged (CDKLM16_kernel.cu:509-525)
@@ -295,9 +318,9 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
     const int Cl = min(C, nx + 3);                   // load column (clamped)
     const int j0 = 2 + sy * bh;
     const int j1 = min(j0 + bh, ny + 2);
-    const bool wN = A.bc_n() == kWall, wS = A.bc_s() == kWall;
+    const int wN = sflag(A.bc_n() == kWall), wS = sflag(A.bc_s() == kWall);
     const bool wE = A.bc_e() == kWall, wW = A.bc_w() == kWall;
-    const bool pNS = A.bc_n() == kPeriodic, pEW = A.bc_e() == kPeriodic;
+    const int pNS = sflag(A.bc_n() == kPeriodic), pEW = sflag(A.bc_e() == kPeriodic);
 
     // the lane's source column (wall mirror with u negated, or periodic wrap)
     int sc = Cl;
@@ -432,7 +455,7 @@ __device__ __forceinline__ void march_strip(const SubstepArgs& A, int sx, int sy
     float hUyj = hUy1, hVyj = hVy1, Sj = S1, dhij = dhi1;
     float hcj = hcB, hnj = hnB, hcj1 = hcC, hnj1 = hnC, hcj2 = hcD, hnj2 = hnD;
     float Hsj = Hs_0, Hnj = Hs_1;
-    const bool friction = A.r > 0.0f;
+    const int friction = sflag(A.r > 0.0f);
     // gridded wind: the lane's column weights, fixed for the strip
     constexpr bool wgrid = WGRID;
     WindAxis wax;
