@@ -660,20 +660,20 @@ int gocean_ensemble_create(const gocean_member_t* members, int n, int tile_confi
         }
     }
     E->n_tiles = (int)tile_member.size();
-    hipGetDevice(&E->device);
+    (void)hipGetDevice(&E->device);
     if (hipMalloc(&E->d_members, sizeof(EnsMember) * n) != hipSuccess ||
         hipMalloc(&E->d_tile_member, sizeof(int) * E->n_tiles) != hipSuccess) {
         (void)hipGetLastError();
-        hipFree(E->d_members);
-        hipFree(E->d_tile_member);
+        (void)hipFree(E->d_members);
+        (void)hipFree(E->d_tile_member);
         delete E;
         return set_error(GOCEAN_ENOMEM, "device allocation of the member table failed");
     }
-    hipMemcpy(E->d_members, E->members.data(), sizeof(EnsMember) * n, hipMemcpyHostToDevice);
-    hipMemcpy(E->d_tile_member, tile_member.data(), sizeof(int) * E->n_tiles, hipMemcpyHostToDevice);
+    (void)hipMemcpy(E->d_members, E->members.data(), sizeof(EnsMember) * n, hipMemcpyHostToDevice);
+    (void)hipMemcpy(E->d_tile_member, tile_member.data(), sizeof(int) * E->n_tiles, hipMemcpyHostToDevice);
     if (int e = check_launch("ensemble table upload")) {
-        hipFree(E->d_members);
-        hipFree(E->d_tile_member);
+        (void)hipFree(E->d_members);
+        (void)hipFree(E->d_tile_member);
         delete E;
         return e;
     }
@@ -710,14 +710,14 @@ int gocean_ensemble_set_tile_config(gocean_ensemble_t* E, int tile_config) {
         (void)hipGetLastError();
         return set_error(GOCEAN_ENOMEM, "device allocation of the tile map failed");
     }
-    hipMemcpy(d_map, tile_member.data(), sizeof(int) * tile_member.size(), hipMemcpyHostToDevice);
-    hipMemcpy(E->d_members, E->members.data(), sizeof(EnsMember) * E->n, hipMemcpyHostToDevice);
-    hipFree(E->d_tile_member);
+    (void)hipMemcpy(d_map, tile_member.data(), sizeof(int) * tile_member.size(), hipMemcpyHostToDevice);
+    (void)hipMemcpy(E->d_members, E->members.data(), sizeof(EnsMember) * E->n, hipMemcpyHostToDevice);
+    (void)hipFree(E->d_tile_member);
     E->d_tile_member = d_map;
     E->n_tiles = (int)tile_member.size();
     // a captured graph holds the old launch shape: capture again
-    if (E->graph_exec) hipGraphExecDestroy(E->graph_exec);
-    if (E->graph) hipGraphDestroy(E->graph);
+    if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
+    if (E->graph) (void)hipGraphDestroy(E->graph);
     E->graph_exec = nullptr;
     E->graph = nullptr;
     return check_launch("ensemble tile configuration");
@@ -726,14 +726,14 @@ int gocean_ensemble_set_tile_config(gocean_ensemble_t* E, int tile_config) {
 int gocean_ensemble_destroy(gocean_ensemble_t* E) {
     clear_error();
     if (!E) return 0;
-    if (E->graph_exec) hipGraphExecDestroy(E->graph_exec);
-    if (E->graph) hipGraphDestroy(E->graph);
-    if (E->capture_stream) hipStreamDestroy(E->capture_stream);
-    hipFree(E->d_members);
-    hipFree(E->d_tile_member);
-    hipFree(E->d_ftile_member);
-    hipFree(E->d_atile_member);
-    hipFree(E->d_astrip_member);
+    if (E->graph_exec) (void)hipGraphExecDestroy(E->graph_exec);
+    if (E->graph) (void)hipGraphDestroy(E->graph);
+    if (E->capture_stream) (void)hipStreamDestroy(E->capture_stream);
+    (void)hipFree(E->d_members);
+    (void)hipFree(E->d_tile_member);
+    (void)hipFree(E->d_ftile_member);
+    (void)hipFree(E->d_atile_member);
+    (void)hipFree(E->d_astrip_member);
     delete E;
     return 0;
 }
@@ -808,8 +808,8 @@ int gocean_ensemble_capture(gocean_ensemble_t* E, int n_steps, float* state0,
     if (!E || !state0 || !state1 || !Hi || !dt) return set_error(GOCEAN_EINVAL, "NULL argument");
     if (rk_order < 1 || rk_order > 3 || (rk_order == 3 && !state2) || (rk_order == 1 && n_steps % 2))
         return set_error(GOCEAN_EINVAL, "invalid rk_order / buffers / step count");
-    if (E->graph_exec) { hipGraphExecDestroy(E->graph_exec); E->graph_exec = nullptr; }
-    if (E->graph) { hipGraphDestroy(E->graph); E->graph = nullptr; }
+    if (E->graph_exec) { (void)hipGraphExecDestroy(E->graph_exec); E->graph_exec = nullptr; }
+    if (E->graph) { (void)hipGraphDestroy(E->graph); E->graph = nullptr; }
     if (!E->capture_stream && hipStreamCreateWithFlags(&E->capture_stream, hipStreamNonBlocking) != hipSuccess)
         return check_launch("capture stream");
     if (hipStreamBeginCapture(E->capture_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -818,7 +818,7 @@ int gocean_ensemble_capture(gocean_ensemble_t* E, int n_steps, float* state0,
                                E->capture_stream);
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(E->capture_stream, &g);
-    if (e) { if (g) hipGraphDestroy(g); return e; }
+    if (e) { if (g) (void)hipGraphDestroy(g); return e; }
     if (ee != hipSuccess) return set_error((int)ee, "end capture: %s", hipGetErrorString(ee));
     E->graph = g;
     const hipError_t ei = hipGraphInstantiate(&E->graph_exec, g, nullptr, nullptr, 0);
@@ -891,9 +891,9 @@ long long gocean_ensemble_adjoint_setup(gocean_ensemble_t* E, int max_steps) {
         M.astrip_begin = (int)astrips.size();
         for (int t = 0; t < snx[k] * ((sny[k] + abh - 1) / abh); ++t) astrips.push_back(k);
     }
-    hipFree(E->d_ftile_member);
-    hipFree(E->d_atile_member);
-    hipFree(E->d_astrip_member);
+    (void)hipFree(E->d_ftile_member);
+    (void)hipFree(E->d_atile_member);
+    (void)hipFree(E->d_astrip_member);
     E->d_ftile_member = nullptr;
     E->d_atile_member = nullptr;
     E->d_astrip_member = nullptr;
@@ -903,14 +903,14 @@ long long gocean_ensemble_adjoint_setup(gocean_ensemble_t* E, int max_steps) {
         (void)hipGetLastError();
         return set_error(GOCEAN_ENOMEM, "device allocation of the adjoint tile maps failed");
     }
-    hipMemcpy(E->d_ftile_member, ftiles.data(), sizeof(int) * ftiles.size(), hipMemcpyHostToDevice);
-    hipMemcpy(E->d_atile_member, atiles.data(), sizeof(int) * atiles.size(), hipMemcpyHostToDevice);
-    hipMemcpy(E->d_astrip_member, astrips.data(), sizeof(int) * astrips.size(), hipMemcpyHostToDevice);
+    (void)hipMemcpy(E->d_ftile_member, ftiles.data(), sizeof(int) * ftiles.size(), hipMemcpyHostToDevice);
+    (void)hipMemcpy(E->d_atile_member, atiles.data(), sizeof(int) * atiles.size(), hipMemcpyHostToDevice);
+    (void)hipMemcpy(E->d_astrip_member, astrips.data(), sizeof(int) * astrips.size(), hipMemcpyHostToDevice);
     E->n_astrips = (int)astrips.size();
     E->adj_bh = abh;
     E->adj_cells = cells;
     E->adj_march_fits = fits;
-    hipMemcpy(E->d_members, E->members.data(), sizeof(EnsMember) * E->n, hipMemcpyHostToDevice);
+    (void)hipMemcpy(E->d_members, E->members.data(), sizeof(EnsMember) * E->n, hipMemcpyHostToDevice);
     if (int e = check_launch("ensemble adjoint setup")) return e;
     E->adj_kmax = max_steps;
     E->n_ftiles = (int)ftiles.size();
